@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark: plain FoF (binning + linking + relabel) on one MI355X.
+
+Workload (default): 134,217,728 particles, float64 (3, N), L_Box = 512,
+50 % in 250,000 NFW clumps (c = 8, Pareto(0.9) counts in [20, 2e6],
+r_vir = 0.6 (n/100)^(1/3)), 50 % uniform; linking_length 0.2,
+np_halo_min 10, bound_check False. Positions are generated on the host
+(seeded), copied to HBM once, and every timed step is one full fof() call
+through the C-ABI on device-resident input and outputs.
+
+Prints one JSON line (see the keys below). The CPU baseline is the oracle
+(the C restatement of the reference's FoF) timed on a bounded sample: the
+same generator at 2,097,152 particles in a 128^3 box (same density), with
+the reference's thread rule (2 linking threads for one region).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FoF Mparticles/s; bound halos/s batched vs plain at stated err; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=134_217_728, help="particles")
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=2_097_152, help="CPU baseline sample size")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(args, msg):
+    if not args.quiet:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def box_for(n):
+    # same mean density (1 particle per unit cell) as 2^27 particles in 512^3
+    return 512.0 * (n / 134_217_728) ** (1.0 / 3.0)
+
+
+def cpu_baseline(args):
+    from oracle import oracle_fof
+    from py_fof_amd.synth import clumpy_nfw
+    n = args.cpu_n
+    L = box_for(n)
+    xp = clumpy_nfw(n=n, L=L, seed=args.seed)
+    t0 = time.perf_counter()
+    oracle_fof(xp, [n], L, 0.2, 10, n_threads=2)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(n / dt / 1e6, 4), "unit": "Mparticles/s", "cores": 2, "kind": "port",
+        "sample": f"oracle FoF on the same generator at {n} particles, L={L:g} "
+                  f"(same density), seed {args.seed}, 2 linking threads as the reference's "
+                  f"rule for one region; {dt:.2f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from py_fof_amd import _native
+    from py_fof_amd.synth import clumpy_nfw
+
+    n = args.n
+    L = box_for(n)
+    log(args, f"generating {n} particles, L={L:g}")
+    t = time.time()
+    xp = clumpy_nfw(n=n, L=L, seed=args.seed)
+    log(args, f"generated in {time.time() - t:.1f} s")
+    dev = torch.device("cuda:0")
+    x_d = torch.from_numpy(xp.T).to(dev)  # (N, 3) row-major == (3, N) Fortran order
+    del xp
+    in_h = torch.empty(n, dtype=torch.int32, device=dev)
+    hid = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ctx = _native.Context(0)
+    np_halo = np.array([n], np.int32)
+
+    def step():
+        err = ctx.fof(x_d, None, None, np_halo, L, 0.2, 10, 0, 0, 0.1, 1.0, 0.0, 17.152, 0, n,
+                      256, 512.0, in_h, hid, None, None, _native.MEM_DEVICE)
+        if err:
+            raise RuntimeError(f"pyfof ierr={err}: {_native.last_error()}")
+
+    for i in range(args.warmup):
+        step()
+        log(args, f"warmup {i} done: phases ms {[round(v, 2) for v in ctx.timings()]}")
+    ref_in = in_h.clone()
+    torch.cuda.synchronize()
+    phases = np.zeros(6)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        phases += np.array(ctx.timings()[:6])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(args, f"timed {args.steps} steps in {t1 - t0:.3f} s")
+    ms = (t1 - t0) * 1e3 / args.steps
+    phases /= args.steps
+    st = ctx.stats()
+    same = bool(torch.equal(ref_in, in_h))
+    ngroups = st.get("nhalo", 0)
+    # compulsory HBM traffic of binning + linking: N*64 B + cells*12 B
+    alg_bytes = n * 64 + st["ncell"] * 12
+    t_bl = (phases[0] + phases[1] + phases[2]) * 1e-3
+    achieved = alg_bytes / t_bl / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(n / (ms * 1e-3) / 1e6, 3),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64/f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain FoF, 134,217,728 particles (NFW clumps 50% + uniform), "
+                        "L_Box 512, b 0.2, np_halo_min 10, bound_check False"
+                        if n == 134_217_728 else f"plain FoF, {n} particles, L_Box {L:g}",
+            "particles": n, "L_Box": L, "linking_length": 0.2, "np_halo_min": 10,
+            "parallelism": "single GPU", "input": "device-resident f64 (3,N)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "binning + linking (k_bbox..k_link), algorithmic bytes N*64 + cells*12",
+        },
+        "phases_ms": {k: round(float(v), 3) for k, v in zip(
+            ["bbox", "binning", "linking", "relabel", "catalog", "unbinding"], phases)},
+        "groups": ngroups,
+        "cells": st["ncell"],
+        "dense_cells": st["ndense"],
+        "in_halo": int(in_h.sum().item()),
+        "deterministic": same,
+    }
+    if not args.no_cpu:
+        log(args, "cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+/*
+ * C-ABI of libpyfof_hip.so -- the MI355X (gfx950) FoF + unbinding library.
+ *
+ * Plain pointers and sizes only. Error handling follows the reference's
+ * integer ierr convention (fof_fortran.f90:116-119, py_fof/__init__.py:338-348):
+ *   0 ok, 1 invalid input/configuration, 2 candidate span > s_15,
+ *   3 allocation failure, 4 non-finite values, 10 n_threads > 256,
+ *   20 HIP runtime failure, 21 size beyond the device layout.
+ * Nothing throws across this boundary; pyfof_last_error() gives the message.
+ */
+#ifndef PYFOF_H
+#define PYFOF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Where the particle and output arrays of a call live. */
+#define PYFOF_MEM_HOST 0   /* host memory: the library stages through HBM */
+#define PYFOF_MEM_DEVICE 1 /* HBM of the context's device: used in place */
+
+typedef struct pyfof_ctx pyfof_ctx;
+
+/*
+ * Replaces the f2py entry point fof_fortran.fof_fortran(...) that
+ * py_fof/__init__.py:331-336 calls (declared at fof_fortran.f90:10-58):
+ * same argument order and meaning, same ierr codes, caller-allocated outputs.
+ *   xp, vp      (3, total_np) f64, Fortran order (x0,y0,z0,x1,...); vp may be
+ *               NULL when bound_check == 0 (zero velocities).
+ *   mp          (total_np) f64; may be NULL (unit masses).
+ *   np_halo     (nhalo) int32 region sizes, always host memory.
+ *   in_halo, halo_id  (total_np) int32 outputs.
+ *   ang_mom     (3, nhalo) f64 Fortran order output; may be NULL to skip the
+ *               per-region angular momentum (it is then not computed).
+ *   mem_kind    PYFOF_MEM_HOST or PYFOF_MEM_DEVICE for xp/vp/mp and outputs.
+ * Runs on the calling thread's current HIP device with a per-device default
+ * context; calls on one device are serialised.
+ */
+int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const int32_t* np_halo,
+                  int32_t nhalo, double L_Box, double linking_length, int32_t np_halo_min,
+                  int32_t bound_check, int32_t verbose, double softening, double a_val,
+                  double b_val, double s_15_val, int32_t n_threads, int32_t total_np,
+                  int64_t ng_val, double ng_global_val, int32_t* in_halo, int32_t* halo_id,
+                  double* ang_mom, int32_t mem_kind);
+
+/* Context API: explicit device, own stream and reusable HBM workspace. */
+pyfof_ctx* pyfof_ctx_create(int32_t device);
+void pyfof_ctx_destroy(pyfof_ctx* ctx);
+
+/* Same as pyfof_fof on an explicit context. root_out (optional, total_np
+ * int32, same memory kind) receives for every particle the 0-based index of
+ * the first member, in the reference's cell-sorted (CRS) order, of its
+ * friends-of-friends component: the canonical label of the partition
+ * (fof_fortran.f90:452-457, atomic_dsu.c:31-35). -1 for skipped regions. */
+int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const double* mp,
+                      const int32_t* np_halo, int32_t nhalo, double L_Box, double linking_length,
+                      int32_t np_halo_min, int32_t bound_check, int32_t verbose, double softening,
+                      double a_val, double b_val, double s_15_val, int32_t n_threads,
+                      int32_t total_np, int64_t ng_val, double ng_global_val, int32_t* in_halo,
+                      int32_t* halo_id, double* ang_mom, int32_t* root_out, int32_t mem_kind);
+
+/* Context options. PYFOF_OPT_CATALOG (default 0): when non-zero, every call
+ * also groups halo members for pyfof_catalog_* (fof_with_info). */
+#define PYFOF_OPT_CATALOG 1
+int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value);
+
+/*
+ * Catalog of the last call on ctx, as fof_with_info builds it on the host
+ * (py_fof/__init__.py:481-507): halos in halo_id order, each with its member
+ * indices ascending. pyfof_catalog_size gives nhalo_found and the total
+ * member count; pyfof_catalog_copy fills offsets (nhalo_found + 1) and
+ * indices (total members), in memory of kind mem_kind.
+ */
+int32_t pyfof_catalog_size(pyfof_ctx* ctx, int32_t* nhalo_found, int64_t* n_members);
+int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, int32_t mem_kind);
+
+/* Device time in milliseconds of the phases of the last call on ctx
+ * (0: bbox+geometry, 1: binning, 2: linking, 3: relabel/outputs,
+ * 4: catalog+angular momentum, 5: unbinding), measured with HIP events on
+ * the context's stream. Returns the number of phases written. */
+int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
+
+/* Sizes of the last call on ctx: [0] particles, [1] reference cells,
+ * [2] dense cells given sub-cells, [3] halos (candidates), [4] members of
+ * halos. Returns the number of values written. */
+int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n);
+
+/* HIP stream (hipStream_t) the context launches on. */
+void* pyfof_ctx_stream(pyfof_ctx* ctx);
+
+/* Message of the last error on this thread ("" if none). */
+const char* pyfof_last_error(void);
+
+/* Library version string. */
+const char* pyfof_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYFOF_H */

@@ -1,0 +1,139 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes front end of the C oracle (liboracle.so).
+
+Mirrors the argument conventions of the reference's native entry point
+(fof_fortran.f90:10-58): positions/velocities (3, N) in Fortran order, f64.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class OracleGeom(ctypes.Structure):
+    _fields_ = [
+        ("origin", ctypes.c_double * 3),
+        ("cell", ctypes.c_double * 3),
+        ("bmin", ctypes.c_double * 3),
+        ("bmax", ctypes.c_double * 3),
+        ("n", ctypes.c_int32 * 3),
+        ("periodic", ctypes.c_int32 * 3),
+        ("is_periodic", ctypes.c_int32),
+    ]
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load_oracle():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build_oracle()
+    lib = ctypes.CDLL(_LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.oracle_fof.restype = ctypes.c_int32
+    lib.oracle_fof.argtypes = [dp, dp, dp, ip, ctypes.c_int32, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_int32, ip, ip, dp, ip]
+    lib.oracle_fof_candidates.restype = ctypes.c_int32
+    lib.oracle_fof_candidates.argtypes = [dp, ip, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip]
+    lib.oracle_geometry_of.restype = ctypes.c_int32
+    lib.oracle_geometry_of.argtypes = [dp, ctypes.c_int32, ctypes.c_double,
+                                       ctypes.c_double, ctypes.POINTER(OracleGeom)]
+    _lib = lib
+    return lib
+
+
+def oracle_available():
+    try:
+        load_oracle()
+        return True
+    except Exception:
+        return False
+
+
+def _f64_fortran(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def oracle_fof(xp, np_halo, L_Box, b, np_halo_min=10, vp=None, mp=None,
+               n_threads=1, with_labels=False):
+    """Plain FoF (bound_check=False) exactly as the reference computes it.
+
+    xp: (3, N) f64; np_halo: region sizes. Returns (in_halo int32[N],
+    halo_id int32[N], ang_mom (3, nhalo) f64[, labels int32[N]]).
+    """
+    lib = load_oracle()
+    xp = _f64_fortran(xp)
+    n = xp.shape[1]
+    np_halo = np.ascontiguousarray(np_halo, dtype=np.int32)
+    nh = len(np_halo)
+    in_halo = np.zeros(n, np.int32)
+    halo_id = np.zeros(n, np.int32)
+    ang = np.zeros((3, max(nh, 1)), np.float64, order="F")
+    labels = np.zeros(max(n, 1), np.int32) if with_labels else None
+    vpa = _f64_fortran(vp) if vp is not None else None
+    mpa = np.ascontiguousarray(mp, dtype=np.float64) if mp is not None else None
+    dp = ctypes.c_double
+    ierr = lib.oracle_fof(
+        _ptr(xp, dp), _ptr(vpa, dp) if vpa is not None else None,
+        _ptr(mpa, dp) if mpa is not None else None, _ptr(np_halo, ctypes.c_int32), nh,
+        float(L_Box), float(b), int(np_halo_min), int(n_threads), n,
+        _ptr(in_halo, ctypes.c_int32), _ptr(halo_id, ctypes.c_int32), _ptr(ang, dp),
+        _ptr(labels, ctypes.c_int32) if labels is not None else None)
+    if ierr != 0:
+        raise RuntimeError(f"oracle_fof ierr={ierr}")
+    ang = ang[:, :nh]
+    if with_labels:
+        return in_halo, halo_id, ang, labels[:n]
+    return in_halo, halo_id, ang
+
+
+def oracle_candidates(xp, np_halo, L_Box, b, np_halo_min=10):
+    """Candidates in the reference's order with members in CRS order
+    (fof_fortran.f90:263-308). Returns (cand_np, cand_region, members)."""
+    lib = load_oracle()
+    xp = _f64_fortran(xp)
+    n = xp.shape[1]
+    np_halo = np.ascontiguousarray(np_halo, dtype=np.int32)
+    nm = ctypes.c_int32(0)
+    cap_c = n
+    cand_np = np.zeros(max(cap_c, 1), np.int32)
+    cand_reg = np.zeros(max(cap_c, 1), np.int32)
+    members = np.zeros(max(n, 1), np.int32)
+    nc = lib.oracle_fof_candidates(_ptr(xp, ctypes.c_double), _ptr(np_halo, ctypes.c_int32),
+                                   len(np_halo), float(L_Box), float(b), int(np_halo_min), n,
+                                   cap_c, n, _ptr(cand_np, ctypes.c_int32),
+                                   _ptr(cand_reg, ctypes.c_int32), _ptr(members, ctypes.c_int32),
+                                   ctypes.byref(nm))
+    if nc < 0:
+        raise RuntimeError("oracle_fof_candidates failed")
+    return cand_np[:nc].copy(), cand_reg[:nc].copy(), members[:nm.value].copy()
+
+
+def oracle_geometry(xp, L_Box, b):
+    """Reference cell geometry of one region (fof_fortran.f90:518-570)."""
+    lib = load_oracle()
+    xp = _f64_fortran(xp)
+    g = OracleGeom()
+    lib.oracle_geometry_of(_ptr(xp, ctypes.c_double), xp.shape[1], float(L_Box), float(b),
+                           ctypes.byref(g))
+    return {
+        "origin": list(g.origin), "cell": list(g.cell), "n": list(g.n),
+        "periodic": [bool(v) for v in g.periodic], "is_periodic": bool(g.is_periodic),
+        "bmin": list(g.bmin), "bmax": list(g.bmax),
+    }

@@ -1,0 +1,283 @@
+"""MI355X-native Friends-of-Friends halo finder with gravitational unbinding.
+
+Drop-in for ``py_fof``: ``fof()`` and ``fof_with_info()`` keep the reference's
+signatures, defaults, unit handling, return types and error messages
+(py_fof/__init__.py:168-518). The work runs in hand-written HIP kernels for
+gfx950 in ``libpyfof_hip.so``, reached through the C-ABI in include/pyfof.h.
+There is no CPU fallback: without the library every call raises.
+"""
+import numpy as np
+
+from . import _native
+from ._native import MEM_DEVICE, MEM_HOST, Context, NativeLibraryError  # noqa: F401
+
+__all__ = ["fof", "fof_with_info", "Context", "NativeLibraryError"]
+__version__ = "0.1.0"
+
+
+def _error_for(ierr):
+    # py_fof/__init__.py:338-348
+    if ierr == 1:
+        return ValueError("Invalid input parameters or configuration in FOF/bound_test")
+    if ierr == 2:
+        return ValueError("Candidate halo span exceeds 15 Mpc limit (s_15)")
+    if ierr == 4:
+        return ValueError("Non-finite values (NaN/Inf) encountered during calculation")
+    if ierr == 10:
+        return ValueError("n_threads exceeds maximum supported slots (256)")
+    return RuntimeError(f"FOF Fortran computation failed with error code ierr={ierr}")
+
+
+def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
+              np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
+              n_threads=None, mp_list=None, softening=None, z=0.0,
+              box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
+              units='cube', include_hubble_flow=False,
+              ng=None, ng_global=None, _ctx=None, _want_catalog=False, _want_roots=False):
+    """Validation, unit conversion and the native call (py_fof/__init__.py:168-351).
+
+    Returns (in_halo bool[N], halo_id int32[N], ang_mom (3, nhalo) f64) and,
+    when asked, the device catalog and the partition roots.
+    """
+    if units not in ('cube', 'physical'):
+        raise ValueError(f"units must be 'cube' or 'physical', got '{units}'")
+
+    if n_threads is None or n_threads <= 0:
+        n_threads_val = 0
+    else:
+        if n_threads > 256:
+            raise ValueError("n_threads cannot exceed 256")
+        n_threads_val = int(n_threads)
+
+    if isinstance(xp_list, np.ndarray) and xp_list.ndim == 2:
+        xp_list_norm = [xp_list]
+    elif isinstance(xp_list, (list, tuple)):
+        xp_list_norm = list(xp_list)
+    else:
+        raise ValueError("xp_list must be a 2D numpy array (3, N) or a list of 2D arrays")
+
+    nhalo = len(xp_list_norm)
+    for i, xp in enumerate(xp_list_norm):
+        if not isinstance(xp, np.ndarray) or xp.ndim != 2 or xp.shape[0] != 3:
+            raise ValueError(f"xp_list[{i}] must have shape (3, N), got {getattr(xp, 'shape', type(xp))}")
+
+    np_halo = np.array([xp.shape[1] for xp in xp_list_norm], dtype=np.int32)
+    total_np = int(np.sum(np_halo))
+
+    extras = {}
+    if total_np == 0:
+        in_halo = np.zeros(0, dtype=bool)
+        halo_id = np.zeros(0, dtype=np.int32)
+        ang_mom = np.zeros((3, nhalo), dtype=np.float64)
+        if _want_catalog:
+            extras["catalog"] = (np.zeros(1, np.int64), np.zeros(0, np.int32))
+        if _want_roots:
+            extras["roots"] = np.zeros(0, np.int32)
+        return (in_halo, halo_id, ang_mom, extras) if (_want_catalog or _want_roots) else (in_halo, halo_id, ang_mom)
+
+    xp_halo = np.asfortranarray(np.hstack(xp_list_norm), dtype=np.float64)
+
+    if bound_check and vp_list is None:
+        raise ValueError("vp_list is required when bound_check=True")
+
+    vp_halo = None
+    if vp_list is not None:
+        if isinstance(vp_list, np.ndarray) and vp_list.ndim == 2:
+            vp_list_norm = [vp_list]
+        elif isinstance(vp_list, (list, tuple)):
+            vp_list_norm = list(vp_list)
+        else:
+            raise ValueError("vp_list must be a 2D numpy array (3, N) or a list of 2D arrays")
+        if len(vp_list_norm) != nhalo:
+            raise ValueError(f"vp_list length ({len(vp_list_norm)}) must match xp_list length ({nhalo})")
+        for i, vp in enumerate(vp_list_norm):
+            if not isinstance(vp, np.ndarray) or vp.ndim != 2 or vp.shape != xp_list_norm[i].shape:
+                raise ValueError(f"vp_list[{i}] shape {getattr(vp, 'shape', None)} must match xp_list[{i}] shape {xp_list_norm[i].shape}")
+        vp_halo = np.asfortranarray(np.hstack(vp_list_norm), dtype=np.float64)
+    # a missing vp_list means zero velocities (py_fof/__init__.py:233); the
+    # library treats a null vp that way, so no zero array is materialised.
+
+    mp_raw = None
+    if mp_list is not None:
+        if np.isscalar(mp_list) or (isinstance(mp_list, np.ndarray) and mp_list.ndim == 0):
+            val = float(mp_list)
+            if val <= 0 or not np.isfinite(val):
+                raise ValueError("mp_list scalar must be positive and finite")
+            mp_raw = np.full(total_np, val, dtype=np.float64)
+        elif isinstance(mp_list, np.ndarray):
+            if mp_list.ndim == 1 and len(mp_list) == total_np:
+                mp_raw = np.ascontiguousarray(mp_list, dtype=np.float64)
+            elif nhalo > 1 and mp_list.ndim == 1 and len(mp_list) == nhalo:
+                mp_raw = np.repeat(np.ascontiguousarray(mp_list, dtype=np.float64), np_halo)
+            else:
+                raise ValueError(f"mp_list 1D array length ({len(mp_list)}) does not match total particles ({total_np})")
+        elif isinstance(mp_list, (list, tuple)):
+            if len(mp_list) == nhalo and all(isinstance(m, np.ndarray) and m.ndim == 1 and len(m) == np_halo[i] for i, m in enumerate(mp_list)):
+                mp_raw = np.concatenate(mp_list).astype(np.float64)
+            elif len(mp_list) == nhalo and all(np.isscalar(m) for m in mp_list):
+                scalars = np.array([float(m) for m in mp_list], dtype=np.float64)
+                mp_raw = np.repeat(scalars, np_halo)
+            elif len(mp_list) == total_np and all(np.isscalar(m) for m in mp_list):
+                mp_raw = np.array(mp_list, dtype=np.float64)
+            else:
+                raise ValueError("mp_list structure does not match xp_list structure or particle counts")
+        else:
+            raise ValueError("Unsupported mp_list type")
+
+        if not np.all(np.isfinite(mp_raw)) or not np.all(mp_raw > 0):
+            raise ValueError("All particle masses in mp_list must be positive and finite")
+
+    z_val = float(z)
+    if 1.0 + z_val <= 0.0:
+        raise ValueError(f"1 + z must be positive, got {1.0 + z_val}")
+    a_val = 1.0 / (1.0 + z_val)
+
+    if include_hubble_flow:
+        t0 = 1.0 / (150.0 * float(h0) * np.sqrt(float(omega_m)))
+        hubble = 100.0 * float(h0) * np.sqrt(float(omega_m) / (a_val ** 3) + float(omega_l))
+        b_val = hubble * (a_val ** 2) * t0
+    else:
+        b_val = 0.0
+
+    if units == 'cube':
+        softening_grid = 0.1 if softening is None else float(softening)
+        if bound_check and softening_grid <= 0:
+            raise ValueError("softening must be positive")
+        ng_eff_global = float(ng_global) if ng_global is not None else float(L_Box)
+        s_15_val = 15.0 * float(h0) * ng_eff_global / float(box)
+        xp_code = xp_halo
+        L_Box_code = float(L_Box)
+        link_code = float(linking_length)
+        vp_code = vp_halo
+        mp_code = mp_raw  # None: unit masses inside the library
+        ang_mom_scale = 1.0
+    else:
+        if bound_check:
+            if softening is None or float(softening) <= 0:
+                raise ValueError("softening must be explicitly provided and > 0 when units='physical' and bound_check=True")
+            if mp_raw is None and n_sim is None:
+                raise ValueError("Either mp_list or n_sim must be provided when units='physical' and bound_check=True")
+
+        ng_eff = float(round(float(n_sim) ** (1.0 / 3.0))) if (n_sim is not None and float(n_sim) > 0) else (float(ng_global) if ng_global is not None else 512.0)
+        box_mpc_h = float(L_Box)
+        dx = box_mpc_h / ng_eff
+
+        xp_code = np.asfortranarray(xp_halo / dx, dtype=np.float64)
+        L_Box_code = ng_eff
+        link_code = float(linking_length) / dx
+        softening_grid = 0.1 if softening is None else (float(softening) / dx)
+        s_15_val = 15.0 * float(h0) * ng_eff / box_mpc_h
+
+        v_scale = (a_val * ng_eff) / (150.0 * np.sqrt(float(omega_m)) * box_mpc_h)
+        vp_code = np.asfortranarray(vp_halo * v_scale, dtype=np.float64) if vp_halo is not None else None
+
+        rho_crit_0 = 2.77536627e11
+        m_grid = float(omega_m) * rho_crit_0 * (box_mpc_h ** 3) / (ng_eff ** 3)
+
+        if mp_raw is not None:
+            mp_code = np.ascontiguousarray(mp_raw / m_grid, dtype=np.float64)
+        elif n_sim is not None:
+            mp_code = np.full(total_np, (ng_eff ** 3) / float(n_sim), dtype=np.float64)
+        else:
+            mp_code = None
+            m_grid = 1.0
+
+        ang_mom_scale = m_grid * dx / v_scale
+
+    ng_val = int(ng) if ng is not None else 256
+    ng_global_val = float(ng_global) if ng_global is not None else 512.0
+
+    in_halo_out = np.zeros(total_np, dtype=np.int32)
+    halo_id_out = np.zeros(total_np, dtype=np.int32)
+    ang_mom_out = np.zeros((3, nhalo), dtype=np.float64, order='F')
+    roots = np.zeros(total_np, dtype=np.int32) if _want_roots else None
+    # the moment is only observable with return_ang_mom; with zero velocities it is 0
+    need_am = bool(return_ang_mom) and vp_code is not None
+    ctx = _ctx if _ctx is not None else _native.default_context()
+    ctx.set_option(_native.OPT_CATALOG, 1 if _want_catalog else 0)
+    ierr = ctx.fof(xp_code, vp_code, mp_code, np_halo, L_Box_code, link_code, int(np_halo_min),
+                   int(bool(bound_check)), int(verbose), softening_grid, a_val, b_val, s_15_val,
+                   n_threads_val, total_np, ng_val, ng_global_val, in_halo_out, halo_id_out,
+                   ang_mom_out if need_am else None, roots, MEM_HOST)
+
+    if ierr != 0:
+        if verbose:
+            print(f"[py_fof_amd] ierr={ierr}: {_native.last_error()}")
+        raise _error_for(ierr)
+
+    ang_mom_scaled = ang_mom_out * ang_mom_scale
+    out = (in_halo_out.astype(bool), halo_id_out, ang_mom_scaled)
+    if _want_catalog or _want_roots:
+        if _want_catalog:
+            extras["catalog"] = ctx.catalog()
+        if _want_roots:
+            extras["roots"] = roots
+        return out + (extras,)
+    return out
+
+
+def fof(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
+        np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
+        n_threads=None, mp_list=None, softening=None, z=0.0,
+        box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
+        units='cube', include_hubble_flow=False,
+        ng=None, ng_global=None):
+    """Friends-of-Friends halo finder (py_fof/__init__.py:354-447).
+
+    Returns ``in_halo`` (bool, shape (total_np,)) and, with
+    ``return_ang_mom=True``, also the (3, nhalo) per-region angular momentum.
+    """
+    in_halo, _, ang_mom = _fof_core(
+        xp_list=xp_list, vp_list=vp_list, L_Box=L_Box, linking_length=linking_length,
+        np_halo_min=np_halo_min, bound_check=bound_check, verbose=verbose,
+        return_ang_mom=return_ang_mom, n_threads=n_threads, mp_list=mp_list,
+        softening=softening, z=z, box=box, h0=h0, omega_m=omega_m, omega_l=omega_l,
+        n_sim=n_sim, units=units, include_hubble_flow=include_hubble_flow,
+        ng=ng, ng_global=ng_global)
+    if return_ang_mom:
+        return in_halo, ang_mom
+    return in_halo
+
+
+def fof_with_info(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
+                  np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
+                  n_threads=None, mp_list=None, softening=None, z=0.0,
+                  box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
+                  units='cube', include_hubble_flow=False,
+                  ng=None, ng_global=None):
+    """FoF with the halo catalog (py_fof/__init__.py:450-518).
+
+    The catalog (members grouped by halo id, ascending indices) is built on
+    the device by a stable radix sort instead of a host argsort.
+    """
+    in_halo, halo_id, ang_mom, extras = _fof_core(
+        xp_list=xp_list, vp_list=vp_list, L_Box=L_Box, linking_length=linking_length,
+        np_halo_min=np_halo_min, bound_check=bound_check, verbose=verbose,
+        return_ang_mom=return_ang_mom, n_threads=n_threads, mp_list=mp_list,
+        softening=softening, z=z, box=box, h0=h0, omega_m=omega_m, omega_l=omega_l,
+        n_sim=n_sim, units=units, include_hubble_flow=include_hubble_flow,
+        ng=ng, ng_global=ng_global, _want_catalog=True)
+
+    off, idx = extras["catalog"]
+    nh = len(off) - 1
+    if nh <= 0:
+        nhalo_found = 0
+        np_per_halo = np.array([], dtype=int)
+        halo_indices = []
+    else:
+        counts = np.diff(off).astype(np.int64)
+        idx64 = idx.astype(np.int64)
+        halo_indices = [idx64[off[h]:off[h + 1]] for h in range(nh) if counts[h] > 0]
+        np_per_halo = counts[counts > 0]
+        nhalo_found = len(halo_indices)
+
+    res = {
+        'in_halo': in_halo,
+        'nhalo_found': nhalo_found,
+        'np_per_halo': np_per_halo,
+        'halo_indices': halo_indices,
+        'bound_status': np.zeros(nhalo_found, dtype=np.int32),
+    }
+    if return_ang_mom:
+        res['ang_mom'] = ang_mom
+    return res

@@ -1,0 +1,166 @@
+"""ctypes binding of libpyfof_hip.so (the C-ABI declared in include/pyfof.h).
+
+There is no fallback: if the library is missing or cannot be loaded, every
+call raises. Build it with ``python -m py_fof_amd.build``.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyfof_hip.so")
+
+MEM_HOST = 0
+MEM_DEVICE = 1
+OPT_CATALOG = 1
+
+EXPORTS = [
+    "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",
+    "pyfof_catalog_size", "pyfof_catalog_copy", "pyfof_ctx_timings", "pyfof_ctx_stats",
+    "pyfof_ctx_stream", "pyfof_ctx_set_option",
+    "pyfof_last_error", "pyfof_version",
+]
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Loads and types the library once; raises NativeLibraryError if absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            _lib = _load_from(path)
+        return _lib
+
+
+def _load_from(path):
+    if not os.path.exists(path):
+        raise NativeLibraryError(
+            f"{path} not found: build it with `python -m py_fof_amd.build`")
+    if True:
+        lib = ctypes.CDLL(path)
+        d, i32, i64, vp = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+        lib.pyfof_fof.restype = i32
+        lib.pyfof_fof.argtypes = [vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d, i32, i32,
+                                  i64, d, vp, vp, vp, i32]
+        lib.pyfof_fof_ctx.restype = i32
+        lib.pyfof_fof_ctx.argtypes = [vp, vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d,
+                                      i32, i32, i64, d, vp, vp, vp, vp, i32]
+        lib.pyfof_ctx_create.restype = vp
+        lib.pyfof_ctx_create.argtypes = [i32]
+        lib.pyfof_ctx_destroy.restype = None
+        lib.pyfof_ctx_destroy.argtypes = [vp]
+        lib.pyfof_catalog_size.restype = i32
+        lib.pyfof_catalog_size.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i64)]
+        lib.pyfof_catalog_copy.restype = i32
+        lib.pyfof_catalog_copy.argtypes = [vp, vp, vp, i32]
+        lib.pyfof_ctx_timings.restype = i32
+        lib.pyfof_ctx_timings.argtypes = [vp, ctypes.POINTER(d), i32]
+        lib.pyfof_ctx_set_option.restype = i32
+        lib.pyfof_ctx_set_option.argtypes = [vp, i32, i64]
+        lib.pyfof_ctx_stats.restype = i32
+        lib.pyfof_ctx_stats.argtypes = [vp, ctypes.POINTER(i64), i32]
+        lib.pyfof_ctx_stream.restype = vp
+        lib.pyfof_ctx_stream.argtypes = [vp]
+        lib.pyfof_last_error.restype = ctypes.c_char_p
+        lib.pyfof_last_error.argtypes = []
+        lib.pyfof_version.restype = ctypes.c_char_p
+        lib.pyfof_version.argtypes = []
+        return lib
+
+
+def last_error():
+    return load().pyfof_last_error().decode(errors="replace")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor on the device
+
+
+class Context:
+    """A device, its HIP stream and the reusable HBM workspace of the library."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = self.lib.pyfof_ctx_create(int(device))
+        if not self.h:
+            raise NativeLibraryError("pyfof_ctx_create failed: " + last_error())
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.pyfof_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fof(self, xp, vp, mp, np_halo, L_Box, link, np_halo_min, bound_check, verbose,
+            softening, a_val, b_val, s_15, n_threads, total_np, ng, ng_global, in_halo,
+            halo_id, ang_mom, root_out=None, mem_kind=MEM_HOST):
+        np_halo = np.ascontiguousarray(np_halo, dtype=np.int32)
+        return self.lib.pyfof_fof_ctx(
+            self.h, _ptr(xp), _ptr(vp), _ptr(mp), np_halo.ctypes.data, len(np_halo),
+            float(L_Box), float(link), int(np_halo_min), int(bound_check), int(verbose),
+            float(softening), float(a_val), float(b_val), float(s_15), int(n_threads),
+            int(total_np), int(ng), float(ng_global), _ptr(in_halo), _ptr(halo_id),
+            _ptr(ang_mom), _ptr(root_out), int(mem_kind))
+
+    def set_option(self, option, value):
+        err = self.lib.pyfof_ctx_set_option(self.h, int(option), int(value))
+        if err:
+            raise RuntimeError("pyfof_ctx_set_option failed: " + last_error())
+
+    def catalog(self):
+        """(offsets int64[nh+1], indices int32[total]) of the last call."""
+        nh = ctypes.c_int32(0)
+        nm = ctypes.c_int64(0)
+        err = self.lib.pyfof_catalog_size(self.h, ctypes.byref(nh), ctypes.byref(nm))
+        if err:
+            raise RuntimeError("pyfof_catalog_size failed: " + last_error())
+        off = np.zeros(nh.value + 1, np.int64)
+        idx = np.zeros(max(nm.value, 1), np.int32)
+        err = self.lib.pyfof_catalog_copy(self.h, off.ctypes.data, idx.ctypes.data, MEM_HOST)
+        if err:
+            raise RuntimeError("pyfof_catalog_copy failed: " + last_error())
+        return off, idx[:nm.value]
+
+    def timings(self):
+        buf = (ctypes.c_double * 8)()
+        n = self.lib.pyfof_ctx_timings(self.h, buf, 8)
+        return [buf[i] for i in range(n)]
+
+    def stats(self):
+        """dict of the last call's sizes (particles, cells, dense cells, halos, members)."""
+        buf = (ctypes.c_int64 * 5)()
+        n = self.lib.pyfof_ctx_stats(self.h, buf, 5)
+        keys = ["n", "ncell", "ndense", "nhalo", "nmem"]
+        return {keys[i]: int(buf[i]) for i in range(n)}
+
+    def stream(self):
+        return self.lib.pyfof_ctx_stream(self.h)
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    c = _default_ctx.get(device)
+    if c is None:
+        c = Context(device)
+        _default_ctx[device] = c
+    return c

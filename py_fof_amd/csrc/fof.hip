@@ -1,0 +1,1151 @@
+// Friends-of-Friends on MI355X: cell binning, linking, relabel and catalog.
+//
+// Pipeline of one call (all on the context's stream):
+//   k_bbox         per-region bounding box of the wrapped coordinates
+//   (host)         reference cell geometry per region (fof_fortran.f90:518-570)
+//   k_keys         reference cell id of every particle
+//   radix sort     stable sort by cell id = the reference's CRS order
+//   k_run_ends     + inclusive max-scan -> cell_ptr
+//   k_gather       float CRS coordinates (the reference's x_crs)
+//   k_dense_*      sub-cells (side <= (b - m)/sqrt 3) inside dense cells
+//   k_link         pair tests with the reference predicate + atomicCAS DSU
+//   k_flatten / k_count / k_cand / k_write   roots, sizes, numbering, outputs
+//   catalog        halo members grouped by halo id (fof_with_info, ang. mom.)
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pyfof.h"
+#include "common.hpp"
+#include "fof_geom.hpp"
+#include "fof_kernels.hpp"
+#include "prims.hpp"
+
+namespace pyfof {
+
+constexpr int kBlk = 256;
+
+// ---------------------------------------------------------------------------
+// small fills
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_iota_u32(uint32_t* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// per-region bounding box of x_rel (fof_fortran.f90:524-531)
+__global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
+                                               const Chunk* __restrict__ chunks,
+                                               const int32_t* __restrict__ reg_start,
+                                               unsigned long long* __restrict__ bb, double L,
+                                               double Lh) {
+    const Chunk ch = chunks[blockIdx.x];
+    const size_t a = (size_t)reg_start[ch.region];
+    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
+        const double v0 = rel_coord(xp[3 * (size_t)i], xa0, L, Lh);
+        const double v1 = rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh);
+        const double v2 = rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh);
+        mn[0] = fmin(mn[0], v0); mx[0] = fmax(mx[0], v0);
+        mn[1] = fmin(mn[1], v1); mx[1] = fmax(mx[1], v1);
+        mn[2] = fmin(mn[2], v2); mx[2] = fmax(mx[2], v2);
+    }
+    __shared__ double red[6][kBlk / kWave];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[d] = fmin(mn[d], __shfl_down(mn[d], o, kWave));
+            mx[d] = fmax(mx[d], __shfl_down(mx[d], o, kWave));
+        }
+    }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        for (int d = 0; d < 3; ++d) { red[d][w] = mn[d]; red[3 + d][w] = mx[d]; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        double v = red[d][0];
+        for (int k = 1; k < kBlk / kWave; ++k) v = d < 3 ? fmin(v, red[d][k]) : fmax(v, red[d][k]);
+        unsigned long long* dst = bb + 6 * (size_t)ch.region + d;
+        if (d < 3) atomicMin(dst, dbl_to_ordered(v));
+        else atomicMax(dst, dbl_to_ordered(v));
+    }
+}
+
+// reference cell id per particle (fof_fortran.f90:580-585, 0-based global id)
+__global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
+                                               const Chunk* __restrict__ chunks,
+                                               const RegionDev* __restrict__ reg,
+                                               uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ vals, double L, double Lh) {
+    const Chunk ch = chunks[blockIdx.x];
+    const RegionDev& g = reg[ch.region];
+    const uint32_t off = g.cell_off;
+    if (g.skip) {
+        for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
+            keys[i] = off;
+            vals[i] = (uint32_t)i;
+        }
+        return;
+    }
+    const size_t a = (size_t)g.start;
+    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
+    const double o0 = g.origin[0], o1 = g.origin[1], o2 = g.origin[2];
+    const double c0 = g.cell[0], c1 = g.cell[1], c2 = g.cell[2];
+    const int32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
+        const int32_t ix = ref_cell_coord(rel_coord(xp[3 * (size_t)i], xa0, L, Lh), o0, c0, n0);
+        const int32_t iy = ref_cell_coord(rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh), o1, c1, n1);
+        const int32_t iz = ref_cell_coord(rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), o2, c2, n2);
+        keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
+        vals[i] = (uint32_t)i;
+    }
+}
+
+// ends[c] = 1 + last position of cell c (0 for empty cells, zero-filled before)
+__global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t* __restrict__ ends) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = skeys[p];
+    if (p + 1 == n || skeys[p + 1] != k) ends[k] = p + 1;
+}
+
+// x_crs = real(x_rel, 4) in CRS order (fof_fortran.f90:608); w holds the region.
+__global__ void __launch_bounds__(kBlk) k_gather(const double* __restrict__ xp,
+                                                 const uint32_t* __restrict__ perm,
+                                                 const uint32_t* __restrict__ skeys,
+                                                 const uint32_t* __restrict__ cell_off,
+                                                 const RegionDev* __restrict__ reg,
+                                                 float4* __restrict__ xc, int32_t* __restrict__ parent,
+                                                 uint32_t n, int32_t nreg, double L, double Lh) {
+    const uint32_t p = blockIdx.x * kBlk + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t r = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, skeys[p]);
+    const size_t i = perm[p];
+    const size_t a = (size_t)reg[r].start;
+    float4 v;
+    v.x = (float)rel_coord(xp[3 * i], xp[3 * a], L, Lh);
+    v.y = (float)rel_coord(xp[3 * i + 1], xp[3 * a + 1], L, Lh);
+    v.z = (float)rel_coord(xp[3 * i + 2], xp[3 * a + 2], L, Lh);
+    v.w = __int_as_float((int)r);
+    xc[p] = v;
+    parent[p] = (int32_t)p;
+}
+
+// marks cells with at least kDenseMin particles in regions with sub-cells
+__global__ void k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
+                             const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg,
+                             int32_t nreg, int32_t* __restrict__ aux, uint32_t* __restrict__ dense_cells,
+                             uint32_t* __restrict__ ndense) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int32_t d = -1;
+    if (cell_ptr[c + 1] - cell_ptr[c] >= (uint32_t)kDenseMin) {
+        const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+        if (!g.skip && g.prune && g.k[0] * g.k[1] * g.k[2] > 1) {
+            d = (int32_t)atomicAdd(ndense, 1u);
+            dense_cells[d] = c;
+        }
+    }
+    aux[c] = d;
+}
+
+__device__ inline void cell_lower_corner(const RegionDev& g, uint32_t lc, double lo[3], int32_t ijk[3]) {
+    const uint32_t n0 = (uint32_t)g.n[0], n1 = (uint32_t)g.n[1];
+    ijk[0] = (int32_t)(lc % n0);
+    ijk[1] = (int32_t)((lc / n0) % n1);
+    ijk[2] = (int32_t)(lc / (n0 * n1));
+    for (int d = 0; d < 3; ++d) lo[d] = g.origin[d] + (double)ijk[d] * g.cell[d];
+}
+
+// Sub-cell lists of the dense cells: per dense cell a counting sort of its
+// positions by sub-cell (sub_ptr: offsets relative to the cell start,
+// sub_list: positions) and, when sub-cells are cliques, the pre-union of
+// each sub-cell onto its smallest position.
+__global__ void __launch_bounds__(kBlk) k_dense_build(
+    const uint32_t* __restrict__ dense_cells, const uint32_t* __restrict__ cell_ptr,
+    const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg, int32_t nreg,
+    const float4* __restrict__ xc, uint32_t* __restrict__ sub_ptr, int32_t sub_stride,
+    uint32_t* __restrict__ sub_list, int32_t* __restrict__ parent) {
+    __shared__ uint32_t cnt[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim + 1];
+    __shared__ uint32_t minp[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim];
+    __shared__ uint32_t wsum[kBlk / kWave];
+    const uint32_t c = dense_cells[blockIdx.x];
+    const uint32_t r = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c);
+    const RegionDev& g = reg[r];
+    double lo[3];
+    int32_t ijk[3];
+    cell_lower_corner(g, c - g.cell_off, lo, ijk);
+    const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
+    const int32_t K = k0 * k1 * k2;
+    const uint32_t s = cell_ptr[c], e = cell_ptr[c + 1];
+    for (int i = threadIdx.x; i < K; i += kBlk) { cnt[i] = 0; minp[i] = 0xffffffffu; }
+    __syncthreads();
+    for (uint32_t p = s + threadIdx.x; p < e; p += kBlk) {
+        const float4 v = xc[p];
+        const int32_t sx = sub_coord((double)v.x, lo[0], g.sub[0], k0);
+        const int32_t sy = sub_coord((double)v.y, lo[1], g.sub[1], k1);
+        const int32_t sz = sub_coord((double)v.z, lo[2], g.sub[2], k2);
+        const int32_t si = sx + k0 * (sy + k1 * sz);
+        atomicAdd(&cnt[si], 1u);
+        atomicMin(&minp[si], p);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0..K) by one wave per 64-entry stripe, serially over stripes
+    if (threadIdx.x < kWave) {
+        uint32_t carry = 0;
+        for (int base = 0; base < K; base += kWave) {
+            const int i = base + threadIdx.x;
+            const uint32_t v = i < K ? cnt[i] : 0u;
+            uint32_t incl = v;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o, kWave);
+                if ((int)__lane_id() >= o) incl += t;
+            }
+            if (i < K) cnt[i] = carry + incl - v;
+            carry += __shfl(incl, kWave - 1, kWave);
+        }
+        if (threadIdx.x == 0) cnt[K] = carry;
+    }
+    (void)wsum;
+    __syncthreads();
+    uint32_t* row = sub_ptr + (size_t)blockIdx.x * sub_stride;
+    for (int i = threadIdx.x; i <= K; i += kBlk) row[i] = cnt[i];
+    __syncthreads();
+    for (uint32_t p = s + threadIdx.x; p < e; p += kBlk) {
+        const float4 v = xc[p];
+        const int32_t sx = sub_coord((double)v.x, lo[0], g.sub[0], k0);
+        const int32_t sy = sub_coord((double)v.y, lo[1], g.sub[1], k1);
+        const int32_t sz = sub_coord((double)v.z, lo[2], g.sub[2], k2);
+        const int32_t si = sx + k0 * (sy + k1 * sz);
+        const uint32_t slot = atomicAdd(&cnt[si], 1u);
+        sub_list[s + slot] = p;
+        if (g.clique) parent[p] = (int32_t)minp[si];
+    }
+}
+
+// Per-thread linking context.
+struct LinkCtx {
+    int32_t* parent;
+    int32_t rp;  // last known root of the thread's particle
+    __device__ __forceinline__ void link(int32_t q) {
+        const int32_t rq = dsu_find(parent, q);
+        if (rq != rp) rp = dsu_unite(parent, rp, rq);
+    }
+};
+
+template <bool PERIODIC>
+__device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, const LinkParams& P) {
+    if (PERIODIC) return pred_periodic(a, q, P.L, P.Lh, P.rp2);
+    return pred_flat(a, q, P.rp2);
+}
+
+// Tests the particle `me` (position p) against cell jc. For the own cell only
+// partners q > p are tested. Pruning (pr) is exact: a pair passing the
+// predicate is within b (+ float rounding < m) per axis, so its partner lies
+// inside the query box [x - R, x + R] with R = b + m; the cell and sub-cell
+// coordinates used to prune are monotone in x.
+template <bool PERIODIC>
+__device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, const int32_t jijk[3],
+                                           const double q[3], bool own, const RegionDev& g,
+                                           const LinkParams& P, const uint32_t* __restrict__ cell_ptr,
+                                           const int32_t* __restrict__ aux,
+                                           const uint32_t* __restrict__ sub_ptr,
+                                           const uint32_t* __restrict__ sub_list,
+                                           const float4* __restrict__ xc, LinkCtx& L) {
+    double lo[3];
+    for (int d = 0; d < 3; ++d) lo[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
+    if (g.prune && !own) {
+        for (int d = 0; d < 3; ++d) {
+            if (q[d] + P.R < lo[d] - P.m || q[d] - P.R > lo[d] + g.cell[d] + P.m) return;
+        }
+    }
+    const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
+    if (s >= e) return;
+    const int32_t dcell = aux[jc];
+    if (dcell >= 0) {
+        const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
+        int32_t a[3], bnd[3], mine[3] = {-1, -1, -1};
+        const int32_t kk[3] = {k0, k1, k2};
+        for (int d = 0; d < 3; ++d) {
+            a[d] = sub_coord(q[d] - P.R, lo[d], g.sub[d], kk[d]);
+            bnd[d] = sub_coord(q[d] + P.R, lo[d], g.sub[d], kk[d]);
+        }
+        if (own && g.clique) {
+            mine[0] = sub_coord((double)me.x, lo[0], g.sub[0], k0);
+            mine[1] = sub_coord((double)me.y, lo[1], g.sub[1], k1);
+            mine[2] = sub_coord((double)me.z, lo[2], g.sub[2], k2);
+        }
+        const uint32_t* row0 = sub_ptr + (size_t)dcell * P.sub_stride;
+        for (int32_t sz = a[2]; sz <= bnd[2]; ++sz) {
+            for (int32_t sy = a[1]; sy <= bnd[1]; ++sy) {
+                const uint32_t* row = row0 + (size_t)k0 * (sy + k1 * sz);
+                for (int32_t sx = a[0]; sx <= bnd[0]; ++sx) {
+                    if (own && g.clique && sx == mine[0] && sy == mine[1] && sz == mine[2]) continue;
+                    const uint32_t t0 = row[sx], t1 = row[sx + 1];
+                    for (uint32_t t = t0; t < t1; ++t) {
+                        const uint32_t qq = sub_list[s + t];
+                        if (own && qq <= p) continue;
+                        if (pair_linked<PERIODIC>(me, xc[qq], P)) {
+                            L.link((int32_t)qq);
+                            if (g.clique) break;
+                        }
+                    }
+                }
+            }
+        }
+    } else {
+        for (uint32_t qq = own ? p + 1 : s; qq < e; ++qq) {
+            if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
+        }
+    }
+}
+
+// Neighbour cell of (ijk + off) under the reference's rule (fof_fortran.f90:
+// 729-755 periodic, 684-704 non-periodic): wrap only along periodic axes,
+// keep jcell > icell. Returns false if the neighbour does not exist. shift
+// receives the periodic image offset that brings this particle next to it.
+__device__ __forceinline__ bool neighbour_cell(const RegionDev& g, const int32_t ijk[3], int o,
+                                               int32_t jijk[3], double shift[3], double L) {
+    const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+    for (int d = 0; d < 3; ++d) {
+        int32_t j = ijk[d] + off[d];
+        shift[d] = 0.0;
+        if (j < 0 || j >= g.n[d]) {
+            if (!g.periodic[d]) return false;
+            if (j < 0) { j += g.n[d]; shift[d] = L; } else { j -= g.n[d]; shift[d] = -L; }
+        }
+        jijk[d] = j;
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint32_t local_cell(const RegionDev& g, const int32_t jijk[3]) {
+    return (uint32_t)jijk[0] + (uint32_t)g.n[0] * ((uint32_t)jijk[1] + (uint32_t)g.n[1] * (uint32_t)jijk[2]);
+}
+
+template <bool PERIODIC>
+__device__ __forceinline__ void link_particle(uint32_t p, const float4& me, const RegionDev& g,
+                                              uint32_t c, const LinkParams& P,
+                                              const uint32_t* __restrict__ cell_ptr,
+                                              const int32_t* __restrict__ aux,
+                                              const uint32_t* __restrict__ sub_ptr,
+                                              const uint32_t* __restrict__ sub_list,
+                                              const float4* __restrict__ xc, int32_t* parent) {
+    LinkCtx L{parent, dsu_find(parent, (int32_t)p)};
+    const uint32_t lc = c - g.cell_off;
+    int32_t ijk[3];
+    double lo_unused[3];
+    cell_lower_corner(g, lc, lo_unused, ijk);
+    const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
+    visit_cell<PERIODIC>(me, p, c, ijk, x0, true, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, L);
+    for (int o = 0; o < 27; ++o) {
+        if (o == 13) continue;
+        int32_t jijk[3];
+        double sh[3];
+        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
+        const uint32_t jl = local_cell(g, jijk);
+        if (jl <= lc) continue;
+        if (!g.prune) {
+            // tiny periodic grids: the same neighbour can be reached twice
+            bool dup = false;
+            for (int o2 = 0; o2 < o && !dup; ++o2) {
+                if (o2 == 13) continue;
+                int32_t j2[3];
+                double s2[3];
+                if (neighbour_cell(g, ijk, o2, j2, s2, P.L) && local_cell(g, j2) == jl) dup = true;
+            }
+            if (dup) continue;
+        }
+        const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, jijk, q, false, g, P, cell_ptr, aux, sub_ptr,
+                             sub_list, xc, L);
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_link(const float4* __restrict__ xc,
+                                               const uint32_t* __restrict__ skeys,
+                                               const uint32_t* __restrict__ cell_ptr,
+                                               const int32_t* __restrict__ aux,
+                                               const uint32_t* __restrict__ sub_ptr,
+                                               const uint32_t* __restrict__ sub_list,
+                                               const RegionDev* __restrict__ reg, int32_t* parent,
+                                               LinkParams P) {
+    const uint32_t p = blockIdx.x * kBlk + threadIdx.x;
+    if (p >= P.N) return;
+    const float4 me = xc[p];
+    const RegionDev& g = reg[__float_as_int(me.w)];
+    if (g.skip) return;
+    const uint32_t c = skeys[p];
+    if (g.is_periodic)
+        link_particle<true>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
+    else
+        link_particle<false>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
+}
+
+__global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) label[p] = dsu_find(parent, (int32_t)p);
+}
+
+// component sizes; runs of equal labels inside a wave are summed first
+__global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n;
+    const int32_t lab = valid ? label[p] : -1;
+    const int32_t prev = __shfl_up(lab, 1, kWave);
+    const int lane = __lane_id();
+    const bool head = valid && (lane == 0 || prev != lab);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    if (head) {
+        const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+        const int next = above ? __ffsll((long long)above) - 1 : 64;
+        const int last_valid = 63 - __clzll((long long)vmask);
+        const int end = next < last_valid + 1 ? next : last_valid + 1;
+        atomicAdd(&size[lab], (uint32_t)(end - lane));
+    }
+}
+
+// candidate roots: root position with size >= np_min (fof_fortran.f90:866-872)
+__global__ void k_cand_flags(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
+                             uint32_t n, int32_t np_min, uint32_t* __restrict__ flag) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    flag[p] = (label[p] == (int32_t)p && (int64_t)size[p] >= (int64_t)np_min) ? 1u : 0u;
+}
+
+// in_halo / halo_id per particle (fof_fortran.f90:209-223); candidate anchors
+__global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
+                        const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ perm,
+                        const float4* __restrict__ xc, const RegionDev* __restrict__ reg,
+                        uint32_t n, int32_t np_min,
+                        int32_t* __restrict__ in_halo, int32_t* __restrict__ halo_id,
+                        int32_t* __restrict__ root_out, uint32_t* __restrict__ anchor,
+                        int32_t* __restrict__ hregion) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t r = label[p];
+    const bool cand = (int64_t)size[r] >= (int64_t)np_min;
+    const uint32_t i = perm[p];
+    in_halo[i] = cand ? 1 : 0;
+    halo_id[i] = cand ? (int32_t)cand_idx[r] + 1 : 0;
+    if (root_out) {
+        const int32_t regid = __float_as_int(xc[p].w);
+        root_out[i] = reg[regid].skip ? -1 : (int32_t)perm[r];
+    }
+    if (cand && r == (int32_t)p) {
+        anchor[cand_idx[r]] = i;
+        hregion[cand_idx[r]] = __float_as_int(xc[p].w);
+    }
+}
+
+// catalog: in-halo particles compacted in index order, keyed by halo id - 1
+__global__ void k_cat_flags(const int32_t* __restrict__ in_halo, uint32_t n, uint32_t* __restrict__ f) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = in_halo[i] ? 1u : 0u;
+}
+__global__ void k_cat_compact(const int32_t* __restrict__ halo_id, const uint32_t* __restrict__ pos,
+                              uint32_t n, uint32_t* __restrict__ ckeys, uint32_t* __restrict__ cvals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t h = halo_id[i];
+    if (h > 0) {
+        const uint32_t j = pos[i];
+        ckeys[j] = (uint32_t)(h - 1);
+        cvals[j] = i;
+    }
+}
+
+// Angular momentum of each halo about its f64 centre of mass, relative
+// coordinates wrapped about the anchor (fof_fortran.f90:224-248). One block
+// per halo; fixed reduction tree, so the result does not depend on timing.
+__device__ inline void block_sum7(double v[7], double (*lds)[kBlk / kWave]) {
+    for (int k = 0; k < 7; ++k)
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, kWave);
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int k = 0; k < 7; ++k) lds[k][w] = v[k];
+    __syncthreads();
+    for (int k = 0; k < 7; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < kBlk / kWave; ++j) s += lds[k][j];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlk) k_halo_angmom(const double* __restrict__ xp,
+                                                      const double* __restrict__ vp,
+                                                      const double* __restrict__ mp,
+                                                      const uint32_t* __restrict__ hoff,
+                                                      const uint32_t* __restrict__ members,
+                                                      const uint32_t* __restrict__ anchor,
+                                                      double* __restrict__ am_h, double L,
+                                                      double Lh) {
+    __shared__ double lds[7][kBlk / kWave];
+    const uint32_t h = blockIdx.x;
+    const uint32_t s = h == 0 ? 0u : hoff[h - 1], e = hoff[h];
+    const size_t a = anchor[h];
+    const double xa[3] = {xp[3 * a], xp[3 * a + 1], xp[3 * a + 2]};
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t t = s + threadIdx.x; t < e; t += kBlk) {
+        const size_t i = members[t];
+        const double w = mp ? mp[i] : 1.0;
+        v[0] += w;
+        for (int d = 0; d < 3; ++d) {
+            v[1 + d] += w * rel_coord(xp[3 * i + d], xa[d], L, Lh);
+            v[4 + d] += w * vp[3 * i + d];
+        }
+    }
+    block_sum7(v, lds);
+    if (!(v[0] > 0.0)) {
+        if (threadIdx.x < 3) am_h[3 * (size_t)h + threadIdx.x] = 0.0;
+        return;
+    }
+    const double qc[3] = {v[1] / v[0], v[2] / v[0], v[3] / v[0]};
+    const double vc[3] = {v[4] / v[0], v[5] / v[0], v[6] / v[0]};
+    double u[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t t = s + threadIdx.x; t < e; t += kBlk) {
+        const size_t i = members[t];
+        const double w = mp ? mp[i] : 1.0;
+        double dx[3], dv[3];
+        for (int d = 0; d < 3; ++d) {
+            dx[d] = rel_coord(xp[3 * i + d], xa[d], L, Lh) - qc[d];
+            dv[d] = vp[3 * i + d] - vc[d];
+        }
+        u[0] += w * (dx[1] * dv[2] - dx[2] * dv[1]);
+        u[1] += w * (dx[2] * dv[0] - dx[0] * dv[2]);
+        u[2] += w * (dx[0] * dv[1] - dx[1] * dv[0]);
+    }
+    block_sum7(u, lds);
+    if (threadIdx.x < 3) am_h[3 * (size_t)h + threadIdx.x] = u[threadIdx.x];
+}
+
+// Per-region sums of the halo moments; a region's halos are contiguous in
+// halo-id order. One block per region, fixed tree.
+__global__ void __launch_bounds__(kBlk) k_region_angmom(const double* __restrict__ am_h,
+                                                        const int32_t* __restrict__ hregion,
+                                                        uint32_t nh, double* __restrict__ am) {
+    __shared__ double lds[7][kBlk / kWave];
+    const int32_t r = blockIdx.x;
+    // first halo with region >= r, first with region > r
+    uint32_t lo = 0, hi = nh;
+    while (lo < hi) { const uint32_t mid = (lo + hi) / 2; if (hregion[mid] < r) lo = mid + 1; else hi = mid; }
+    const uint32_t s = lo;
+    hi = nh;
+    while (lo < hi) { const uint32_t mid = (lo + hi) / 2; if (hregion[mid] <= r) lo = mid + 1; else hi = mid; }
+    const uint32_t e = lo;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t h = s + threadIdx.x; h < e; h += kBlk)
+        for (int d = 0; d < 3; ++d) v[d] += am_h[3 * (size_t)h + d];
+    block_sum7(v, lds);
+    if (threadIdx.x < 3) am[3 * (size_t)r + threadIdx.x] = v[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes > cap) {
+            if (p) PYFOF_HIP(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            const size_t want = bytes + bytes / 8 + 256;
+            hipError_t e = hipMalloc(&p, want);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                p = nullptr;
+                throw HipError(hipErrorOutOfMemory, "device allocation failed");
+            }
+            cap = want;
+        }
+        return p;
+    }
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvCat, kEvBound, kNumEv };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[kNumEv] = {};
+    bool ev_recorded[kNumEv] = {};
+    DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
+        xc, parent, label, size, flag, cidx, aux, dense, sub_ptr, sub_list, temp, out_in, out_id,
+        out_root, anchor, hregion, scal, cat_keys, cat_vals, cat_keys2, cat_vals2, hoff, am_h, am,
+        pin;
+    // last call's sizes and catalog
+    uint64_t last_n = 0, last_ncell = 0, last_ndense = 0;
+    bool want_catalog = false;  // build the fof_with_info catalog on every call
+    uint32_t cat_nh = 0;
+    uint64_t cat_nmem = 0;
+    uint32_t* cat_members = nullptr;  // device, grouped by halo
+    uint32_t* cat_hoff = nullptr;     // device, inclusive ends
+    std::mutex mu;
+
+    explicit Ctx(int dev) : device(dev) {
+        PYFOF_HIP(hipSetDevice(dev));
+        PYFOF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
+    }
+    ~Ctx() {
+        (void)hipSetDevice(device);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
+                          &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
+                          &sub_ptr, &sub_list, &temp, &out_in, &out_id, &out_root, &anchor,
+                          &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
+                          &am_h, &am, &pin})
+            b->release();
+    }
+    void mark(int e) {
+        PYFOF_HIP(hipEventRecord(ev[e], stream));
+        ev_recorded[e] = true;
+    }
+};
+
+static thread_local std::string g_last_error;
+
+static void set_error(const std::string& m) { g_last_error = m; }
+
+static inline uint32_t nblk(uint64_t n) { return div_up(n == 0 ? 1 : n, kBlk); }
+
+struct CallArgs {
+    const double* xp;
+    const double* vp;
+    const double* mp;
+    const int32_t* np_halo;
+    int32_t nhalo;
+    double L, b;
+    int32_t np_min;
+    int32_t bound_check;
+    int32_t verbose;
+    int32_t total_np;
+    int32_t* in_halo;
+    int32_t* halo_id;
+    double* ang_mom;
+    int32_t* root_out;
+    int32_t mem_kind;
+};
+
+static int32_t run_fof(Ctx& C, const CallArgs& A) {
+    hipStream_t s = C.stream;
+    const uint32_t N = (uint32_t)A.total_np;
+    const int32_t H = A.nhalo;
+    const bool dev_mem = A.mem_kind == PYFOF_MEM_DEVICE;
+    for (auto& r : C.ev_recorded) r = false;
+    C.cat_nh = 0;
+    C.cat_nmem = 0;
+    C.cat_hoff = nullptr;
+    C.cat_members = nullptr;
+    C.mark(kEvStart);
+
+    // inputs in HBM
+    const double* xp = A.xp;
+    const double* vp = A.vp;
+    const double* mp = A.mp;
+    if (!dev_mem) {
+        double* d = C.xp.as<double>(3 * (size_t)N);
+        PYFOF_HIP(hipMemcpyAsync(d, A.xp, 24 * (size_t)N, hipMemcpyHostToDevice, s));
+        xp = d;
+        if (A.vp && A.ang_mom) {
+            double* dv = C.vp.as<double>(3 * (size_t)N);
+            PYFOF_HIP(hipMemcpyAsync(dv, A.vp, 24 * (size_t)N, hipMemcpyHostToDevice, s));
+            vp = dv;
+        }
+        if (A.mp && A.ang_mom) {
+            double* dm = C.mp.as<double>((size_t)N);
+            PYFOF_HIP(hipMemcpyAsync(dm, A.mp, 8 * (size_t)N, hipMemcpyHostToDevice, s));
+            mp = dm;
+        }
+    }
+
+    // regions and chunks (host)
+    constexpr int32_t kChunk = 4096;
+    std::vector<int32_t> start(H);
+    std::vector<Chunk> chunks;
+    {
+        int64_t acc = 0;
+        for (int32_t h = 0; h < H; ++h) {
+            start[h] = (int32_t)acc;
+            const int32_t n = A.np_halo[h];
+            for (int32_t b0 = 0; b0 < n; b0 += kChunk)
+                chunks.push_back(Chunk{h, (int32_t)acc + b0, (int32_t)acc + std::min(n, b0 + kChunk)});
+            acc += n;
+        }
+    }
+    const uint32_t nch = (uint32_t)chunks.size();
+    Chunk* d_chunks = C.chunks.as<Chunk>(nch);
+    PYFOF_HIP(hipMemcpyAsync(d_chunks, chunks.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, s));
+    int32_t* d_start = C.regstart.as<int32_t>((size_t)H + 1);
+    PYFOF_HIP(hipMemcpyAsync(d_start, start.data(), sizeof(int32_t) * H, hipMemcpyHostToDevice, s));
+    unsigned long long* d_bb = C.bbox.as<unsigned long long>(6 * (size_t)H);
+    {
+        std::vector<unsigned long long> init(6 * (size_t)H);
+        for (int32_t h = 0; h < H; ++h)
+            for (int d = 0; d < 3; ++d) {
+                init[6 * h + d] = dbl_to_ordered(INFINITY);
+                init[6 * h + 3 + d] = dbl_to_ordered(-INFINITY);
+            }
+        PYFOF_HIP(hipMemcpyAsync(d_bb, init.data(), 8 * init.size(), hipMemcpyHostToDevice, s));
+        const double Lh = 0.5 * A.L;
+        k_bbox<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_start, d_bb, A.L, Lh);
+        PYFOF_LAUNCH_CHECK();
+        std::vector<unsigned long long> bb(6 * (size_t)H);
+        PYFOF_HIP(hipMemcpyAsync(bb.data(), d_bb, 8 * bb.size(), hipMemcpyDeviceToHost, s));
+        PYFOF_HIP(hipStreamSynchronize(s));
+        init.swap(bb);
+        // geometry per region (host), reference rule + sub-cell plan
+        std::vector<RegionDev> regs(H);
+        std::vector<uint32_t> cell_off(H + 1);
+        uint64_t ncell = 0;
+        int32_t max_sub = 1;
+        double amax = 0.0;
+        for (int32_t h = 0; h < H; ++h) {
+            for (int d = 0; d < 3; ++d)
+                if (A.np_halo[h] > 0)
+                    amax = std::max(amax, std::max(fabs(ordered_to_dbl(init[6 * h + d])),
+                                                   fabs(ordered_to_dbl(init[6 * h + 3 + d]))));
+        }
+        // rounding margin: a few float ulps at the largest coordinate plus a
+        // relative slack on b; covers every float rounding of the predicate
+        const double m = 8.0 * std::max(ulp_f32(amax), ulp_f32(A.L)) + 1e-7 * A.b;
+        for (int32_t h = 0; h < H; ++h) {
+            RegionDev& g = regs[h];
+            memset(&g, 0, sizeof(g));
+            g.start = start[h];
+            g.np = A.np_halo[h];
+            g.skip = !(g.np >= A.np_min && g.np > 0);
+            if (g.skip) {
+                for (int d = 0; d < 3; ++d) { g.n[d] = 1; g.k[d] = 1; g.cell[d] = 1.0; g.sub[d] = 1.0; }
+                g.cell_off = (uint32_t)ncell;
+                g.ncell = 1;
+                cell_off[h] = (uint32_t)ncell;
+                ncell += 1;
+                continue;
+            }
+            double bmin[3], bmax[3];
+            for (int d = 0; d < 3; ++d) {
+                bmin[d] = ordered_to_dbl(init[6 * h + d]);
+                bmax[d] = ordered_to_dbl(init[6 * h + 3 + d]);
+            }
+            reference_geometry(bmin, bmax, g.np, A.L, A.b, g);
+            g.prune = 1;
+            for (int d = 0; d < 3; ++d)
+                if (g.periodic[d] && g.n[d] < 4) g.prune = 0;
+            // sub-cells: side <= (b - m)/sqrt(3) makes each sub-cell a clique
+            const double smax = (A.b - 2.0 * m) / sqrt(3.0);
+            g.clique = smax > 0.0 ? 1 : 0;
+            for (int d = 0; d < 3; ++d) {
+                int32_t k = 1;
+                if (smax > 0.0) {
+                    const double kk = ceil(g.cell[d] / smax * (1.0 + 1e-12));
+                    k = kk > kMaxSubPerDim ? kMaxSubPerDim : (int32_t)kk;
+                    if (kk > kMaxSubPerDim) g.clique = 0;
+                }
+                g.k[d] = k < 1 ? 1 : k;
+                g.sub[d] = g.cell[d] / (double)g.k[d];
+                if (g.sub[d] > smax) g.clique = 0;
+            }
+            if (!g.prune) { g.k[0] = g.k[1] = g.k[2] = 1; g.clique = 0; }
+            const uint64_t nc = (uint64_t)g.n[0] * (uint64_t)g.n[1] * (uint64_t)g.n[2];
+            g.cell_off = (uint32_t)ncell;
+            g.ncell = (uint32_t)nc;
+            cell_off[h] = (uint32_t)ncell;
+            ncell += nc;
+            if (ncell >= 0xffffffffull) {
+                set_error("total cell count exceeds 2^32 - 1");
+                return IERR_TOO_LARGE;
+            }
+            max_sub = std::max(max_sub, g.k[0] * g.k[1] * g.k[2]);
+        }
+        cell_off[H] = (uint32_t)ncell;
+        RegionDev* d_reg = C.regions.as<RegionDev>(H);
+        PYFOF_HIP(hipMemcpyAsync(d_reg, regs.data(), sizeof(RegionDev) * H, hipMemcpyHostToDevice, s));
+        uint32_t* d_coff = C.cell_off.as<uint32_t>((size_t)H + 1);
+        PYFOF_HIP(hipMemcpyAsync(d_coff, cell_off.data(), sizeof(uint32_t) * (H + 1),
+                                 hipMemcpyHostToDevice, s));
+        C.mark(kEvBbox);
+
+        LinkParams P;
+        P.L = A.L;
+        P.Lh = 0.5 * A.L;
+        P.b = A.b;
+        P.rp2 = A.b * A.b;
+        P.m = m;
+        P.R = A.b + m;
+        P.np_min = A.np_min;
+        P.nregions = H;
+        P.N = N;
+        P.ncell = (uint32_t)ncell;
+        P.sub_stride = max_sub + 1;
+
+        // ---- binning: keys, stable sort, cell_ptr, float CRS coordinates
+        uint32_t* k0 = C.keys0.as<uint32_t>(N);
+        uint32_t* k1 = C.keys1.as<uint32_t>(N);
+        uint32_t* v0 = C.vals0.as<uint32_t>(N);
+        uint32_t* v1 = C.vals1.as<uint32_t>(N);
+        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, A.L, P.Lh);
+        PYFOF_LAUNCH_CHECK();
+        const size_t tmp_bytes = std::max(radix_temp_bytes(N), scan_temp_bytes(ncell + 1));
+        void* tmp = C.temp.get(std::max(tmp_bytes, scan_temp_bytes(N) + 64));
+        const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s);
+        uint32_t* skeys = alt ? k1 : k0;
+        uint32_t* perm = alt ? v1 : v0;
+        uint32_t* cell_ptr = C.ends.as<uint32_t>(ncell + 1);
+        k_fill_u32<<<nblk(ncell + 1), kBlk, 0, s>>>(cell_ptr, ncell + 1, 0u);
+        PYFOF_LAUNCH_CHECK();
+        k_run_ends<<<nblk(N), kBlk, 0, s>>>(skeys, N, cell_ptr + 1);
+        PYFOF_LAUNCH_CHECK();
+        inclusive_max_scan_u32(cell_ptr + 1, cell_ptr + 1, ncell, tmp, s);
+        float4* xc = C.xc.as<float4>(N);
+        int32_t* parent = C.parent.as<int32_t>(N);
+        k_gather<<<nblk(N), kBlk, 0, s>>>(xp, perm, skeys, d_coff, d_reg, xc, parent, N, H, A.L, P.Lh);
+        PYFOF_LAUNCH_CHECK();
+        // dense cells and their sub-cell lists
+        int32_t* aux = C.aux.as<int32_t>(ncell);
+        uint32_t* dense_cells = C.dense.as<uint32_t>(N / kDenseMin + 1);
+        uint32_t* scal = C.scal.as<uint32_t>(64);
+        k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
+        PYFOF_LAUNCH_CHECK();
+        k_dense_mark<<<nblk(ncell), kBlk, 0, s>>>(cell_ptr, (uint32_t)ncell, d_coff, d_reg, H, aux,
+                                                   dense_cells, scal);
+        PYFOF_LAUNCH_CHECK();
+        uint32_t ndense = 0;
+        PYFOF_HIP(hipMemcpyAsync(&ndense, scal, 4, hipMemcpyDeviceToHost, s));
+        PYFOF_HIP(hipStreamSynchronize(s));
+        uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
+        uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
+        if (ndense > 0) {
+            k_dense_build<<<ndense, kBlk, 0, s>>>(dense_cells, cell_ptr, d_coff, d_reg, H, xc, sub_ptr,
+                                                  P.sub_stride, sub_list, parent);
+            PYFOF_LAUNCH_CHECK();
+        }
+        C.mark(kEvBin);
+        C.last_n = N;
+        C.last_ncell = ncell;
+        C.last_ndense = ndense;
+
+        // ---- linking
+        k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, d_reg, parent, P);
+        PYFOF_LAUNCH_CHECK();
+        C.mark(kEvLink);
+
+        // ---- roots, sizes, candidate numbering, outputs
+        int32_t* label = C.label.as<int32_t>(N);
+        uint32_t* size = C.size.as<uint32_t>(N);
+        uint32_t* flag = C.flag.as<uint32_t>(N);
+        uint32_t* cidx = C.cidx.as<uint32_t>(N);
+        k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, N);
+        PYFOF_LAUNCH_CHECK();
+        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(size, N, 0u);
+        PYFOF_LAUNCH_CHECK();
+        k_count<<<nblk(N), kBlk, 0, s>>>(label, N, size);
+        PYFOF_LAUNCH_CHECK();
+        k_cand_flags<<<nblk(N), kBlk, 0, s>>>(label, size, N, A.np_min, flag);
+        PYFOF_LAUNCH_CHECK();
+        exclusive_scan_u32(flag, cidx, N, scal + 1, tmp, s);
+        int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
+        int32_t* o_id = dev_mem ? A.halo_id : C.out_id.as<int32_t>(N);
+        int32_t* o_root = nullptr;
+        if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
+        uint32_t* anchor = C.anchor.as<uint32_t>(N);
+        int32_t* hreg = C.hregion.as<int32_t>(N);
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, xc, d_reg, N, A.np_min, o_in, o_id,
+                                         o_root, anchor, hreg);
+        PYFOF_LAUNCH_CHECK();
+        uint32_t nh = 0;
+        PYFOF_HIP(hipMemcpyAsync(&nh, scal + 1, 4, hipMemcpyDeviceToHost, s));
+        C.mark(kEvOut);
+
+        // ---- catalog (members grouped by halo, ascending index) + ang. mom.
+        PYFOF_HIP(hipStreamSynchronize(s));
+        C.cat_nh = nh;
+        const bool need_groups = C.want_catalog || (A.ang_mom && vp);
+        if (nh > 0 && need_groups) {
+            uint32_t* cf = C.flag.as<uint32_t>(N);
+            k_cat_flags<<<nblk(N), kBlk, 0, s>>>(o_in, N, cf);
+            PYFOF_LAUNCH_CHECK();
+            exclusive_scan_u32(cf, cf, N, scal + 2, tmp, s);
+            uint32_t nmem = 0;
+            PYFOF_HIP(hipMemcpyAsync(&nmem, scal + 2, 4, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipStreamSynchronize(s));
+            uint32_t* ck = C.cat_keys.as<uint32_t>(nmem);
+            uint32_t* cv = C.cat_vals.as<uint32_t>(nmem);
+            uint32_t* ck2 = C.cat_keys2.as<uint32_t>(nmem);
+            uint32_t* cv2 = C.cat_vals2.as<uint32_t>(nmem);
+            k_cat_compact<<<nblk(N), kBlk, 0, s>>>(o_id, cf, N, ck, cv);
+            PYFOF_LAUNCH_CHECK();
+            const bool calt = radix_sort_pairs(ck, cv, ck2, cv2, nmem, bits_for(nh - 1), tmp, s);
+            uint32_t* mkeys = calt ? ck2 : ck;
+            uint32_t* members = calt ? cv2 : cv;
+            uint32_t* hoff = C.hoff.as<uint32_t>(nh);
+            k_fill_u32<<<nblk(nh), kBlk, 0, s>>>(hoff, nh, 0u);
+            PYFOF_LAUNCH_CHECK();
+            k_run_ends<<<nblk(nmem), kBlk, 0, s>>>(mkeys, nmem, hoff);
+            PYFOF_LAUNCH_CHECK();
+            inclusive_max_scan_u32(hoff, hoff, nh, tmp, s);
+            C.cat_nmem = nmem;
+            C.cat_members = members;
+            C.cat_hoff = hoff;
+            if (A.ang_mom && vp) {
+                double* am_h = C.am_h.as<double>(3 * (size_t)nh);
+                double* am = dev_mem ? A.ang_mom : C.am.as<double>(3 * (size_t)H);
+                k_halo_angmom<<<nh, kBlk, 0, s>>>(xp, vp, mp, hoff, members, anchor, am_h, A.L, P.Lh);
+                PYFOF_LAUNCH_CHECK();
+                k_region_angmom<<<H, kBlk, 0, s>>>(am_h, hreg, nh, am);
+                PYFOF_LAUNCH_CHECK();
+                if (!dev_mem)
+                    PYFOF_HIP(hipMemcpyAsync(A.ang_mom, am, 24 * (size_t)H, hipMemcpyDeviceToHost, s));
+            } else if (A.ang_mom) {
+                if (dev_mem) PYFOF_HIP(hipMemsetAsync(A.ang_mom, 0, 24 * (size_t)H, s));
+                else memset(A.ang_mom, 0, 24 * (size_t)H);
+            }
+        } else if (A.ang_mom) {
+            if (dev_mem) PYFOF_HIP(hipMemsetAsync(A.ang_mom, 0, 24 * (size_t)H, s));
+            else memset(A.ang_mom, 0, 24 * (size_t)H);
+        }
+        C.mark(kEvCat);
+        if (!dev_mem) {
+            PYFOF_HIP(hipMemcpyAsync(A.in_halo, o_in, 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipMemcpyAsync(A.halo_id, o_id, 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+            if (A.root_out)
+                PYFOF_HIP(hipMemcpyAsync(A.root_out, o_root, 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+        }
+        PYFOF_HIP(hipStreamSynchronize(s));
+        if (A.verbose >= 2)
+            printf("[pyfof] N=%u regions=%d cells=%llu dense=%u halos=%u\n", N, H,
+                   (unsigned long long)ncell, ndense, nh);
+    }
+    return IERR_OK;
+}
+
+static std::mutex g_ctx_mu;
+static std::vector<Ctx*> g_default_ctx;
+
+static Ctx* default_ctx() {
+    int dev = 0;
+    PYFOF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if ((int)g_default_ctx.size() <= dev) g_default_ctx.resize(dev + 1, nullptr);
+    if (!g_default_ctx[dev]) g_default_ctx[dev] = new Ctx(dev);
+    return g_default_ctx[dev];
+}
+
+static int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
+    set_error("");
+    // fof_fortran.f90:110-123 -- outputs cleared, thread bound, empty input
+    const bool dev_mem = A.mem_kind == PYFOF_MEM_DEVICE;
+    if (A.mem_kind != PYFOF_MEM_HOST && !dev_mem) {
+        set_error("mem_kind must be PYFOF_MEM_HOST or PYFOF_MEM_DEVICE");
+        return IERR_INVALID;
+    }
+    if (n_threads > 256) {
+        set_error("n_threads exceeds 256");
+        return IERR_THREADS;
+    }
+    if (A.nhalo <= 0 || A.total_np <= 0) return IERR_OK;
+    int64_t tot = 0;
+    for (int32_t h = 0; h < A.nhalo; ++h) {
+        if (A.np_halo[h] < 0) { set_error("negative region size"); return IERR_INVALID; }
+        tot += A.np_halo[h];
+    }
+    if (tot != A.total_np) {
+        set_error("sum(np_halo) != total_np");
+        return IERR_INVALID;
+    }
+    if (!A.xp || !A.in_halo || !A.halo_id) {
+        set_error("null xp/in_halo/halo_id");
+        return IERR_INVALID;
+    }
+    if (!(A.L > 0.0) || !(A.b > 0.0)) {
+        set_error("L_Box and linking_length must be positive");
+        return IERR_INVALID;
+    }
+    if (A.bound_check) {
+        set_error("bound_check is not available in this build of the library");
+        return IERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(C->mu);
+    try {
+        PYFOF_HIP(hipSetDevice(C->device));
+        return run_fof(*C, A);
+    } catch (const HipError& e) {
+        set_error(e.what());
+        (void)hipStreamSynchronize(C->stream);
+        return e.code == hipErrorOutOfMemory ? IERR_ALLOC : IERR_HIP;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return IERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return IERR_HIP;
+    }
+}
+
+}  // namespace pyfof
+
+using namespace pyfof;
+
+struct pyfof_ctx {
+    Ctx* c;
+};
+
+extern "C" {
+
+const char* pyfof_last_error(void) { return g_last_error.c_str(); }
+
+const char* pyfof_version(void) { return "pyfof-mi355x 0.1 (gfx950)"; }
+
+pyfof_ctx* pyfof_ctx_create(int32_t device) {
+    try {
+        auto* h = new pyfof_ctx;
+        h->c = new Ctx(device);
+        return h;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void pyfof_ctx_destroy(pyfof_ctx* ctx) {
+    if (!ctx) return;
+    delete ctx->c;
+    delete ctx;
+}
+
+void* pyfof_ctx_stream(pyfof_ctx* ctx) { return ctx ? (void*)ctx->c->stream : nullptr; }
+
+int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const double* mp,
+                      const int32_t* np_halo, int32_t nhalo, double L_Box, double linking_length,
+                      int32_t np_halo_min, int32_t bound_check, int32_t verbose, double softening,
+                      double a_val, double b_val, double s_15_val, int32_t n_threads,
+                      int32_t total_np, int64_t ng_val, double ng_global_val, int32_t* in_halo,
+                      int32_t* halo_id, double* ang_mom, int32_t* root_out, int32_t mem_kind) {
+    (void)softening; (void)a_val; (void)b_val; (void)s_15_val; (void)ng_val; (void)ng_global_val;
+    if (!ctx) { set_error("null context"); return IERR_INVALID; }
+    CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
+               verbose, total_np, in_halo, halo_id, ang_mom, root_out, mem_kind};
+    return fof_entry(ctx->c, A, n_threads);
+}
+
+int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const int32_t* np_halo,
+                  int32_t nhalo, double L_Box, double linking_length, int32_t np_halo_min,
+                  int32_t bound_check, int32_t verbose, double softening, double a_val,
+                  double b_val, double s_15_val, int32_t n_threads, int32_t total_np,
+                  int64_t ng_val, double ng_global_val, int32_t* in_halo, int32_t* halo_id,
+                  double* ang_mom, int32_t mem_kind) {
+    (void)softening; (void)a_val; (void)b_val; (void)s_15_val; (void)ng_val; (void)ng_global_val;
+    Ctx* C = nullptr;
+    try {
+        C = default_ctx();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return IERR_HIP;
+    }
+    CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
+               verbose, total_np, in_halo, halo_id, ang_mom, nullptr, mem_kind};
+    return fof_entry(C, A, n_threads);
+}
+
+int32_t pyfof_catalog_size(pyfof_ctx* ctx, int32_t* nhalo_found, int64_t* n_members) {
+    if (!ctx) return IERR_INVALID;
+    if (!ctx->c->cat_hoff && ctx->c->cat_nh > 0) {
+        set_error("catalog not built: enable PYFOF_OPT_CATALOG before the call");
+        return IERR_INVALID;
+    }
+    *nhalo_found = (int32_t)ctx->c->cat_nh;
+    *n_members = (int64_t)ctx->c->cat_nmem;
+    return IERR_OK;
+}
+
+int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, int32_t mem_kind) {
+    if (!ctx) return IERR_INVALID;
+    Ctx& C = *ctx->c;
+    std::lock_guard<std::mutex> lk(C.mu);
+    try {
+        PYFOF_HIP(hipSetDevice(C.device));
+        const uint32_t nh = C.cat_nh;
+        std::vector<uint32_t> ends(nh);
+        if (nh) {
+            PYFOF_HIP(hipMemcpyAsync(ends.data(), C.cat_hoff, 4 * (size_t)nh, hipMemcpyDeviceToHost, C.stream));
+            PYFOF_HIP(hipStreamSynchronize(C.stream));
+        }
+        std::vector<int64_t> off(nh + 1);
+        off[0] = 0;
+        for (uint32_t h = 0; h < nh; ++h) off[h + 1] = ends[h];
+        const hipMemcpyKind k_off = mem_kind == PYFOF_MEM_DEVICE ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+        PYFOF_HIP(hipMemcpyAsync(offsets, off.data(), 8 * (size_t)(nh + 1), k_off, C.stream));
+        if (C.cat_nmem) {
+            const hipMemcpyKind k_idx = mem_kind == PYFOF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+            PYFOF_HIP(hipMemcpyAsync(indices, C.cat_members, 4 * (size_t)C.cat_nmem, k_idx, C.stream));
+        }
+        PYFOF_HIP(hipStreamSynchronize(C.stream));
+        return IERR_OK;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return IERR_HIP;
+    }
+}
+
+int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
+    if (!ctx) return IERR_INVALID;
+    if (option == PYFOF_OPT_CATALOG) {
+        ctx->c->want_catalog = value != 0;
+        return IERR_OK;
+    }
+    set_error("unknown option");
+    return IERR_INVALID;
+}
+
+int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n) {
+    if (!ctx) return 0;
+    const Ctx& C = *ctx->c;
+    const int64_t v[5] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
+                          (int64_t)C.cat_nh, (int64_t)C.cat_nmem};
+    int32_t w = 0;
+    for (; w < n && w < 5; ++w) out[w] = v[w];
+    return w;
+}
+
+int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n) {
+    if (!ctx) return 0;
+    Ctx& C = *ctx->c;
+    int32_t w = 0;
+    for (int e = 1; e < kNumEv && w < n; ++e, ++w) {
+        ms[w] = 0.0;
+        if (!C.ev_recorded[e] || !C.ev_recorded[e - 1]) continue;
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, C.ev[e - 1], C.ev[e]) == hipSuccess) ms[w] = t;
+    }
+    return w;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// Device-wide exclusive scan and stable LSD radix sort (keys + values, u32).
+//
+// Both work on tiles of 4096 elements per 256-thread workgroup (4 waves of 64).
+// The radix sort is the classic reduce-then-scan form: a per-tile digit
+// histogram, one device scan of the digit-major [digit][tile] table, then a
+// scatter in which each tile ranks its items stably (lane order within a wave
+// via 8 ballots, wave order within a round, rounds in item order).
+#include "prims.hpp"
+
+namespace pyfof {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kIpt = 16;
+constexpr int kTile = kBlock * kIpt;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+struct OpSum {
+    static __device__ __host__ uint32_t id() { return 0u; }
+    static __device__ uint32_t f(uint32_t a, uint32_t b) { return a + b; }
+};
+struct OpMax {
+    static __device__ __host__ uint32_t id() { return 0u; }
+    static __device__ uint32_t f(uint32_t a, uint32_t b) { return a > b ? a : b; }
+};
+
+template <class Op>
+__device__ inline uint32_t wave_reduce(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = Op::f(v, __shfl_down(v, o, kWave));
+    return v;
+}
+
+template <class Op>
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = __lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, kWave);
+        if (lane >= o) v = Op::f(v, t);
+    }
+    return v;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                        uint32_t* __restrict__ partial) {
+    __shared__ uint32_t wsum[kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t s = Op::id();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i < n) s = Op::f(s, in[i]);
+    }
+    s = wave_reduce<Op>(s);
+    if (__lane_id() == 0) wsum[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = Op::id();
+        for (int w = 0; w < kBlock / kWave; ++w) t = Op::f(t, wsum[w]);
+        partial[blockIdx.x] = t;
+    }
+}
+
+// LDS index with one pad word per 16 to keep the blocked reads conflict-free.
+__device__ inline int pad_idx(int e) { return e + (e >> 4); }
+
+// Scans one tile per block. offsets[block] (exclusive prefix of the tiles
+// before it) is folded in; inclusive selects the inclusive form.
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                      const uint32_t* __restrict__ offsets,
+                                                      uint32_t* __restrict__ total, int inclusive) {
+    __shared__ uint32_t tile[kTile + kTile / 16];
+    __shared__ uint32_t wtot[kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        int e = j * kBlock + t;
+        uint64_t i = base + e;
+        tile[pad_idx(e)] = i < n ? in[i] : Op::id();
+    }
+    __syncthreads();
+    uint32_t loc[kIpt];
+    uint32_t s = Op::id();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const uint32_t v = tile[pad_idx(t * kIpt + j)];
+        loc[j] = inclusive ? Op::f(s, v) : s;
+        s = Op::f(s, v);
+    }
+    const uint32_t incl = wave_inclusive_scan<Op>(s);
+    const uint32_t excl_in_wave = __shfl_up(incl, 1, kWave);
+    const int w = t / kWave;
+    if (__lane_id() == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint32_t woff = offsets ? offsets[blockIdx.x] : Op::id();
+    for (int k = 0; k < w; ++k) woff = Op::f(woff, wtot[k]);
+    const uint32_t texcl = __lane_id() == 0 ? woff : Op::f(woff, excl_in_wave);
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) tile[pad_idx(t * kIpt + j)] = Op::f(texcl, loc[j]);
+    if (total && blockIdx.x == gridDim.x - 1 && t == kBlock - 1) *total = Op::f(texcl, s);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        int e = j * kBlock + t;
+        uint64_t i = base + e;
+        if (i < n) out[i] = tile[pad_idx(e)];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restrict__ keys, uint64_t n,
+                                                       int shift, uint32_t* __restrict__ hist,
+                                                       uint32_t ntiles) {
+    __shared__ uint32_t h[kBlock / kWave][kRadix];
+    uint32_t* hf = &h[0][0];
+    for (int i = threadIdx.x; i < (kBlock / kWave) * kRadix; i += kBlock) hf[i] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const int w = threadIdx.x / kWave;
+#pragma unroll 4
+    for (int j = 0; j < kIpt; ++j) {
+        uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[w][(keys[i] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kRadix; d += kBlock) {
+        uint32_t c = 0;
+        for (int k = 0; k < kBlock / kWave; ++k) c += h[k][d];
+        hist[(uint64_t)d * ntiles + blockIdx.x] = c;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ vals,
+                                                          uint32_t* __restrict__ okeys,
+                                                          uint32_t* __restrict__ ovals, uint64_t n,
+                                                          int shift,
+                                                          const uint32_t* __restrict__ hist,
+                                                          uint32_t ntiles) {
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wcnt[kBlock / kWave][kRadix];
+    uint32_t* wf = &wcnt[0][0];
+    const int t = threadIdx.x, w = t / kWave;
+    for (int d = t; d < kRadix; d += kBlock) run[d] = hist[(uint64_t)d * ntiles + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const uint64_t lt = lanemask_lt();
+    for (int j = 0; j < kIpt; ++j) {
+        for (int i = t; i < (kBlock / kWave) * kRadix; i += kBlock) wf[i] = 0;
+        __syncthreads();
+        const uint64_t idx = base + (uint64_t)j * kBlock + t;
+        const bool valid = idx < n;
+        const uint32_t k = valid ? keys[idx] : 0u;
+        const uint32_t v = valid ? vals[idx] : 0u;
+        const uint32_t d = (k >> shift) & (kRadix - 1);
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bb = 0; bb < kRadixBits; ++bb) {
+            const bool bit = (d >> bb) & 1u;
+            const uint64_t bal = __ballot(valid && bit);
+            m &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(m & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+        __syncthreads();
+        for (int dd = t; dd < kRadix; dd += kBlock) {
+            uint32_t r = run[dd];
+#pragma unroll
+            for (int k2 = 0; k2 < kBlock / kWave; ++k2) {
+                const uint32_t c = wcnt[k2][dd];
+                wcnt[k2][dd] = r;
+                r += c;
+            }
+            run[dd] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = wcnt[w][d] + rank;
+            okeys[dst] = k;
+            ovals[dst] = v;
+        }
+        __syncthreads();
+    }
+}
+
+size_t scan_levels_bytes(uint64_t n) {
+    size_t b = 0;
+    while (n > (uint64_t)kTile) {
+        n = div_up(n, kTile);
+        b += ((n + 63) / 64) * 64 * sizeof(uint32_t);
+    }
+    return b + 256;
+}
+
+template <class Op>
+void scan_rec(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, uint32_t* temp,
+              int inclusive, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t nb = div_up(n, kTile);
+    if (nb == 1) {
+        k_scan_down<Op><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total, inclusive);
+        PYFOF_LAUNCH_CHECK();
+        return;
+    }
+    uint32_t* partial = temp;
+    uint32_t* next = temp + ((nb + 63) / 64) * 64;
+    k_scan_reduce<Op><<<nb, kBlock, 0, s>>>(in, n, partial);
+    PYFOF_LAUNCH_CHECK();
+    scan_rec<Op>(partial, partial, nb, nullptr, next, 0, s);
+    k_scan_down<Op><<<nb, kBlock, 0, s>>>(in, out, n, partial, total, inclusive);
+    PYFOF_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+int bits_for(uint64_t max_key) {
+    int b = 0;
+    while (b < 64 && (max_key >> b) != 0) ++b;
+    return b;
+}
+
+size_t scan_temp_bytes(uint64_t n) { return scan_levels_bytes(n); }
+
+size_t radix_temp_bytes(uint64_t n) {
+    const uint64_t ntiles = n == 0 ? 1 : div_up(n, kTile);
+    const uint64_t m = ntiles * kRadix;
+    return ((m * sizeof(uint32_t) + 255) / 256) * 256 + scan_levels_bytes(m);
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* temp,
+                        hipStream_t s) {
+    scan_rec<OpSum>(in, out, n, total, static_cast<uint32_t*>(temp), 0, s);
+}
+
+void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp,
+                            hipStream_t s) {
+    scan_rec<OpMax>(in, out, n, nullptr, static_cast<uint32_t*>(temp), 1, s);
+}
+
+bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
+                      uint64_t n, int bits, void* temp, hipStream_t s) {
+    if (n <= 1 || bits <= 0) return false;
+    const uint32_t ntiles = div_up(n, kTile);
+    const uint64_t m = (uint64_t)ntiles * kRadix;
+    uint32_t* hist = static_cast<uint32_t*>(temp);
+    uint32_t* stemp = hist + ((m * sizeof(uint32_t) + 255) / 256) * 256 / sizeof(uint32_t);
+    uint32_t *ik = keys, *iv = vals, *ok = alt_keys, *ov = alt_vals;
+    bool in_alt = false;
+    for (int shift = 0; shift < bits; shift += kRadixBits) {
+        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ik, n, shift, hist, ntiles);
+        PYFOF_LAUNCH_CHECK();
+        scan_rec<OpSum>(hist, hist, m, nullptr, stemp, 0, s);
+        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);
+        PYFOF_LAUNCH_CHECK();
+        uint32_t* tk = ik; ik = ok; ok = tk;
+        uint32_t* tv = iv; iv = ov; ov = tv;
+        in_alt = !in_alt;
+    }
+    return in_alt;
+}
+
+}  // namespace pyfof

@@ -1,0 +1,161 @@
+"""Synthetic particle sets for the benchmark workloads.
+
+All generators are seeded (numpy ``default_rng``) and return positions as a
+(3, N) float64 array in Fortran order, the layout the reference's native entry
+point takes (py_fof/__init__.py:210).
+
+Workloads:
+  * ``clumpy_plummer`` -- 262,144 particles in a 64^3 box: 60 % in 400 Plummer
+    clumps (scale radius 0.3, Pareto(alpha=1, min 20) member counts), 40 %
+    uniform background.
+  * ``clumpy_nfw`` -- 134,217,728 particles in a 512^3 box: 50 % in 250,000
+    NFW clumps (c = 8, Pareto(alpha=0.9) counts clipped to [20, 2e6],
+    r_vir = 0.6 (n/100)^(1/3)), 50 % uniform background. Any size can be asked
+    for; the clump count scales with it.
+"""
+import numpy as np
+
+__all__ = [
+    "pareto_counts",
+    "plummer_radii",
+    "nfw_radii",
+    "isotropic",
+    "clumpy_plummer",
+    "clumpy_nfw",
+]
+
+
+def _largest_remainder(weights, total, nmin):
+    """Integer counts >= nmin summing to ``total``, proportional to weights."""
+    w = np.asarray(weights, dtype=np.float64)
+    k = len(w)
+    if total < nmin * k:
+        raise ValueError("total too small for the per-clump minimum")
+    extra = total - nmin * k
+    share = w / w.sum() * extra
+    base = np.floor(share).astype(np.int64)
+    rem = extra - int(base.sum())
+    if rem > 0:
+        order = np.argsort(-(share - base), kind="stable")
+        base[order[:rem]] += 1
+    return base + nmin
+
+
+def pareto_counts(rng, k, alpha, total, nmin=20, nmax=None):
+    """``k`` member counts from Pareto(alpha) with minimum ``nmin``, rescaled
+    to sum to ``total`` and, when ``nmax`` is given, clipped to [nmin, nmax]."""
+    raw = nmin * (1.0 + rng.pareto(alpha, k))
+    if nmax is None:
+        return _largest_remainder(raw - nmin + 1e-12, total, nmin)
+    w = np.clip(raw, nmin, nmax)
+    for _ in range(64):
+        w = np.clip(w * (total / w.sum()), nmin, nmax)
+        if abs(w.sum() - total) < 0.5:
+            break
+    c = np.floor(w).astype(np.int64)
+    c = np.clip(c, nmin, nmax)
+    diff = int(total - c.sum())
+    if diff != 0:
+        # put the leftover on the clumps with the most headroom
+        room = (nmax - c) if diff > 0 else (c - nmin)
+        order = np.argsort(-room, kind="stable")
+        step = 1 if diff > 0 else -1
+        i = 0
+        while diff != 0:
+            j = order[i % k]
+            if nmin <= c[j] + step <= nmax:
+                c[j] += step
+                diff -= step
+            i += 1
+    return c
+
+
+def isotropic(rng, n, dtype=np.float64):
+    """Unit vectors, (3, n)."""
+    mu = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    s = np.sqrt(1.0 - mu * mu)
+    return np.stack([s * np.cos(phi), s * np.sin(phi), mu]).astype(dtype, copy=False)
+
+
+def plummer_radii(rng, n, a):
+    u = rng.uniform(1e-12, 1.0 - 1e-12, n)
+    return a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+
+
+_NFW_TAB = {}
+
+
+def nfw_radii(rng, n, c, r_vir):
+    """Radii inside r_vir of NFW profiles with concentration ``c`` (scalar or
+    per-particle array) by inverting m(x) = ln(1+x) - x/(1+x) on [0, c]."""
+    c = np.broadcast_to(np.asarray(c, dtype=np.float64), (n,))
+    u = rng.uniform(0.0, 1.0, n)
+    cu = np.unique(c)
+    x = np.empty(n)
+    for cv in cu:
+        tab = _NFW_TAB.get(float(cv))
+        if tab is None:
+            xs = np.concatenate([[0.0], np.geomspace(1e-6, cv, 8192)])
+            ms = np.log1p(xs) - xs / (1.0 + xs)
+            tab = (xs, ms / ms[-1])
+            _NFW_TAB[float(cv)] = tab
+        sel = c == cv if len(cu) > 1 else slice(None)
+        x[sel] = np.interp(u[sel], tab[1], tab[0])
+    return x / c * r_vir
+
+
+def _wrap(x, L):
+    x = np.mod(x, L)
+    x[x >= L] -= L
+    return x
+
+
+def clumpy_plummer(n=262144, L=64.0, n_clumps=400, frac=0.6, a=0.3, seed=0):
+    """Plummer-clump + uniform background set (the 262,144-particle workload)."""
+    rng = np.random.default_rng(seed)
+    n_cl = int(round(frac * n))
+    counts = pareto_counts(rng, n_clumps, 1.0, n_cl, nmin=20)
+    centres = rng.uniform(0.0, L, (3, n_clumps))
+    owner = np.repeat(np.arange(n_clumps), counts)
+    r = plummer_radii(rng, n_cl, a)
+    pos_cl = centres[:, owner] + r * isotropic(rng, n_cl)
+    pos_bg = rng.uniform(0.0, L, (3, n - n_cl))
+    xp = np.concatenate([pos_cl, pos_bg], axis=1)
+    return np.asfortranarray(_wrap(xp, L))
+
+
+def clumpy_nfw(n=134217728, L=512.0, n_clumps=None, frac=0.5, c=8.0, seed=2,
+               nmax=2_000_000, alpha=0.9):
+    """NFW-clump + uniform background set (the 134,217,728-particle workload
+    at the defaults). ``n_clumps`` defaults to 250,000 * n / 2^27."""
+    rng = np.random.default_rng(seed)
+    if n_clumps is None:
+        n_clumps = max(1, int(round(250_000 * n / 134_217_728)))
+    n_cl = int(round(frac * n))
+    nmax = min(nmax, max(20, n_cl))
+    counts = pareto_counts(rng, n_clumps, alpha, n_cl, nmin=20, nmax=nmax)
+    centres = rng.uniform(0.0, L, (3, n_clumps))
+    r_vir = 0.6 * (counts / 100.0) ** (1.0 / 3.0)
+    xp = np.empty((3, n), dtype=np.float64, order="F")
+    # clump members, chunked to bound temporaries
+    start = 0
+    chunk = 1 << 22
+    cum = np.concatenate([[0], np.cumsum(counts)])
+    ci = 0
+    while start < n_cl:
+        stop = min(n_cl, start + chunk)
+        # clumps overlapping [start, stop)
+        c0 = np.searchsorted(cum, start, side="right") - 1
+        c1 = np.searchsorted(cum, stop, side="left")
+        lo = np.maximum(cum[c0:c1], start)
+        hi = np.minimum(cum[c0 + 1:c1 + 1], stop)
+        owner = np.repeat(np.arange(c0, c1), hi - lo)
+        m = stop - start
+        r = nfw_radii(rng, m, c, r_vir[owner])
+        xp[:, start:stop] = centres[:, owner] + r * isotropic(rng, m)
+        start = stop
+        ci += 1
+    xp[:, n_cl:] = rng.uniform(0.0, L, (3, n - n_cl))
+    xp[:, :n_cl] = _wrap(xp[:, :n_cl], L)
+    return xp

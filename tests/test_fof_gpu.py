@@ -1,0 +1,194 @@
+"""GPU parity of the FoF path against the oracle (bit-exact partition).
+
+Every case compares, for the same seeded input, the HIP library (through
+the C-ABI, via py_fof_amd._fof_core) with the C oracle:
+  * roots: per particle, the input index of the first CRS member of its
+    component -- the canonical label of the whole partition;
+  * in_halo and halo_id (numbering order included), exactly;
+  * the fof_with_info catalog against the host construction of the reference
+    (py_fof/__init__.py:481-507) applied to the oracle's halo_id;
+  * per-region angular momentum to rtol 1e-9 (f64, summation order differs).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import chain_halo
+from oracle import oracle_fof
+
+import py_fof_amd
+from py_fof_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _run_both(xps, L, b, np_min, vps=None, mps=None, am=False):
+    xp = np.asfortranarray(np.hstack(xps))
+    nh = [x.shape[1] for x in xps]
+    vp = np.asfortranarray(np.hstack(vps)) if vps is not None else None
+    mp = np.concatenate(mps) if mps is not None else None
+    _say(f"case: N={xp.shape[1]} regions={len(nh)} L={L} b={b} np_min={np_min}")
+    in_h, hid, ang, ex = py_fof_amd._fof_core(
+        xps, vp_list=vps, L_Box=L, linking_length=b, np_halo_min=np_min,
+        mp_list=mp, return_ang_mom=am, _want_roots=True, _want_catalog=True)
+    o_in, o_hid, o_am, o_lab = oracle_fof(xp, nh, L, b, np_min, vp=vp, mp=mp,
+                                          n_threads=2, with_labels=True)
+    return (in_h, hid, ang, ex), (o_in, o_hid, o_am, o_lab)
+
+
+def _check(gpu, ref, am=False):
+    (in_h, hid, ang, ex), (o_in, o_hid, o_am, o_lab) = gpu, ref
+    assert np.array_equal(ex["roots"], o_lab), "partition roots differ"
+    assert np.array_equal(in_h, o_in.astype(bool)), "in_halo differs"
+    assert np.array_equal(hid, o_hid), "halo_id differs"
+    # catalog == host construction from halo_id (py_fof/__init__.py:489-505)
+    off, idx = ex["catalog"]
+    valid = np.nonzero(o_hid > 0)[0]
+    order = np.argsort(o_hid[valid], kind="stable")
+    assert np.array_equal(idx, valid[order])
+    counts = np.bincount(o_hid[valid], minlength=o_hid.max() + 1)[1:] if len(valid) else []
+    assert np.array_equal(np.diff(off), counts)
+    if am:
+        scale = np.abs(o_am).max() + 1e-300
+        assert np.allclose(ang, o_am, rtol=1e-9, atol=1e-12 * scale)
+
+
+def test_library_loads():
+    lib = py_fof_amd._native.load()
+    assert b"gfx950" in lib.pyfof_version()
+
+
+def test_chain_halo_known_answer():
+    rng = np.random.default_rng(999)
+    xp, expect = chain_halo(200, 2.0, 400, rng)
+    in_h, am = py_fof_amd.fof(xp, L_Box=512.0, linking_length=2.0, np_halo_min=10,
+                              return_ang_mom=True)
+    assert np.array_equal(in_h, expect)
+    assert np.allclose(am, 0.0)
+
+
+def test_multiple_regions_known_answer():
+    shifts = [(-50.0, -50.0, 0.0), (50.0, -50.0, 0.0), (0.0, 50.0, 30.0)]
+    xs, es = [], []
+    for k, s in enumerate(shifts):
+        xp, e = chain_halo(100, 2.0, 200, np.random.default_rng(777 + k))
+        xs.append(xp + np.array(s)[:, None])
+        es.append(e)
+    in_h = py_fof_amd.fof(xs, L_Box=512.0, linking_length=2.0, np_halo_min=10)
+    assert np.array_equal(in_h, np.concatenate(es))
+    g, r = _run_both(xs, 512.0, 2.0, 10)
+    _check(g, r)
+
+
+def test_clumpy_small_periodic():
+    xp = synth.clumpy_plummer(n=32768, L=32.0, n_clumps=60, seed=5)
+    g, r = _run_both([xp], 32.0, 0.2, 10)
+    _check(g, r)
+
+
+def test_clumpy_plummer_262k_bit_exact():
+    xp = synth.clumpy_plummer()  # 262,144 particles, L = 64, seed 0
+    g, r = _run_both([xp], 64.0, 0.2, 10)
+    _check(g, r)
+
+
+def test_dense_clump_subcells_and_angmom():
+    # one very dense clump: dense cells, clique sub-cells; velocities + masses
+    rng = np.random.default_rng(21)
+    n = 30000
+    x = 100.0 + 0.3 * rng.standard_normal((3, n))
+    x = np.concatenate([x, rng.uniform(90, 110, (3, 5000))], axis=1)
+    v = rng.standard_normal(x.shape)
+    m = rng.uniform(0.5, 2.0, x.shape[1])
+    g, r = _run_both([np.asfortranarray(x)], 512.0, 0.05, 10, vps=[np.asfortranarray(v)],
+                     mps=[m], am=True)
+    _check(g, r, am=True)
+
+
+def test_nonperiodic_cells_equal_to_b():
+    # the reference's perf-test geometry: L_mesh clamps to b (cells == b)
+    rng = np.random.default_rng(3)
+    xs = [np.asfortranarray(rng.uniform(0, 12.0, (3, 10000)) + 50.0 + 150.0 * k) for k in range(3)]
+    g, r = _run_both(xs, 512.0, 2.0, 10)
+    _check(g, r)
+
+
+def test_many_regions_mixed():
+    rng = np.random.default_rng(8)
+    xs = []
+    for k in range(300):
+        n = int(rng.integers(1, 400))
+        c = rng.uniform(0, 512.0, 3)
+        xs.append(np.asfortranarray(np.mod(c[:, None] + 0.6 * rng.standard_normal((3, n)), 512.0)))
+    vs = [np.asfortranarray(rng.standard_normal(x.shape)) for x in xs]
+    g, r = _run_both(xs, 512.0, 0.2, 10, vps=vs, am=True)
+    _check(g, r, am=True)
+
+
+def test_slab_periodic_one_axis_and_tiny_grid():
+    rng = np.random.default_rng(5)
+    n = 6000
+    xp = np.stack([rng.uniform(0, 30.0, n), rng.uniform(0, 4.0, n), rng.uniform(0, 4.0, n)])
+    g, r = _run_both([np.asfortranarray(xp)], 30.0, 0.3, 5)
+    _check(g, r)
+    # a periodic box with only 3 cells per axis (no pruning path)
+    xp = rng.uniform(0, 3.0, (3, 4000))
+    g, r = _run_both([np.asfortranarray(xp)], 3.0, 0.9, 3)
+    _check(g, r)
+
+
+def test_edges_inclusive_wrap_and_thresholds():
+    # pair at exactly b (exactly representable): the predicate is inclusive
+    xp = np.asfortranarray(np.array([[1.0, 1.5, 40.0], [2.0, 2.0, 40.0], [3.0, 3.0, 40.0]]))
+    in_h = py_fof_amd.fof(xp, L_Box=512.0, linking_length=0.5, np_halo_min=2)
+    assert list(in_h) == [True, True, False]
+    # periodic wrap across the box edge
+    rng = np.random.default_rng(1)
+    bg = rng.uniform(0, 16.0, (3, 3000))
+    pair = np.array([[0.05, 15.95], [8.0, 8.0], [8.0, 8.0]])
+    xp = np.asfortranarray(np.concatenate([pair, bg], axis=1))
+    g, r = _run_both([xp], 16.0, 0.2, 2)
+    _check(g, r)
+    assert r[3][0] == r[3][1]
+    # np_halo_min <= 1: every particle is a halo; np_min larger than a region
+    xs = [np.asfortranarray(rng.uniform(0, 5, (3, 50))), np.asfortranarray(rng.uniform(20, 25, (3, 7)))]
+    for npmin in (0, 1, 8):
+        g, r = _run_both(xs, 512.0, 0.5, npmin)
+        _check(g, r)
+    # identical positions
+    xp = np.asfortranarray(np.repeat(np.array([[3.0], [4.0], [5.0]]), 100, axis=1))
+    g, r = _run_both([xp], 512.0, 0.2, 10)
+    _check(g, r)
+
+
+def test_empty_inputs():
+    res = py_fof_amd.fof_with_info(np.zeros((3, 0)), L_Box=512.0)
+    assert res["nhalo_found"] == 0 and len(res["in_halo"]) == 0
+    xp = np.asfortranarray(np.random.default_rng(0).uniform(0, 50, (3, 20)))
+    res = py_fof_amd.fof_with_info(xp, L_Box=512.0, linking_length=0.01)
+    assert res["nhalo_found"] == 0 and res["in_halo"].sum() == 0
+
+
+def test_fof_with_info_consistency():
+    xp = synth.clumpy_plummer(n=20000, L=20.0, n_clumps=30, seed=9)
+    res = py_fof_amd.fof_with_info(xp, L_Box=20.0, linking_length=0.2)
+    in_h = py_fof_amd.fof(xp, L_Box=20.0, linking_length=0.2)
+    assert np.array_equal(res["in_halo"], in_h)
+    allidx = np.concatenate(res["halo_indices"])
+    assert len(allidx) == len(np.unique(allidx)) == in_h.sum()
+    assert np.sum(res["np_per_halo"]) == in_h.sum()
+    assert res["nhalo_found"] == len(res["halo_indices"]) == len(res["bound_status"])
+
+
+def test_repeat_calls_identical():
+    xp = synth.clumpy_plummer(n=50000, L=36.0, n_clumps=80, seed=4)
+    a = py_fof_amd._fof_core(xp, L_Box=36.0, linking_length=0.2, _want_roots=True)
+    b = py_fof_amd._fof_core(xp, L_Box=36.0, linking_length=0.2, _want_roots=True)
+    assert np.array_equal(a[3]["roots"], b[3]["roots"])
+    assert np.array_equal(a[1], b[1])
