@@ -421,19 +421,22 @@ __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t*
     }
 }
 
-// candidate roots: root position with size >= np_min (fof_fortran.f90:866-872)
-__global__ void k_cand_flags(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
-                             uint32_t n, int32_t np_min, uint32_t* __restrict__ flag) {
+// Candidates, fof_fortran.f90:866-872: every CRS position whose component
+// size (0 for non-roots) reaches np_min, in position order. With
+// np_halo_min <= 0 this counts non-root positions too (empty candidates).
+__global__ void k_cand_flags(const uint32_t* __restrict__ size, uint32_t n, int32_t np_min,
+                             uint32_t* __restrict__ flag) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    flag[p] = (label[p] == (int32_t)p && (int64_t)size[p] >= (int64_t)np_min) ? 1u : 0u;
+    flag[p] = ((int64_t)size[p] >= (int64_t)np_min) ? 1u : 0u;
 }
 
-// in_halo / halo_id per particle (fof_fortran.f90:209-223); candidate anchors
+// in_halo / halo_id per particle (fof_fortran.f90:209-223); per candidate the
+// anchor particle (its first CRS member) and its region
 __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
-                        const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ perm,
-                        const float4* __restrict__ xc, const RegionDev* __restrict__ reg,
-                        uint32_t n, int32_t np_min,
+                        const uint32_t* __restrict__ flag, const uint32_t* __restrict__ cand_idx,
+                        const uint32_t* __restrict__ perm, const float4* __restrict__ xc,
+                        const RegionDev* __restrict__ reg, uint32_t n, int32_t np_min,
                         int32_t* __restrict__ in_halo, int32_t* __restrict__ halo_id,
                         int32_t* __restrict__ root_out, uint32_t* __restrict__ anchor,
                         int32_t* __restrict__ hregion) {
@@ -444,13 +447,11 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     const uint32_t i = perm[p];
     in_halo[i] = cand ? 1 : 0;
     halo_id[i] = cand ? (int32_t)cand_idx[r] + 1 : 0;
-    if (root_out) {
-        const int32_t regid = __float_as_int(xc[p].w);
-        root_out[i] = reg[regid].skip ? -1 : (int32_t)perm[r];
-    }
-    if (cand && r == (int32_t)p) {
-        anchor[cand_idx[r]] = i;
-        hregion[cand_idx[r]] = __float_as_int(xc[p].w);
+    const int32_t regid = __float_as_int(xc[p].w);
+    if (root_out) root_out[i] = reg[regid].skip ? -1 : (int32_t)perm[r];
+    if (flag[p]) {
+        anchor[cand_idx[p]] = i;
+        hregion[cand_idx[p]] = regid;
     }
 }
 
@@ -500,6 +501,10 @@ __global__ void __launch_bounds__(kBlk) k_halo_angmom(const double* __restrict__
     __shared__ double lds[7][kBlk / kWave];
     const uint32_t h = blockIdx.x;
     const uint32_t s = h == 0 ? 0u : hoff[h - 1], e = hoff[h];
+    if (s >= e) {  // an empty candidate (np_halo_min <= 0) adds nothing
+        if (threadIdx.x < 3) am_h[3 * (size_t)h + threadIdx.x] = 0.0;
+        return;
+    }
     const size_t a = anchor[h];
     const double xa[3] = {xp[3 * a], xp[3 * a + 1], xp[3 * a + 2]};
     double v[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -869,7 +874,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_LAUNCH_CHECK();
         k_count<<<nblk(N), kBlk, 0, s>>>(label, N, size);
         PYFOF_LAUNCH_CHECK();
-        k_cand_flags<<<nblk(N), kBlk, 0, s>>>(label, size, N, A.np_min, flag);
+        k_cand_flags<<<nblk(N), kBlk, 0, s>>>(size, N, A.np_min, flag);
         PYFOF_LAUNCH_CHECK();
         exclusive_scan_u32(flag, cidx, N, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
@@ -878,8 +883,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, xc, d_reg, N, A.np_min, o_in, o_id,
-                                         o_root, anchor, hreg);
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, flag, cidx, perm, xc, d_reg, N, A.np_min, o_in,
+                                         o_id, o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
         uint32_t nh = 0;
         PYFOF_HIP(hipMemcpyAsync(&nh, scal + 1, 4, hipMemcpyDeviceToHost, s));
