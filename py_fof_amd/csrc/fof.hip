@@ -180,10 +180,11 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
     const uint32_t* __restrict__ dense_cells, const uint32_t* __restrict__ cell_ptr,
     const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg, int32_t nreg,
     const float4* __restrict__ xc, uint32_t* __restrict__ sub_ptr, int32_t sub_stride,
-    uint32_t* __restrict__ sub_list, int32_t* __restrict__ parent) {
+    uint32_t* __restrict__ sub_list, int32_t* __restrict__ parent, uint4* __restrict__ subs,
+    uint32_t* __restrict__ nsubs) {
     __shared__ uint32_t cnt[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim + 1];
     __shared__ uint32_t minp[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim];
-    __shared__ uint32_t wsum[kBlk / kWave];
+    __shared__ uint32_t nonempty, base;
     const uint32_t c = dense_cells[blockIdx.x];
     const uint32_t r = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c);
     const RegionDev& g = reg[r];
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
     const int32_t K = k0 * k1 * k2;
     const uint32_t s = cell_ptr[c], e = cell_ptr[c + 1];
     for (int i = threadIdx.x; i < K; i += kBlk) { cnt[i] = 0; minp[i] = 0xffffffffu; }
+    if (threadIdx.x == 0) nonempty = 0;
     __syncthreads();
     for (uint32_t p = s + threadIdx.x; p < e; p += kBlk) {
         const float4 v = xc[p];
@@ -201,15 +203,17 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
         const int32_t sy = sub_coord((double)v.y, lo[1], g.sub[1], k1);
         const int32_t sz = sub_coord((double)v.z, lo[2], g.sub[2], k2);
         const int32_t si = sx + k0 * (sy + k1 * sz);
-        atomicAdd(&cnt[si], 1u);
+        if (atomicAdd(&cnt[si], 1u) == 0u) atomicAdd(&nonempty, 1u);
         atomicMin(&minp[si], p);
     }
     __syncthreads();
+    // the block's slots in the list of non-empty clique sub-cells
+    if (threadIdx.x == 0) base = g.clique ? atomicAdd(nsubs, nonempty) : 0u;
     // exclusive scan of cnt[0..K) by one wave per 64-entry stripe, serially over stripes
     if (threadIdx.x < kWave) {
         uint32_t carry = 0;
-        for (int base = 0; base < K; base += kWave) {
-            const int i = base + threadIdx.x;
+        for (int b0 = 0; b0 < K; b0 += kWave) {
+            const int i = b0 + threadIdx.x;
             const uint32_t v = i < K ? cnt[i] : 0u;
             uint32_t incl = v;
             for (int o = 1; o < kWave; o <<= 1) {
@@ -221,10 +225,18 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
         }
         if (threadIdx.x == 0) cnt[K] = carry;
     }
-    (void)wsum;
     __syncthreads();
     uint32_t* row = sub_ptr + (size_t)blockIdx.x * sub_stride;
     for (int i = threadIdx.x; i <= K; i += kBlk) row[i] = cnt[i];
+    __syncthreads();
+    if (threadIdx.x == 0) nonempty = 0;
+    __syncthreads();
+    if (g.clique) {
+        // one entry per non-empty sub-cell (any order)
+        for (int i = threadIdx.x; i < K; i += kBlk)
+            if (cnt[i + 1] > cnt[i])
+                subs[base + atomicAdd(&nonempty, 1u)] = make_uint4(blockIdx.x, (uint32_t)i, minp[i], r);
+    }
     __syncthreads();
     for (uint32_t p = s + threadIdx.x; p < e; p += kBlk) {
         const float4 v = xc[p];
@@ -359,7 +371,11 @@ __device__ __forceinline__ void link_particle(uint32_t p, const float4& me, cons
         double sh[3];
         if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
         const uint32_t jl = local_cell(g, jijk);
-        if (jl <= lc) continue;
+        // sparse-sparse pairs once (forward cells); every pair with a dense
+        // clique cell is taken here, from the sparse side, in both directions
+        const bool fwd = jl > lc;
+        if (!fwd && !g.clique) continue;
+        if (!fwd && aux[g.cell_off + jl] < 0) continue;
         if (!g.prune) {
             // tiny periodic grids: the same neighbour can be reached twice
             bool dup = false;
@@ -391,10 +407,389 @@ __global__ void __launch_bounds__(kBlk) k_link(const float4* __restrict__ xc,
     const RegionDev& g = reg[__float_as_int(me.w)];
     if (g.skip) return;
     const uint32_t c = skeys[p];
+    if (g.clique && aux[c] >= 0) return;  // handled per sub-cell by k_link_dense
     if (g.is_periodic)
         link_particle<true>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
     else
         link_particle<false>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
+}
+
+// Linking of dense cells whose sub-cells are cliques.
+//
+// Sub-cells tile the region on a fine grid (k per cell and axis, side
+// s <= (b - 2m)/sqrt 3), so a pair passing the predicate lies in fine cells at
+// most 2 apart per axis (2s > b + m). Each unordered pair of sub-cells is
+// visited once: a lexicographically positive offset inside one cell
+// (k_dense_intra), any offset into a coarse cell that is forward under the
+// reference's neighbour rule (k_dense_cross). A visit tests member pairs until
+// the first one that passes and then unites the two cliques.
+
+// forward offsets in [-2,2]^3 (lexicographically positive), as (dx,dy,dz)
+__constant__ int8_t c_fwd_off[62][3];
+
+__device__ __forceinline__ int32_t lds_find(int32_t* lp, int32_t x) {
+    for (;;) {
+        const int32_t p = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (p == x) return x;
+        const int32_t gp = __hip_atomic_load(&lp[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gp == p) return p;
+        __hip_atomic_store(&lp[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void lds_unite(int32_t* lp, int32_t a, int32_t b) {
+    for (;;) {
+        a = lds_find(lp, a);
+        b = lds_find(lp, b);
+        if (a == b) return;
+        const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        if (atomicCAS(&lp[hi], hi, lo) == hi) return;
+    }
+}
+
+template <bool PERIODIC>
+__device__ __forceinline__ bool cliques_linked(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1,
+                                               const uint32_t* __restrict__ sub_list,
+                                               const float4* __restrict__ xc, const LinkParams& P) {
+    for (uint32_t ta = a0; ta < a1; ++ta) {
+        const float4 xa = xc[sub_list[ta]];
+        for (uint32_t tb = b0; tb < b1; ++tb)
+            if (pair_linked<PERIODIC>(xa, xc[sub_list[tb]], P)) return true;
+    }
+    return false;
+}
+
+// One workgroup per dense cell: union-find over its sub-cells in LDS, then
+// every member's parent is set to the smallest position of its local
+// component (a valid DSU state: parents point to smaller positions). Offsets
+// are processed nearest first in classes separated by barriers, so that once
+// the close sub-cells have merged the far offsets are skipped on the root
+// check alone. Cells with at most kIntraStage members are staged in LDS.
+constexpr int kIntraStage = 1024;
+constexpr int kFwdClassEnd[5] = {3, 9, 13, 16, 62};
+
+template <bool PERIODIC>
+__device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
+                            const uint32_t* __restrict__ dense_cells, const uint32_t* __restrict__ cell_ptr,
+                            const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
+                            const float4* __restrict__ xc, int32_t* parent, int32_t* __restrict__ ncomp,
+                            int32_t* __restrict__ croot, int32_t* __restrict__ needs_sub,
+                            unsigned char* lds) {
+    const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
+    const int32_t K = k0 * k1 * k2;
+    uint32_t* rp = reinterpret_cast<uint32_t*>(lds);
+    int32_t* lp = reinterpret_cast<int32_t*>(rp + P.sub_stride);
+    uint32_t* mh = reinterpret_cast<uint32_t*>(lp + P.sub_stride);
+    uint32_t* list = mh + P.sub_stride;
+    uint32_t* nl = list + P.sub_stride;  // [0] list size, [1] component count
+    float4* pos = reinterpret_cast<float4*>(lds + (size_t)16 * ((4 * P.sub_stride + 2 + 3) / 4));
+    const uint32_t c = dense_cells[d];
+    const uint32_t cs = cell_ptr[c], n_c = cell_ptr[c + 1] - cs;
+    const bool staged = n_c <= (uint32_t)P.intra_stage;
+    const uint32_t* row = sub_ptr + (size_t)d * P.sub_stride;
+    if (threadIdx.x == 0) { nl[0] = 0; nl[1] = 0; }
+    for (int i = threadIdx.x; i <= K; i += kBlk) rp[i] = row[i];
+    if (staged)
+        for (uint32_t t = threadIdx.x; t < n_c; t += kBlk) pos[t] = xc[sub_list[cs + t]];
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += kBlk) {
+        lp[i] = i;
+        mh[i] = 0xffffffffu;
+        if (rp[i + 1] > rp[i]) list[atomicAdd(&nl[0], 1u)] = (uint32_t)i;
+    }
+    __syncthreads();
+    const uint32_t na = nl[0];
+    int cls_begin = 0;
+    for (int cls = 0; cls < 5; ++cls) {
+        const uint32_t noff = (uint32_t)(kFwdClassEnd[cls] - cls_begin);
+        for (uint32_t it = threadIdx.x; it < na * noff; it += kBlk) {
+            const int32_t A = (int32_t)list[it / noff];
+            const int oi = cls_begin + (int)(it % noff);
+            const int32_t ax = A % k0 + c_fwd_off[oi][0];
+            const int32_t ay = (A / k0) % k1 + c_fwd_off[oi][1];
+            const int32_t az = A / (k0 * k1) + c_fwd_off[oi][2];
+            if (ax < 0 || ax >= k0 || ay < 0 || ay >= k1 || az < 0 || az >= k2) continue;
+            const int32_t B = ax + k0 * (ay + k1 * az);
+            if (rp[B + 1] == rp[B]) continue;
+            if (lds_find(lp, A) == lds_find(lp, B)) continue;
+            bool hit = false;
+            if (staged) {
+                for (uint32_t ta = rp[A]; ta < rp[A + 1] && !hit; ++ta) {
+                    const float4 xa = pos[ta];
+                    for (uint32_t tb = rp[B]; tb < rp[B + 1]; ++tb)
+                        if (pair_linked<PERIODIC>(xa, pos[tb], P)) { hit = true; break; }
+                }
+            } else {
+                hit = cliques_linked<PERIODIC>(cs + rp[A], cs + rp[A + 1], cs + rp[B], cs + rp[B + 1],
+                                               sub_list, xc, P);
+            }
+            if (hit) lds_unite(lp, A, B);
+        }
+        cls_begin = kFwdClassEnd[cls];
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
+        const int32_t A = (int32_t)list[i];
+        // head of A: its smallest position (set as parent of every member)
+        const uint32_t h = (uint32_t)parent[sub_list[cs + rp[A]]];
+        const int32_t r = lds_find(lp, A);
+        atomicMin(&mh[r], h);
+        if (r == A) atomicAdd(&nl[1], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
+        const int32_t A = (int32_t)list[i];
+        const int32_t root = (int32_t)mh[lds_find(lp, A)];
+        for (uint32_t t = rp[A]; t < rp[A + 1]; ++t) parent[sub_list[cs + t]] = root;
+    }
+    if (threadIdx.x == 0) {
+        ncomp[d] = (int32_t)nl[1];
+        croot[d] = nl[1] == 1 ? (int32_t)mh[lds_find(lp, (int32_t)list[0])] : -1;
+        needs_sub[d] = nl[1] != 1 ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_dense_intra(const uint32_t* __restrict__ dense_cells,
+                                                      const uint32_t* __restrict__ cell_ptr,
+                                                      const uint32_t* __restrict__ cell_off,
+                                                      const RegionDev* __restrict__ reg, int32_t nreg,
+                                                      const uint32_t* __restrict__ sub_ptr,
+                                                      const uint32_t* __restrict__ sub_list,
+                                                      const float4* __restrict__ xc, int32_t* parent,
+                                                      int32_t* __restrict__ ncomp,
+                                                      int32_t* __restrict__ croot,
+                                                      int32_t* __restrict__ needs_sub, LinkParams P) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t d = blockIdx.x;
+    const uint32_t c = dense_cells[d];
+    const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+    if (!g.clique) {
+        if (threadIdx.x == 0) { ncomp[d] = -1; croot[d] = -1; needs_sub[d] = 0; }
+        return;
+    }
+    if (g.is_periodic)
+        dense_intra<true>(d, g, P, dense_cells, cell_ptr, sub_ptr, sub_list, xc, parent, ncomp, croot,
+                          needs_sub, lds);
+    else
+        dense_intra<false>(d, g, P, dense_cells, cell_ptr, sub_ptr, sub_list, xc, parent, ncomp, croot,
+                           needs_sub, lds);
+}
+
+// One wave per dense cell of a single local component: for each forward
+// neighbour dense cell that is also a single component, decide once whether
+// the two components link. If their roots differ, the sub-cell pairs across
+// the shared face/edge/corner (fine offset <= 2 per axis) are searched in
+// parallel, 64 at a time, until one pair of cliques links.
+template <bool PERIODIC>
+__device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
+                           const uint32_t* __restrict__ dense_cells, const uint32_t* __restrict__ cell_ptr,
+                           const int32_t* __restrict__ aux, const int32_t* __restrict__ ncomp,
+                           const int32_t* __restrict__ croot, int32_t* __restrict__ needs_sub,
+                           const uint32_t* __restrict__ sub_ptr,
+                           const uint32_t* __restrict__ sub_list, const float4* __restrict__ xc,
+                           int32_t* parent) {
+    const uint32_t c = dense_cells[d];
+    const uint32_t lc = c - g.cell_off;
+    int32_t ijk[3];
+    double lo_unused[3];
+    cell_lower_corner(g, lc, lo_unused, ijk);
+    const int32_t kk[3] = {g.k[0], g.k[1], g.k[2]};
+    const uint32_t* rowA = sub_ptr + (size_t)d * P.sub_stride;
+    const uint32_t csA = cell_ptr[c];
+    const int lane = __lane_id();
+    for (int o = 0; o < 27; ++o) {
+        if (o == 13) continue;
+        int32_t jijk[3];
+        double sh[3];
+        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
+        const uint32_t jl = local_cell(g, jijk);
+        if (jl <= lc) continue;
+        const uint32_t cB = g.cell_off + jl;
+        const int32_t dB = aux[cB];
+        if (dB < 0) continue;
+        if (ncomp[dB] != 1) {
+            if (lane == 0) needs_sub[d] = 1;
+            continue;
+        }
+        int32_t rA = dsu_find(parent, croot[d]);
+        int32_t rB = dsu_find(parent, croot[dB]);
+        if (rA == rB) continue;
+        // per axis: number of (sub-cell of A, sub-cell of B) coordinate pairs
+        const int32_t co[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+        int32_t npair[3];
+        for (int ax = 0; ax < 3; ++ax) npair[ax] = co[ax] == 0 ? 5 * kk[ax] : 3;
+        const uint32_t total = (uint32_t)npair[0] * (uint32_t)npair[1] * (uint32_t)npair[2];
+        const uint32_t* rowB = sub_ptr + (size_t)dB * P.sub_stride;
+        const uint32_t csB = cell_ptr[cB];
+        for (uint32_t base = 0; base < total; base += kWave) {
+            const uint32_t it = base + (uint32_t)lane;
+            bool hit = false;
+            if (it < total) {
+                uint32_t rem = it;
+                int32_t sa[3], sb[3];
+                bool valid = true;
+                for (int ax = 0; ax < 3; ++ax) {
+                    const uint32_t pi = rem % (uint32_t)npair[ax];
+                    rem /= (uint32_t)npair[ax];
+                    if (co[ax] == 0) {
+                        sa[ax] = (int32_t)(pi / 5u);
+                        sb[ax] = sa[ax] + (int32_t)(pi % 5u) - 2;
+                        if (sb[ax] < 0 || sb[ax] >= kk[ax]) valid = false;
+                    } else {
+                        // (k-1,0) (k-1,1) (k-2,0) toward +1; mirrored toward -1
+                        const int32_t pa[3] = {kk[ax] - 1, kk[ax] - 1, kk[ax] - 2};
+                        const int32_t pb[3] = {0, 1, 0};
+                        sa[ax] = co[ax] > 0 ? pa[pi] : kk[ax] - 1 - pa[pi];
+                        sb[ax] = co[ax] > 0 ? pb[pi] : kk[ax] - 1 - pb[pi];
+                    }
+                }
+                if (valid) {
+                    const uint32_t ia = (uint32_t)sa[0] + (uint32_t)kk[0] * ((uint32_t)sa[1] + (uint32_t)kk[1] * (uint32_t)sa[2]);
+                    const uint32_t ib = (uint32_t)sb[0] + (uint32_t)kk[0] * ((uint32_t)sb[1] + (uint32_t)kk[1] * (uint32_t)sb[2]);
+                    const uint32_t a0 = csA + rowA[ia], a1 = csA + rowA[ia + 1];
+                    const uint32_t b0 = csB + rowB[ib], b1 = csB + rowB[ib + 1];
+                    if (a0 < a1 && b0 < b1) hit = cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P);
+                }
+            }
+            const uint64_t hits = __ballot(hit);
+            if (hits) {
+                if (lane == __ffsll((long long)hits) - 1) dsu_unite(parent, rA, rB);
+                break;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kWave) k_dense_cell_cross(const uint32_t* __restrict__ dense_cells,
+                                                            uint32_t ndense,
+                                                            const uint32_t* __restrict__ cell_ptr,
+                                                            const uint32_t* __restrict__ cell_off,
+                                                            const RegionDev* __restrict__ reg, int32_t nreg,
+                                                            const int32_t* __restrict__ aux,
+                                                            const int32_t* __restrict__ ncomp,
+                                                            const int32_t* __restrict__ croot,
+                                                            int32_t* __restrict__ needs_sub,
+                                                            const uint32_t* __restrict__ sub_ptr,
+                                                            const uint32_t* __restrict__ sub_list,
+                                                            const float4* __restrict__ xc, int32_t* parent,
+                                                            LinkParams P) {
+    const uint32_t d = blockIdx.x;
+    if (d >= ndense || ncomp[d] != 1) return;
+    const uint32_t c = dense_cells[d];
+    const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+    if (g.is_periodic)
+        cell_cross<true>(d, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
+    else
+        cell_cross<false>(d, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
+}
+
+// One thread per non-empty clique sub-cell A of a cell with several local
+// components, or next to such a cell: offsets into forward dense coarse cells,
+// sub-cell against sub-cell. (Pairs with sparse cells are taken by k_link.)
+template <bool PERIODIC>
+__device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g, const LinkParams& P,
+                                            const uint32_t* __restrict__ dense_cells,
+                                            const uint32_t* __restrict__ cell_ptr,
+                                            const int32_t* __restrict__ aux,
+                                            const int32_t* __restrict__ ncomp,
+                                            const int32_t* __restrict__ croot,
+                                            const int32_t* __restrict__ needs_sub,
+                                            const uint32_t* __restrict__ sub_ptr,
+                                            const uint32_t* __restrict__ sub_list,
+                                            const float4* __restrict__ xc, int32_t* parent) {
+    const uint32_t d = ent.x, si = ent.y;
+    const uint32_t c = dense_cells[d];
+    const uint32_t lc = c - g.cell_off;
+    int32_t ijk[3];
+    double lo[3];
+    cell_lower_corner(g, lc, lo, ijk);
+    const int32_t kk[3] = {g.k[0], g.k[1], g.k[2]};
+    const int32_t sa[3] = {(int32_t)(si % (uint32_t)kk[0]), (int32_t)((si / (uint32_t)kk[0]) % (uint32_t)kk[1]),
+                           (int32_t)(si / (uint32_t)(kk[0] * kk[1]))};
+    // cells whose pairs were all decided per cell pair, and interior
+    // sub-cells (they reach no other cell), have nothing left to do
+    if (!needs_sub[d]) return;
+    bool interior = true;
+    for (int ax = 0; ax < 3; ++ax)
+        if (sa[ax] < 2 || sa[ax] > kk[ax] - 3) interior = false;
+    if (interior) return;
+    // per axis, for offsets -2..2: neighbour cell coordinate (or -1) and sub-coordinate
+    int32_t jt[3][5], st[3][5];
+    for (int ax = 0; ax < 3; ++ax) {
+        for (int o = 0; o < 5; ++o) {
+            int32_t f = sa[ax] + o - 2, j = ijk[ax];
+            if (f < 0) { f += kk[ax]; j -= 1; }
+            else if (f >= kk[ax]) { f -= kk[ax]; j += 1; }
+            if (j < 0 || j >= g.n[ax]) {
+                if (g.periodic[ax]) j = j < 0 ? j + g.n[ax] : j - g.n[ax];
+                else j = -1;
+            }
+            jt[ax][o] = j;
+            st[ax][o] = f;
+        }
+    }
+    const uint32_t* rowA = sub_ptr + (size_t)d * P.sub_stride;
+    const uint32_t cs = cell_ptr[c];
+    const uint32_t a0 = cs + rowA[si], a1 = cs + rowA[si + 1];
+    int32_t rA = dsu_find(parent, (int32_t)ent.z);
+    // pairs of single-component dense cells are decided per cell pair by
+    // k_dense_cell_cross; the last neighbour found joined is remembered
+    const bool single = ncomp[d] == 1;
+    int32_t same_cell = -1;
+    for (int oz = 0; oz < 5; ++oz) {
+        const int32_t jz = jt[2][oz];
+        if (jz < 0) continue;
+        for (int oy = 0; oy < 5; ++oy) {
+            const int32_t jy = jt[1][oy];
+            if (jy < 0) continue;
+            for (int ox = 0; ox < 5; ++ox) {
+                const int32_t jx = jt[0][ox];
+                if (jx < 0) continue;
+                if (jx == ijk[0] && jy == ijk[1] && jz == ijk[2]) continue;  // same cell: intra
+                const uint32_t jl = (uint32_t)jx + (uint32_t)g.n[0] * ((uint32_t)jy + (uint32_t)g.n[1] * (uint32_t)jz);
+                if (jl <= lc) continue;
+                const uint32_t cB = g.cell_off + jl;
+                const int32_t dB = aux[cB];
+                if (dB < 0) continue;  // sparse cell: below
+                if (single && ncomp[dB] == 1) continue;  // decided by k_dense_cell_cross
+                if (dB == same_cell) continue;
+                const uint32_t sb = (uint32_t)st[0][ox] + (uint32_t)kk[0] * ((uint32_t)st[1][oy] + (uint32_t)kk[1] * (uint32_t)st[2][oz]);
+                const uint32_t* rowB = sub_ptr + (size_t)dB * P.sub_stride;
+                const uint32_t bs = cell_ptr[cB];
+                const uint32_t b0 = bs + rowB[sb], b1 = bs + rowB[sb + 1];
+                if (b0 >= b1) continue;
+                const int32_t rB = dsu_find(parent, (int32_t)sub_list[b0]);
+                if (rB == rA) continue;
+                rA = dsu_find(parent, rA);
+                if (rB == rA) continue;
+                if (cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P)) rA = dsu_unite(parent, rA, rB);
+                if (ncomp[dB] == 1 && dsu_find(parent, croot[dB]) == rA) same_cell = dB;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ subs, uint32_t nsubs,
+                                                      const uint32_t* __restrict__ dense_cells,
+                                                      const uint32_t* __restrict__ cell_ptr,
+                                                      const int32_t* __restrict__ aux,
+                                                      const int32_t* __restrict__ ncomp,
+                                                      const int32_t* __restrict__ croot,
+                                                      const int32_t* __restrict__ needs_sub,
+                                                      const uint32_t* __restrict__ sub_ptr,
+                                                      const uint32_t* __restrict__ sub_list,
+                                                      const RegionDev* __restrict__ reg,
+                                                      const float4* __restrict__ xc, int32_t* parent,
+                                                      LinkParams P) {
+    const uint32_t t = blockIdx.x * kBlk + threadIdx.x;
+    if (t >= nsubs) return;
+    const uint4 ent = subs[t];
+    const RegionDev& g = reg[ent.w];
+    if (g.is_periodic)
+        dense_cross<true>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
+    else
+        dense_cross<false>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
 }
 
 __global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t n) {
@@ -602,7 +997,7 @@ struct Ctx {
     hipEvent_t ev[kNumEv] = {};
     bool ev_recorded[kNumEv] = {};
     DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
-        xc, parent, label, size, flag, cidx, aux, dense, sub_ptr, sub_list, temp, out_in, out_id,
+        xc, parent, label, size, flag, cidx, aux, dense, sub_ptr, sub_list, subs, ncomp, croot, needs_sub, temp, out_in, out_id,
         out_root, anchor, hregion, scal, cat_keys, cat_vals, cat_keys2, cat_vals2, hoff, am_h, am,
         pin;
     // last call's sizes and catalog
@@ -616,6 +1011,19 @@ struct Ctx {
 
     explicit Ctx(int dev) : device(dev) {
         PYFOF_HIP(hipSetDevice(dev));
+        // forward offsets of [-2,2]^3, nearest first (|o|^2 = 1, 2, 3, 4, ...)
+        int8_t off[62][3];
+        int n = 0;
+        for (int r2 = 1; r2 <= 12; ++r2)
+            for (int o = 63; o < 125; ++o) {
+                const int dx = o % 5 - 2, dy = (o / 5) % 5 - 2, dz = o / 25 - 2;
+                if (dx * dx + dy * dy + dz * dz != r2) continue;
+                off[n][0] = (int8_t)dx;
+                off[n][1] = (int8_t)dy;
+                off[n][2] = (int8_t)dz;
+                ++n;
+            }
+        PYFOF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fwd_off), off, sizeof(off)));
         PYFOF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
     }
@@ -625,7 +1033,7 @@ struct Ctx {
         if (stream) (void)hipStreamDestroy(stream);
         for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
                           &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
-                          &sub_ptr, &sub_list, &temp, &out_in, &out_id, &out_root, &anchor,
+                          &sub_ptr, &sub_list, &subs, &ncomp, &croot, &needs_sub, &temp, &out_in, &out_id, &out_root, &anchor,
                           &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
                           &am_h, &am, &pin})
             b->release();
@@ -811,6 +1219,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         P.N = N;
         P.ncell = (uint32_t)ncell;
         P.sub_stride = max_sub + 1;
+        P.intra_stage = 0;
 
         // ---- binning: keys, stable sort, cell_ptr, float CRS coordinates
         uint32_t* k0 = C.keys0.as<uint32_t>(N);
@@ -848,19 +1257,46 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipStreamSynchronize(s));
         uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
+        uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);
+        uint32_t nsubs = 0;
         if (ndense > 0) {
             k_dense_build<<<ndense, kBlk, 0, s>>>(dense_cells, cell_ptr, d_coff, d_reg, H, xc, sub_ptr,
-                                                  P.sub_stride, sub_list, parent);
+                                                  P.sub_stride, sub_list, parent, subs, scal + 3);
             PYFOF_LAUNCH_CHECK();
+            PYFOF_HIP(hipMemcpyAsync(&nsubs, scal + 3, 4, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipStreamSynchronize(s));
         }
         C.mark(kEvBin);
         C.last_n = N;
         C.last_ncell = ncell;
         C.last_ndense = ndense;
 
-        // ---- linking
+        // ---- linking. k_dense_intra writes the parents of dense-cell members
+        // directly (local components), so it runs before any atomic union.
+        int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
+        int32_t* croot = C.croot.as<int32_t>(std::max<uint32_t>(ndense, 1));
+        int32_t* needs_sub = C.needs_sub.as<int32_t>(std::max<uint32_t>(ndense, 1));
+        if (nsubs > 0) {
+            // LDS: sub-cell tables, plus staged positions within a 64 KiB budget
+            const size_t base = (size_t)16 * ((4 * P.sub_stride + 2 + 3) / 4);
+            P.intra_stage = (int32_t)std::min<size_t>(kIntraStage, base < 65536 ? (65536 - base) / 16 : 0);
+            const size_t lds = base + 16 * (size_t)P.intra_stage;
+            k_dense_intra<<<ndense, kBlk, lds, s>>>(dense_cells, cell_ptr, d_coff, d_reg, H, sub_ptr,
+                                                    sub_list, xc, parent, ncomp, croot, needs_sub, P);
+            PYFOF_LAUNCH_CHECK();
+        }
         k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, d_reg, parent, P);
         PYFOF_LAUNCH_CHECK();
+        if (nsubs > 0) {
+            k_dense_cell_cross<<<ndense, kWave, 0, s>>>(dense_cells, ndense, cell_ptr, d_coff, d_reg, H,
+                                                        aux, ncomp, croot, needs_sub, sub_ptr, sub_list,
+                                                        xc, parent, P);
+            PYFOF_LAUNCH_CHECK();
+            k_dense_cross<<<nblk(nsubs), kBlk, 0, s>>>(subs, nsubs, dense_cells, cell_ptr, aux, ncomp,
+                                                       croot, needs_sub, sub_ptr, sub_list, d_reg, xc,
+                                                       parent, P);
+            PYFOF_LAUNCH_CHECK();
+        }
         C.mark(kEvLink);
 
         // ---- roots, sizes, candidate numbering, outputs

@@ -16,6 +16,7 @@ struct LinkParams {
     int32_t np_min, nregions;
     uint32_t N, ncell;
     int32_t sub_stride;
+    int32_t intra_stage;  // dense-cell members staged in LDS by k_dense_intra
 };
 
 constexpr int kDenseMin = 48;     // cells with at least this many particles get sub-cells

@@ -111,6 +111,18 @@ def test_dense_clump_subcells_and_angmom():
     _check(g, r, am=True)
 
 
+def test_dense_clique_subcells_nonperiodic_and_coarse_k():
+    rng = np.random.default_rng(31)
+    # isolated dense blob: L_mesh clamps to b, two clique sub-cells per axis
+    x = 200.0 + 0.5 * rng.standard_normal((3, 20000))
+    g, r = _run_both([np.asfortranarray(x)], 512.0, 0.2, 10)
+    _check(g, r)
+    # periodic clumps with b = 0.35 in unit cells (k = 5 sub-cells per axis)
+    xp = synth.clumpy_plummer(n=65536, L=40.0, n_clumps=50, a=0.5, seed=12)
+    g, r = _run_both([xp], 40.0, 0.35, 10)
+    _check(g, r)
+
+
 def test_nonperiodic_cells_equal_to_b():
     # the reference's perf-test geometry: L_mesh clamps to b (cells == b)
     rng = np.random.default_rng(3)
