@@ -14,4 +14,8 @@ from .oracle import (  # noqa: F401
     oracle_candidates,
     oracle_geometry,
     load_oracle,
+    oracle_fof_bound,
+    oracle_green,
+    oracle_select,
+    oracle_depth,
 )

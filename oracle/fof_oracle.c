@@ -36,7 +36,7 @@
 #endif
 
 /* gfortran MODULO for reals: fmod, then shift into the sign of p. */
-static double f_modulo(double a, double p)
+double oracle_modulo(double a, double p)
 {
     double r = fmod(a, p);
     if (r != 0.0 && ((r < 0.0) != (p < 0.0))) r += p;
@@ -170,12 +170,7 @@ static int32_t cell_coord(double x, double o, double c, int32_t n)
     return imin(n, imax(1, i));
 }
 
-typedef struct {
-    int32_t ncand;
-    int32_t *cand_np;      /* [ncand] */
-    int32_t *member_ptr;   /* [ncand+1], 0-based offsets */
-    int32_t *members;      /* global 0-based particle indices, CRS order */
-} region_cands_t;
+#include "oracle_internal.h"
 
 /*
  * process_single_region, fof_fortran.f90:496-947 (minus the unused provisional
@@ -183,10 +178,9 @@ typedef struct {
  * for each particle of the region the 0-based global index of its component's
  * root (first member in CRS order).
  */
-static int process_region(const double *xp, int32_t istart, int32_t np, double L,
+int oracle_process_region(const double *xp, int32_t istart, int32_t np, double L,
                           double Lh, double b, int32_t np_min, int32_t nest,
-                          region_cands_t *out, int32_t *label_out,
-                          oracle_geom_t *geom_out)
+                          region_cands_t *out, int32_t *label_out, void *geom_vp)
 {
     out->ncand = 0;
     out->cand_np = out->member_ptr = out->members = NULL;
@@ -197,7 +191,7 @@ static int process_region(const double *xp, int32_t istart, int32_t np, double L
     double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
     for (int32_t i = 0; i < np; ++i) {
         for (int d = 0; d < 3; ++d) {
-            double v = f_modulo(xp[3 * (size_t)(istart + i) + d] - xa[d] + Lh, L) - Lh;
+            double v = oracle_modulo(xp[3 * (size_t)(istart + i) + d] - xa[d] + Lh, L) - Lh;
             xr[3 * (size_t)i + d] = v;
             if (i == 0 || v < bmin[d]) bmin[d] = v;
             if (i == 0 || v > bmax[d]) bmax[d] = v;
@@ -205,7 +199,7 @@ static int process_region(const double *xp, int32_t istart, int32_t np, double L
     }
     oracle_geom_t g;
     oracle_region_geometry(bmin, bmax, np, L, b, &g);
-    if (geom_out) *geom_out = g;
+    if (geom_vp) *(oracle_geom_t *)geom_vp = g;
     const int32_t Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
     const int32_t ncell = Nx * Ny * Nz;
 
@@ -380,7 +374,7 @@ static void add_ang_mom(const double *xp, const double *vp, const double *mp,
         double wi = mp[o], dx[3];
         w_sum += wi;
         for (int d = 0; d < 3; ++d) {
-            dx[d] = f_modulo(xp[3 * (size_t)o + d] - xa[d] + Lh, L) - Lh;
+            dx[d] = oracle_modulo(xp[3 * (size_t)o + d] - xa[d] + Lh, L) - Lh;
             qc[d] += wi * dx[d];
             vc[d] += wi * vp[3 * (size_t)o + d];
         }
@@ -391,7 +385,7 @@ static void add_ang_mom(const double *xp, const double *vp, const double *mp,
         int32_t o = mem[k];
         double wi = mp[o], dx[3], dv[3];
         for (int d = 0; d < 3; ++d) {
-            dx[d] = (f_modulo(xp[3 * (size_t)o + d] - xa[d] + Lh, L) - Lh) - qc[d];
+            dx[d] = (oracle_modulo(xp[3 * (size_t)o + d] - xa[d] + Lh, L) - Lh) - qc[d];
             dv[d] = vp[3 * (size_t)o + d] - vc[d];
         }
         am[0] += wi * (dx[1] * dv[2] - dx[2] * dv[1]);
@@ -439,7 +433,7 @@ int32_t oracle_fof(const double *xp, const double *vp, const double *mp,
     for (int32_t h = 0; h < nhalo; ++h) {
         int32_t np = np_halo[h];
         if (np >= np_min && np > 0) {
-            int e = process_region(xp, start[h], np, L, Lh, b, np_min, nest, &rc[h],
+            int e = oracle_process_region(xp, start[h], np, L, Lh, b, np_min, nest, &rc[h],
                                    label_out ? label_out + start[h] : NULL, NULL);
             if (e) ierr = e;
         }
@@ -488,7 +482,7 @@ int32_t oracle_fof_candidates(const double *xp, const int32_t *np_halo, int32_t 
         int32_t np = np_halo[h];
         if (np >= np_min && np > 0) {
             region_cands_t rc;
-            if (process_region(xp, start, np, L, Lh, b, np_min, 1, &rc, NULL, NULL)) return -3;
+            if (oracle_process_region(xp, start, np, L, Lh, b, np_min, 1, &rc, NULL, NULL)) return -3;
             for (int32_t g = 0; g < rc.ncand; ++g) {
                 if (nc < cap_cand) { cand_np[nc] = rc.cand_np[g]; cand_region[nc] = h; }
                 for (int32_t k = 0; k < rc.cand_np[g]; ++k) {
@@ -512,7 +506,7 @@ int32_t oracle_geometry_of(const double *xp, int32_t np, double L, double b, ora
     double Lh = 0.5 * L, bmin[3], bmax[3];
     for (int32_t i = 0; i < np; ++i)
         for (int d = 0; d < 3; ++d) {
-            double v = f_modulo(xp[3 * (size_t)i + d] - xp[d] + Lh, L) - Lh;
+            double v = oracle_modulo(xp[3 * (size_t)i + d] - xp[d] + Lh, L) - Lh;
             if (i == 0 || v < bmin[d]) bmin[d] = v;
             if (i == 0 || v > bmax[d]) bmax[d] = v;
         }

@@ -47,6 +47,20 @@ def load_oracle():
     lib.oracle_fof_candidates.argtypes = [dp, ip, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip]
+    lib.oracle_fof_bound.restype = ctypes.c_int32
+    lib.oracle_fof_bound.argtypes = [dp, dp, dp, ip, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int64, ip, ip, dp, ctypes.c_int32, ip, ip, ip, dp,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_int64]
+    lib.oracle_green.restype = None
+    lib.oracle_green.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_float)]
+    lib.oracle_select.restype = ctypes.c_int32
+    lib.oracle_select.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                  ctypes.c_double, ip, ip, dp]
+    lib.oracle_depth.restype = ctypes.c_double
+    lib.oracle_depth.argtypes = [ctypes.c_double, ctypes.c_double]
     lib.oracle_geometry_of.restype = ctypes.c_int32
     lib.oracle_geometry_of.argtypes = [dp, ctypes.c_int32, ctypes.c_double,
                                        ctypes.c_double, ctypes.POINTER(OracleGeom)]
@@ -137,3 +151,68 @@ def oracle_geometry(xp, L_Box, b):
         "periodic": [bool(v) for v in g.periodic], "is_periodic": bool(g.is_periodic),
         "bmin": list(g.bmin), "bmax": list(g.bmax),
     }
+
+
+def oracle_fof_bound(xp, np_halo, L_Box, b, np_halo_min, vp, mp=None, app=0.1, a_val=1.0,
+                     b_val=0.0, s15=17.152, ng=256, want_phi=False):
+    """FoF + unbinding as the reference's bound_check=True branch
+    (fof_fortran.f90:253-356). Returns a dict with in_halo, halo_id,
+    ang_mom, ierr and, per FoF candidate, status / np_bound / method gid /
+    spacing (and the first-pass potential of every candidate member, pid
+    order, when want_phi)."""
+    lib = load_oracle()
+    xp = _f64_fortran(xp)
+    vp = _f64_fortran(vp)
+    n = xp.shape[1]
+    np_halo = np.ascontiguousarray(np_halo, dtype=np.int32)
+    nh = len(np_halo)
+    in_halo = np.zeros(max(n, 1), np.int32)
+    halo_id = np.zeros(max(n, 1), np.int32)
+    ang = np.zeros((3, max(nh, 1)), np.float64, order="F")
+    cap = max(n, 1)
+    st = np.full(cap, -1, np.int32)
+    nb = np.zeros(cap, np.int32)
+    gid = np.full(cap, -1, np.int32)
+    sp = np.zeros(cap, np.float64)
+    phi = np.zeros(max(n, 1), np.float32) if want_phi else None
+    mpa = np.ascontiguousarray(mp, dtype=np.float64) if mp is not None else None
+    dp = ctypes.c_double
+    ierr = lib.oracle_fof_bound(
+        _ptr(xp, dp), _ptr(vp, dp), _ptr(mpa, dp) if mpa is not None else None,
+        _ptr(np_halo, ctypes.c_int32), nh, float(L_Box), float(b), int(np_halo_min), float(app),
+        float(a_val), float(b_val), float(s15), int(ng), _ptr(in_halo, ctypes.c_int32),
+        _ptr(halo_id, ctypes.c_int32), _ptr(ang, dp), cap, _ptr(st, ctypes.c_int32),
+        _ptr(nb, ctypes.c_int32), _ptr(gid, ctypes.c_int32), _ptr(sp, dp),
+        _ptr(phi, ctypes.c_float) if phi is not None else None, len(phi) if phi is not None else 0)
+    ncand = int(np.sum(st >= 0))
+    out = {"ierr": int(ierr), "in_halo": in_halo[:n], "halo_id": halo_id[:n],
+           "ang_mom": ang[:, :nh], "status": st[:ncand], "np_bound": nb[:ncand],
+           "gid": gid[:ncand], "spacing": sp[:ncand]}
+    if want_phi:
+        out["phi"] = phi[:n]
+    return out
+
+
+def oracle_green(n):
+    """Half-spectrum Green function (n, n, n/2+1) float32 [z, y, x] and the
+    5^3 self kernel [z, y, x] of an n^3 isolated grid."""
+    lib = load_oracle()
+    nh = n // 2 + 1
+    G = np.zeros(nh * n * n, np.float32)
+    self_ = np.zeros(125, np.float32)
+    lib.oracle_green(int(n), _ptr(G, ctypes.c_float), _ptr(self_, ctypes.c_float))
+    return G.reshape(n, n, nh), self_.reshape(5, 5, 5)
+
+
+def oracle_select(np_, span, s15=17.152, ng=256, app=0.1):
+    lib = load_oracle()
+    g = ctypes.c_int32(0)
+    l = ctypes.c_int32(0)
+    sp = ctypes.c_double(0.0)
+    st = lib.oracle_select(float(app), float(s15), int(ng), int(np_), float(span),
+                           ctypes.byref(g), ctypes.byref(l), ctypes.byref(sp))
+    return st, g.value, l.value, sp.value
+
+
+def oracle_depth(r, h):
+    return load_oracle().oracle_depth(float(r), float(h))
