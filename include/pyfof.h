@@ -83,8 +83,11 @@ int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, i
 int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
 
 /* Sizes of the last call on ctx: [0] particles, [1] reference cells,
- * [2] dense cells given sub-cells, [3] halos (candidates), [4] members of
- * halos. Returns the number of values written. */
+ * [2] dense cells given sub-cells, [3] halos in the output, [4] members of
+ * those halos; with bound_check: [5] valid bound halos, [6..8] candidates
+ * ending OK / TOO_FEW / MAX_ITER, [9] direct-summation candidates,
+ * [10..13] candidates on the 48/64/128/256 grids, [14] potential passes.
+ * Returns the number of values written. */
 int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n);
 
 /* HIP stream (hipStream_t) the context launches on. */

@@ -494,7 +494,7 @@ static void c2_incremental(const c2cells_t *cc, int n, const float *q, const cha
 }
 
 /* Fortran NINT for real(4): round half away from zero */
-static int nint_f(float x) { return (int)(x >= 0.0f ? floorf(x + 0.5f) : -floorf(-x + 0.5f)); }
+static int nint_f(float x) { return (int)roundf(x); }
 
 typedef struct {
     int gid, lev;

@@ -146,9 +146,10 @@ class Context:
 
     def stats(self):
         """dict of the last call's sizes (particles, cells, dense cells, halos, members)."""
-        buf = (ctypes.c_int64 * 5)()
-        n = self.lib.pyfof_ctx_stats(self.h, buf, 5)
-        keys = ["n", "ncell", "ndense", "nhalo", "nmem"]
+        buf = (ctypes.c_int64 * 15)()
+        n = self.lib.pyfof_ctx_stats(self.h, buf, 15)
+        keys = ["n", "ncell", "ndense", "nhalo", "nmem", "nvalid", "n_ok", "n_too_few",
+                "n_max_iter", "n_direct", "n_pm48", "n_pm64", "n_pm128", "n_pm256", "passes"]
         return {keys[i]: int(buf[i]) for i in range(n)}
 
     def stream(self):

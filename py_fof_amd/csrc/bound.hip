@@ -1,0 +1,1270 @@
+// Gravitational unbinding of FoF candidates (bound_check=True) on MI355X.
+//
+// Restates bound_fortran.f90's solver for every candidate at once:
+//   packing + span       (fof_fortran.f90:263-308, bound_fortran.f90:2940-2977)
+//   method selection     (host, bound_fortran.f90:1854-2082, 2901-3073)
+//   direct summation     k_direct_solve: one workgroup per halo of <= 1000
+//                        members, the whole pass/removal loop in LDS, FP64
+//                        potentials summed in the reference's order
+//   particle mesh        per grid class, all active halos batched per pass:
+//                        TSC setup + self term, fixed-point deposit, FFT,
+//                        x G(k), inverse FFT, interpolation (C1), short-range
+//                        residual (C2), then k_pm_inner (removal loop)
+//   write-back           valid = status OK and np_bound >= np_halo_min
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bound.hpp"
+#include "fft.hpp"
+#include "prims.hpp"
+
+namespace pyfof {
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kDirectMax = 1000;  // bound_fortran.f90:1976-1980
+constexpr double kPi = 3.141592653589793238462643383279502884197;
+
+inline uint32_t blocks(uint64_t n) { return div_up(n == 0 ? 1 : n, kB); }
+
+struct Params {
+    double app, a, b, L, Lh;
+    int32_t np_min;
+};
+
+// ---------------------------------------------------------------------------
+// depth kernels (bound_fortran.f90:3080-3117)
+
+__device__ double depth_continuous(double r, double h) {
+    if (r <= 0.0) return 208.0 / (70.0 * h);
+    if (r >= h) return 1.0 / r;
+    const double e = 2.0 * r / h;
+    if (e <= 1.0) {
+        const double prim = e * e * (112.0 + e * e * (-56.0 + e * (14.0 + e * (8.0 - 3.0 * e))));
+        return (208.0 - prim) / (70.0 * h);
+    }
+    const double prim = e * e * (448.0 + e * (-280.0 + e * (56.0 + e * (14.0 + e * (-8.0 + e))))) -
+                        224.0 * e - 12.0 / e;
+    return (128.0 - prim) / (70.0 * h);
+}
+
+__device__ double c2_delta_depth(double r, double ht, double hs) {
+    if (r <= 0.0) return (208.0 / 70.0) * (1.0 / ht - 1.0 / hs);
+    if (r >= ht && r >= hs) return 0.0;
+    return depth_continuous(r, ht) - depth_continuous(r, hs);
+}
+
+__device__ __forceinline__ double gm_rel(double x, double xa, double L, double Lh) {
+    double a = (x - xa) + Lh;
+    double r = fmod(a, L);
+    if (r != 0.0 && ((r < 0.0) != (L < 0.0))) r += L;
+    return r - Lh;
+}
+
+// ---------------------------------------------------------------------------
+// packing and span
+
+// member slot t: float position relative to the halo's anchor (the reference's
+// xv = real(dx_rel, 4), and q = xv - xv(anchor) = xv), float velocity, mass
+__global__ void k_bound_pack(const uint32_t* __restrict__ members, const uint32_t* __restrict__ mhalo,
+                             const uint32_t* __restrict__ anchor, const double* __restrict__ xp,
+                             const double* __restrict__ vp, const double* __restrict__ mp,
+                             uint32_t nmem, double L, double Lh, float4* __restrict__ qw,
+                             float4* __restrict__ vv) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const size_t i = members[t];
+    const size_t a = anchor[mhalo[t]];
+    float4 q;
+    q.x = (float)gm_rel(xp[3 * i], xp[3 * a], L, Lh);
+    q.y = (float)gm_rel(xp[3 * i + 1], xp[3 * a + 1], L, Lh);
+    q.z = (float)gm_rel(xp[3 * i + 2], xp[3 * a + 2], L, Lh);
+    q.w = mp ? (float)mp[i] : 1.0f;
+    qw[t] = q;
+    vv[t] = vp ? make_float4((float)vp[3 * i], (float)vp[3 * i + 1], (float)vp[3 * i + 2], 0.f)
+               : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// per halo: lo/hi start at 0 (the anchor) as bound_classify's float min/max
+__global__ void __launch_bounds__(kB) k_bound_span(const float4* __restrict__ qw,
+                                                   const uint32_t* __restrict__ hoff,
+                                                   float* __restrict__ lohi) {
+    __shared__ float red[6][kB / kWave];
+    const uint32_t h = blockIdx.x;
+    const uint32_t s = h ? hoff[h - 1] : 0u, e = hoff[h];
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+    for (uint32_t t = s + threadIdx.x; t < e; t += kB) {
+        const float4 q = qw[t];
+        lo[0] = fminf(lo[0], q.x); hi[0] = fmaxf(hi[0], q.x);
+        lo[1] = fminf(lo[1], q.y); hi[1] = fmaxf(hi[1], q.y);
+        lo[2] = fminf(lo[2], q.z); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[d] = fminf(lo[d], __shfl_down(lo[d], o, kWave));
+            hi[d] = fmaxf(hi[d], __shfl_down(hi[d], o, kWave));
+        }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        float v = red[d][0];
+        for (int k = 1; k < kB / kWave; ++k) v = d < 3 ? fminf(v, red[d][k]) : fmaxf(v, red[d][k]);
+        lohi[6 * (size_t)h + d] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// direct summation halos: one workgroup each, everything in LDS
+
+struct DirectLds {
+    float4 q[kDirectMax];     // position, mass
+    float3 v[kDirectMax];
+    double phid[kDirectMax];
+    float phi[kDirectMax];
+    float en[kDirectMax];
+    int16_t just[kDirectMax];
+    int16_t ub[kDirectMax];
+    uint8_t keep[kDirectMax];
+    double red[8];
+    int32_t cnt[4];
+};
+
+__device__ __forceinline__ bool e_greater(float eb, int b, float ea, int a) {
+    // bound_fortran.f90:2808-2844: larger energy, ties -> smaller local index
+    return eb > ea || (eb == ea && b < a);
+}
+
+__global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__ dlist,
+                                                     const uint32_t* __restrict__ hoff,
+                                                     const float4* __restrict__ qw,
+                                                     const float4* __restrict__ vv,
+                                                     uint8_t* __restrict__ keep_out,
+                                                     int32_t* __restrict__ status,
+                                                     int32_t* __restrict__ npb_out,
+                                                     int32_t* __restrict__ err, Params P) {
+    __shared__ DirectLds S;
+    const int32_t h = dlist[blockIdx.x];
+    const uint32_t s0 = h ? hoff[h - 1] : 0u;
+    const int n = (int)(hoff[h] - s0);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < n; i += kB) {
+        S.q[i] = qw[s0 + i];
+        const float4 v = vv[s0 + i];
+        S.v[i] = make_float3(v.x, v.y, v.z);
+        S.keep[i] = 1;
+    }
+    if (tid == 0) { S.cnt[0] = 0; S.cnt[1] = 0; }
+    __syncthreads();
+    const double Gd = P.a / (6.0 * kPi);
+    const float b32 = (float)P.b;
+    int np_bound = n, n_just = 0, st = 5;
+    for (int pass = 1; pass <= 20; ++pass) {
+        if (pass == 1) {
+            // phi_direct(k) = sum over partners j (ascending) of -w_j (a/6pi) D(r)
+            for (int k = tid; k < n; k += kB) {
+                const float4 qk = S.q[k];
+                double acc = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    if (j == k) continue;
+                    const float4 qj = S.q[j];
+                    const double dx = (double)(qj.x - qk.x), dy = (double)(qj.y - qk.y),
+                                 dz = (double)(qj.z - qk.z);
+                    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+                    const double dd = __dmul_rn(Gd, depth_continuous(r, P.app));
+                    acc = __dsub_rn(acc, __dmul_rn((double)qj.w, dd));
+                }
+                S.phid[k] = acc;
+            }
+        } else {
+            // add back the sources deleted since the last refresh, in deletion order
+            for (int i = tid; i < n; i += kB) {
+                if (!S.keep[i]) continue;
+                const float4 qi = S.q[i];
+                double acc = S.phid[i];
+                for (int k = 0; k < n_just; ++k) {
+                    const int j = S.just[k];
+                    const float4 qj = S.q[j];
+                    const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y),
+                                 dz = (double)(qj.z - qi.z);
+                    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+                    acc = __dadd_rn(acc, __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, P.app)));
+                }
+                S.phid[i] = acc;
+            }
+        }
+        n_just = 0;
+        __syncthreads();
+        bool bad = false;
+        for (int i = tid; i < n; i += kB) {
+            if (!S.keep[i]) continue;
+            float p = (float)S.phid[i];
+            if (np_bound == 1) p = 0.0f;
+            if (!isfinite(p)) bad = true;
+            S.phi[i] = p;
+        }
+        if (__syncthreads_or(bad)) { st = -4; break; }
+        int deleted = 0;
+        bool refresh = false;
+        while (!refresh) {
+            if (tid == 0) {
+                // f64 centre and bulk velocity over kept members, in index order
+                double ws = 0.0, qc[3] = {0, 0, 0}, vc[3] = {0, 0, 0};
+                for (int i = 0; i < n; ++i) {
+                    if (!S.keep[i]) continue;
+                    const double wi = (double)S.q[i].w;
+                    ws = __dadd_rn(ws, wi);
+                    qc[0] = __dadd_rn(qc[0], __dmul_rn(wi, (double)S.q[i].x));
+                    qc[1] = __dadd_rn(qc[1], __dmul_rn(wi, (double)S.q[i].y));
+                    qc[2] = __dadd_rn(qc[2], __dmul_rn(wi, (double)S.q[i].z));
+                    vc[0] = __dadd_rn(vc[0], __dmul_rn(wi, (double)S.v[i].x));
+                    vc[1] = __dadd_rn(vc[1], __dmul_rn(wi, (double)S.v[i].y));
+                    vc[2] = __dadd_rn(vc[2], __dmul_rn(wi, (double)S.v[i].z));
+                }
+                for (int d = 0; d < 3; ++d) { S.red[d] = qc[d] / ws; S.red[3 + d] = vc[d] / ws; }
+                S.cnt[0] = 0;
+            }
+            __syncthreads();
+            const float qc32[3] = {(float)S.red[0], (float)S.red[1], (float)S.red[2]};
+            const float vc32[3] = {(float)S.red[3], (float)S.red[4], (float)S.red[5]};
+            bool badE = false;
+            for (int i = tid; i < n; i += kB) {
+                if (!S.keep[i]) continue;
+                const float4 q = S.q[i];
+                const float3 v = S.v[i];
+                const float d0 = __fadd_rn(__fsub_rn(v.x, vc32[0]), __fmul_rn(b32, __fsub_rn(q.x, qc32[0])));
+                const float d1 = __fadd_rn(__fsub_rn(v.y, vc32[1]), __fmul_rn(b32, __fsub_rn(q.y, qc32[1])));
+                const float d2 = __fadd_rn(__fsub_rn(v.z, vc32[2]), __fmul_rn(b32, __fsub_rn(q.z, qc32[2])));
+                float s = __fmul_rn(d0, d0);
+                s = __fadd_rn(s, __fmul_rn(d1, d1));
+                s = __fadd_rn(s, __fmul_rn(d2, d2));
+                const float E = __fadd_rn(s / 2.0f, S.phi[i]);
+                S.en[i] = E;
+                if (!isfinite(E)) badE = true;
+                if (E >= 0.0f) S.ub[atomicAdd(&S.cnt[0], 1)] = (int16_t)i;
+            }
+            if (__syncthreads_or(badE)) { st = -1; break; }
+            const int nu = S.cnt[0];
+            if (nu == 0) {
+                if (deleted == 0) { st = 0; break; }
+                refresh = true;
+                __syncthreads();
+                continue;
+            }
+            int nb = (int)ceil(0.02 * (double)np_bound);
+            nb = nb < 1 ? 1 : nb;
+            nb = nb > nu ? nu : nb;
+            // rank of each unbound member in decreasing "greatness"; the nb first go
+            for (int u = tid; u < nu; u += kB) {
+                const int a = S.ub[u];
+                const float ea = S.en[a];
+                int rank = 0;
+                for (int t = 0; t < nu; ++t) {
+                    const int b = S.ub[t];
+                    if (e_greater(S.en[b], b, ea, a)) ++rank;
+                }
+                if (rank < nb) S.just[n_just + rank] = (int16_t)a;
+            }
+            __syncthreads();
+            for (int k = tid; k < nb; k += kB) S.keep[S.just[n_just + k]] = 0;
+            n_just += nb;
+            np_bound -= nb;
+            deleted += nb;
+            __syncthreads();
+            if (np_bound < P.np_min) { st = 1; break; }
+            if ((double)deleted >= 0.2 * (double)np_bound) refresh = true;
+        }
+        if (st != 5) break;
+    }
+    __syncthreads();
+    if (st < 0) {
+        if (tid == 0) atomicCAS(err, 0, st == -4 ? 4 : 1);
+    }
+    for (int i = tid; i < n; i += kB) keep_out[s0 + i] = S.keep[i];
+    if (tid == 0) {
+        status[h] = st < 0 ? 5 : st;
+        npb_out[h] = np_bound;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green function (bound_fortran.f90:2300-2382), full spectrum, f64 alias sum
+
+__device__ __forceinline__ double wave_k(int idx, int n) {
+    int mode = idx < n / 2 ? idx : idx - n;
+    if (idx == n / 2) mode = -n / 2;
+    return 2.0 * kPi * (double)mode / (double)n;
+}
+
+__global__ void k_green(float* __restrict__ G, int32_t n) {
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    const uint64_t tot = (uint64_t)n * n * n;
+    if (id >= tot) return;
+    const int x = (int)(id % n), y = (int)((id / n) % n), z = (int)(id / ((uint64_t)n * n));
+    const double kv[3] = {wave_k(x, n), wave_k(y, n), wave_k(z, n)};
+    const double radius = 0.5 * (double)n, soft = 3.5;
+    // per-axis sinc^6 of the aliases
+    double w6[3][5];
+    for (int ax = 0; ax < 3; ++ax)
+        for (int a = 0; a < 5; ++a) {
+            const double k = kv[ax] + 2.0 * kPi * (double)(a - 2);
+            const double hx = 0.5 * k;
+            double s = 1.0;
+            if (hx != 0.0) s = sin(hx) / hx;
+            const double s2 = s * s;
+            w6[ax][a] = s2 * s2 * s2;
+        }
+    double asum = 0.0, bsum = 0.0;
+    for (int ka = 0; ka < 5; ++ka)
+        for (int ja = 0; ja < 5; ++ja)
+            for (int ia = 0; ia < 5; ++ia) {
+                const double k0 = kv[0] + 2.0 * kPi * (double)(ia - 2);
+                const double k1 = kv[1] + 2.0 * kPi * (double)(ja - 2);
+                const double k2 = kv[2] + 2.0 * kPi * (double)(ka - 2);
+                const double kmag = sqrt(k0 * k0 + k1 * k1 + k2 * k2);
+                const double uk2 = w6[0][ia] * w6[1][ja] * w6[2][ka];
+                asum += uk2;
+                double target;
+                if (kmag == 0.0) {
+                    target = -4.0 * kPi * radius * radius / 6.0;
+                } else {
+                    const double yv = soft * kmag / 2.0;
+                    const double y4 = yv * yv * yv * yv;
+                    const double ss = 12.0 * (1.0 / y4) * (2.0 - 2.0 * cos(yv) - yv * sin(yv));
+                    const double kr = kmag * radius;
+                    const double cut = 1.0 - sin(kr) / kr;
+                    target = -4.0 * kPi * (ss * ss) * cut / (kmag * kmag);
+                }
+                bsum += uk2 * target;
+            }
+    G[id] = (float)(bsum / (asum * asum));
+}
+
+__global__ void k_green_to_complex(const float* __restrict__ G, float2* __restrict__ g, uint64_t tot) {
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (id < tot) g[id] = make_float2(G[id], 0.0f);
+}
+
+__global__ void k_self_extract(const float2* __restrict__ g, int32_t n, float* __restrict__ self) {
+    const int t = threadIdx.x;
+    if (t >= 125) return;
+    const int ix = t % 5 - 2, iy = (t / 5) % 5 - 2, iz = t / 25 - 2;
+    const int x = (ix + n) % n, y = (iy + n) % n, z = (iz + n) % n;
+    const float norm = (float)((double)n * n * n);
+    self[t] = g[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)].x / norm;
+}
+
+// ---------------------------------------------------------------------------
+// particle-mesh halos
+
+struct PmHalo {       // per PM halo
+    float4 origin;    // (x, y, z, spacing)
+    int32_t slot;     // grid slot within its class
+    int32_t cls;      // grid class 0..3
+    int32_t n;        // grid size
+    int32_t c2off;    // first C2 cell of the halo (global)
+    int32_t nc[3];    // C2 cells per axis
+    int32_t pad;
+    double umin[3];   // C2 cell grid origin in u = q / spacing
+};
+
+// TSC setup (bound_fortran.f90:2560-2602): nearest node, weights, self term
+__global__ void k_tsc_setup(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                            const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                            const float* __restrict__ selfs, uint32_t nmem,
+                            int4* __restrict__ first, float* __restrict__ wt,
+                            float* __restrict__ self, int32_t* __restrict__ err) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const int32_t k = pm_of_halo[mhalo[t]];
+    if (k < 0) return;
+    const PmHalo H = pm[k];
+    const float4 q = qw[t];
+    const float sp = H.origin.w;
+    const float qa[3] = {q.x, q.y, q.z}, oa[3] = {H.origin.x, H.origin.y, H.origin.z};
+    float w9[9];
+    int f[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        const float tt = __fdiv_rn(__fsub_rn(qa[ax], oa[ax]), sp);
+        if (!(tt >= 0.0f) || !(tt < (float)(H.n / 2 - 1))) {
+            atomicCAS(err, 0, 1);  // bound_fortran.f90:2577-2580
+            return;
+        }
+        const int nearest = (int)roundf(tt);  // Fortran NINT: half away from zero
+        const float d = __fsub_rn(tt, (float)nearest);
+        f[ax] = nearest;
+        const float a0 = __fsub_rn(0.5f, d), a2 = __fadd_rn(0.5f, d);
+        w9[3 * ax + 0] = __fmul_rn(0.5f, __fmul_rn(a0, a0));
+        w9[3 * ax + 1] = __fsub_rn(0.75f, __fmul_rn(d, d));
+        w9[3 * ax + 2] = __fmul_rn(0.5f, __fmul_rn(a2, a2));
+    }
+    const float* sk = selfs + 125 * H.cls;
+    double acc = 0.0;
+    for (int sz = 0; sz < 3; ++sz)
+        for (int sy = 0; sy < 3; ++sy)
+            for (int sx = 0; sx < 3; ++sx) {
+                const float sw = __fmul_rn(__fmul_rn(w9[sx], w9[3 + sy]), w9[6 + sz]);
+                for (int tz = 0; tz < 3; ++tz)
+                    for (int ty = 0; ty < 3; ++ty)
+                        for (int tx = 0; tx < 3; ++tx) {
+                            const float tw = __fmul_rn(__fmul_rn(w9[tx], w9[3 + ty]), w9[6 + tz]);
+                            acc = __dadd_rn(acc, __dmul_rn((double)__fmul_rn(sw, tw),
+                                                           (double)sk[(tx - sx + 2) + 5 * ((ty - sy + 2) + 5 * (tz - sz + 2))]));
+                        }
+            }
+    first[t] = make_int4(f[0], f[1], f[2], 0);
+    for (int i = 0; i < 9; ++i) wt[9 * (size_t)t + i] = w9[i];
+    self[t] = (float)acc;
+}
+
+// deposit of kept members into int64 fixed-point grids (order independent)
+__global__ void k_deposit(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                          const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                          const uint8_t* __restrict__ keep, const int4* __restrict__ first,
+                          const float* __restrict__ wt, const uint8_t* __restrict__ active,
+                          const double* __restrict__ fscale, int32_t cls, uint32_t nmem,
+                          unsigned long long* __restrict__ acc) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = pm_of_halo[h];
+    if (k < 0 || pm[k].cls != cls || !active[h]) return;
+    const PmHalo H = pm[k];
+    const int n = H.n;
+    const float w = qw[t].w;
+    const int4 f = first[t];
+    const float* w9 = wt + 9 * (size_t)t;
+    unsigned long long* g = acc + (size_t)H.slot * n * n * n;
+    for (int sz = 0; sz < 3; ++sz) {
+        const int iz = f.z - 1 + sz;
+        for (int sy = 0; sy < 3; ++sy) {
+            const int iy = f.y - 1 + sy;
+            for (int sx = 0; sx < 3; ++sx) {
+                const int ix = f.x - 1 + sx;
+                const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                const long long fx = llrint((double)v * fscale[h]);
+                atomicAdd(&g[(size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz)], (unsigned long long)fx);
+            }
+        }
+    }
+}
+
+__global__ void k_fix_to_complex(const unsigned long long* __restrict__ acc, float2* __restrict__ g,
+                                 const int32_t* __restrict__ slots, const int32_t* __restrict__ slot_halo,
+                                 const double* __restrict__ fscale, int32_t n) {
+    const uint64_t grid = (uint64_t)n * n * n;
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    const uint32_t sl = blockIdx.y;
+    if (id >= grid) return;
+    const size_t off = (size_t)slots[sl] * grid + id;
+    const double inv = 1.0 / fscale[slot_halo[slots[sl]]];
+    g[off] = make_float2((float)((double)(long long)acc[off] * inv), 0.0f);
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ acc, const int32_t* __restrict__ slots,
+                           uint64_t grid) {
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (id < grid) acc[(size_t)slots[blockIdx.y] * grid + id] = 0ull;
+}
+
+__global__ void k_mul_green(float2* __restrict__ g, const float* __restrict__ G,
+                            const int32_t* __restrict__ slots, uint64_t grid) {
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (id >= grid) return;
+    float2& v = g[(size_t)slots[blockIdx.y] * grid + id];
+    const float gk = G[id];
+    v = make_float2(v.x * gk, v.y * gk);
+}
+
+// C1 potential (bound_fortran.f90:2680-2703)
+__global__ void k_interp(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                         const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                         const uint8_t* __restrict__ keep, const int4* __restrict__ first,
+                         const float* __restrict__ wt, const float* __restrict__ self,
+                         const uint8_t* __restrict__ active, const double* __restrict__ mkeep,
+                         const float2* __restrict__ g, int32_t cls, uint32_t nmem, double a,
+                         float* __restrict__ phi_c1) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = pm_of_halo[h];
+    if (k < 0 || pm[k].cls != cls || !active[h]) return;
+    const PmHalo H = pm[k];
+    const int n = H.n;
+    const float norm = (float)((double)n * n * n);
+    const float2* gg = g + (size_t)H.slot * n * n * n;
+    const int4 f = first[t];
+    const float* w9 = wt + 9 * (size_t)t;
+    float raw = 0.0f;
+    for (int sz = 0; sz < 3; ++sz) {
+        const int iz = f.z - 1 + sz;
+        for (int sy = 0; sy < 3; ++sy) {
+            const int iy = f.y - 1 + sy;
+            for (int sx = 0; sx < 3; ++sx) {
+                const int ix = f.x - 1 + sx;
+                const float rho = __fdiv_rn(gg[(size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz)].x, norm);
+                raw = __fadd_rn(raw, __fmul_rn(__fmul_rn(__fmul_rn(rho, w9[sx]), w9[3 + sy]), w9[6 + sz]));
+            }
+        }
+    }
+    const float w = qw[t].w;
+    const float radius = (float)n / 2.0f;
+    const float dz0 = __fdiv_rn((float)fmax(0.0, mkeep[h] - (double)w), radius);
+    const float pref = (float)(a / (6.0 * kPi * (double)H.origin.w));
+    phi_c1[t] = __fmul_rn(pref, __fsub_rn(__fsub_rn(raw, __fmul_rn(w, self[t])), dz0));
+}
+
+// C2 cells (bound_fortran.f90:3125-3223): width 3.5 in u = q / spacing
+__global__ void __launch_bounds__(kB) k_c2_extent(const float4* __restrict__ qw,
+                                                  const uint32_t* __restrict__ hoff,
+                                                  const int32_t* __restrict__ pm_list,
+                                                  const int32_t* __restrict__ pm_halo,
+                                                  const PmHalo* __restrict__ pm,
+                                                  double* __restrict__ uext) {
+    __shared__ double red[6][kB / kWave];
+    const int32_t k = pm_list[blockIdx.x];
+    const int32_t h = pm_halo[k];
+    const uint32_t s = h ? hoff[h - 1] : 0u, e = hoff[h];
+    const double sp = (double)pm[k].origin.w;
+    double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+    for (uint32_t t = s + threadIdx.x; t < e; t += kB) {
+        const float4 q = qw[t];
+        const double u[3] = {(double)q.x / sp, (double)q.y / sp, (double)q.z / sp};
+        for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], u[d]); hi[d] = fmax(hi[d], u[d]); }
+    }
+    for (int d = 0; d < 3; ++d)
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[d] = fmin(lo[d], __shfl_down(lo[d], o, kWave));
+            hi[d] = fmax(hi[d], __shfl_down(hi[d], o, kWave));
+        }
+    const int w = threadIdx.x / kWave;
+    if (__lane_id() == 0)
+        for (int d = 0; d < 3; ++d) { red[d][w] = lo[d]; red[3 + d][w] = hi[d]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        double v = red[d][0];
+        for (int q = 1; q < kB / kWave; ++q) v = d < 3 ? fmin(v, red[d][q]) : fmax(v, red[d][q]);
+        uext[6 * (size_t)k + d] = v;
+    }
+}
+
+__device__ __forceinline__ int c2_axis_cell(double u, double umin, int nc) {
+    int v = (int)floor((u - umin) / 3.5) + 1;
+    v = v < 1 ? 1 : v;
+    return (v > nc ? nc : v) - 1;  // 0-based
+}
+
+// C2 cell of every member (the sentinel `ncells` for members of direct halos)
+__global__ void k_c2_keys(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                          const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                          uint32_t nmem, uint32_t ncells, uint32_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    vals[t] = t;
+    const int32_t k = pm_of_halo[mhalo[t]];
+    if (k < 0) {
+        keys[t] = ncells;
+        return;
+    }
+    const PmHalo H = pm[k];
+    const float4 q = qw[t];
+    const double sp = (double)H.origin.w;
+    const int cx = c2_axis_cell((double)q.x / sp, H.umin[0], H.nc[0]);
+    const int cy = c2_axis_cell((double)q.y / sp, H.umin[1], H.nc[1]);
+    const int cz = c2_axis_cell((double)q.z / sp, H.umin[2], H.nc[2]);
+    keys[t] = (uint32_t)H.c2off + (uint32_t)cx + (uint32_t)H.nc[0] * ((uint32_t)cy + (uint32_t)H.nc[1] * (uint32_t)cz);
+}
+
+// cell of each member, and run ends of the sorted cell keys
+__global__ void k_c2_cells(const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
+                           uint32_t n, uint32_t ncells, uint32_t* __restrict__ cell_of,
+                           uint32_t* __restrict__ ends) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = skeys[i];
+    cell_of[svals[i]] = k;
+    if (k < ncells && (i + 1 == n || skeys[i + 1] != k)) ends[k] = i + 1;
+}
+
+__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// total mass of each PM halo (the kept mass of the first pass)
+// and the fixed-point scale of its deposit: 2^40 units per unit mass, fewer
+// when the whole halo mass would not stay below 2^61 units
+__global__ void __launch_bounds__(kB) k_halo_mass(const float4* __restrict__ qw,
+                                                  const uint32_t* __restrict__ hoff,
+                                                  const int32_t* __restrict__ pm_halo,
+                                                  double* __restrict__ mkeep,
+                                                  double* __restrict__ fscale) {
+    __shared__ double red[kB / kWave];
+    const int32_t h = pm_halo[blockIdx.x];
+    const uint32_t s = h ? hoff[h - 1] : 0u, e = hoff[h];
+    double m = 0.0;
+    for (uint32_t t = s + threadIdx.x; t < e; t += kB) m += (double)qw[t].w;
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_down(m, o, kWave);
+    if (__lane_id() == 0) red[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int j = 0; j < kB / kWave; ++j) sm += red[j];
+        mkeep[h] = sm;
+        int e = 60 - ilogb(fmax(sm, 1.0));
+        fscale[h] = ldexp(1.0, e < 40 ? e : 40);
+    }
+}
+
+// Full C2 residual over kept members (bound_fortran.f90:3403-3483), as a
+// gather: member i sums over the kept members j of its 27 neighbour cells.
+__global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                          const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                          const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
+                          const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cstart,
+                          const uint32_t* __restrict__ clist, uint32_t nmem, double a, double app,
+                          float* __restrict__ phi_c2) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = pm_of_halo[h];
+    if (k < 0 || !active[h]) return;
+    const PmHalo H = pm[k];
+    const float sp = H.origin.w;
+    const double ht = (double)(float)app / (double)sp, hs = 3.5;
+    const double r2cut = 12.25 * (double)sp * (double)sp;
+    const float pref = (float)(a / (6.0 * kPi * (double)sp));
+    const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
+    const int cx = (int)(lc % (uint32_t)H.nc[0]);
+    const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
+    const int cz = (int)(lc / (uint32_t)(H.nc[0] * H.nc[1]));
+    const float4 qi = qw[t];
+    float acc = 0.0f;
+    for (int z = cz - 1; z <= cz + 1; ++z) {
+        if (z < 0 || z >= H.nc[2]) continue;
+        for (int y = cy - 1; y <= cy + 1; ++y) {
+            if (y < 0 || y >= H.nc[1]) continue;
+            for (int x = cx - 1; x <= cx + 1; ++x) {
+                if (x < 0 || x >= H.nc[0]) continue;
+                const uint32_t c = (uint32_t)H.c2off + (uint32_t)x + (uint32_t)H.nc[0] * ((uint32_t)y + (uint32_t)H.nc[1] * (uint32_t)z);
+                for (uint32_t p = cstart[c]; p < cstart[c + 1]; ++p) {
+                    const uint32_t j = clist[p];
+                    if (j == t || !keep[j]) continue;
+                    const float4 qj = qw[j];
+                    const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
+                    const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                    if (r2 < r2cut) {
+                        const double rs = sqrt(r2) / (double)sp;
+                        const float dphi = __fmul_rn(-pref, (float)c2_delta_depth(rs, ht, hs));
+                        acc = __fadd_rn(acc, __fmul_rn(qj.w, dphi));
+                    }
+                }
+            }
+        }
+    }
+    phi_c2[t] = acc;
+}
+
+// phi = C1 + C2 (bound_fortran.f90:2739-2754); zero for a single member
+__global__ void k_phi_combine(const uint32_t* __restrict__ mhalo, const int32_t* __restrict__ pm_of_halo,
+                              const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
+                              const int32_t* __restrict__ npb, const float* __restrict__ c1,
+                              const float* __restrict__ c2, uint32_t nmem, float* __restrict__ phi,
+                              int32_t* __restrict__ err) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const uint32_t h = mhalo[t];
+    if (pm_of_halo[h] < 0 || !active[h]) return;
+    float p = __fadd_rn(c1[t], c2[t]);
+    if (npb[h] == 1) p = 0.0f;
+    if (!isfinite(p)) atomicCAS(err, 0, 4);
+    phi[t] = p;
+}
+
+// Removal loop of one PM halo per workgroup (bound_fortran.f90:2756-2856):
+// repeated 2 % batches by energy until no unbound member is left or 20 % of
+// the kept members have gone since the potential was computed.
+__device__ __forceinline__ uint32_t ekey(float e) { return e > 0.0f ? __float_as_uint(e) : 0u; }
+
+__global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ alist,
+                                                 const uint32_t* __restrict__ hoff,
+                                                 const float4* __restrict__ qw,
+                                                 const float4* __restrict__ vv,
+                                                 const float* __restrict__ phi,
+                                                 float* __restrict__ en, uint8_t* __restrict__ keep,
+                                                 int32_t* __restrict__ status,
+                                                 int32_t* __restrict__ npb,
+                                                 uint8_t* __restrict__ active,
+                                                 double* __restrict__ mkeep, int32_t last_pass,
+                                                 int32_t* __restrict__ err, Params P) {
+    __shared__ double red[7][kB / kWave];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh[8];
+    __shared__ double bc[7];
+    const int32_t h = alist[blockIdx.x];
+    const uint32_t s0 = h ? hoff[h - 1] : 0u, e0 = hoff[h];
+    const int tid = threadIdx.x, w = tid / kWave;
+    const float b32 = (float)P.b;
+    int np_bound = npb[h];
+    int deleted = 0, st = -2;  // -2: still iterating
+    for (;;) {
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t t = s0 + tid; t < e0; t += kB) {
+            if (!keep[t]) continue;
+            const float4 q = qw[t];
+            const float4 u = vv[t];
+            const double wi = (double)q.w;
+            v[0] += wi;
+            v[1] += wi * (double)q.x; v[2] += wi * (double)q.y; v[3] += wi * (double)q.z;
+            v[4] += wi * (double)u.x; v[5] += wi * (double)u.y; v[6] += wi * (double)u.z;
+        }
+        for (int k = 0; k < 7; ++k)
+            for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, kWave);
+        if (__lane_id() == 0) for (int k = 0; k < 7; ++k) red[k][w] = v[k];
+        __syncthreads();
+        if (tid < 7) {
+            double s = 0.0;
+            for (int j = 0; j < kB / kWave; ++j) s += red[tid][j];
+            bc[tid] = s;
+        }
+        if (tid == 0) sh[0] = 0;  // unbound count
+        __syncthreads();
+        const double ws = bc[0];
+        const float qc32[3] = {(float)(bc[1] / ws), (float)(bc[2] / ws), (float)(bc[3] / ws)};
+        const float vc32[3] = {(float)(bc[4] / ws), (float)(bc[5] / ws), (float)(bc[6] / ws)};
+        uint32_t nu_loc = 0;
+        bool bad = false;
+        for (uint32_t t = s0 + tid; t < e0; t += kB) {
+            if (!keep[t]) continue;
+            const float4 q = qw[t];
+            const float4 u = vv[t];
+            const float d0 = __fadd_rn(__fsub_rn(u.x, vc32[0]), __fmul_rn(b32, __fsub_rn(q.x, qc32[0])));
+            const float d1 = __fadd_rn(__fsub_rn(u.y, vc32[1]), __fmul_rn(b32, __fsub_rn(q.y, qc32[1])));
+            const float d2 = __fadd_rn(__fsub_rn(u.z, vc32[2]), __fmul_rn(b32, __fsub_rn(q.z, qc32[2])));
+            float s = __fmul_rn(d0, d0);
+            s = __fadd_rn(s, __fmul_rn(d1, d1));
+            s = __fadd_rn(s, __fmul_rn(d2, d2));
+            const float E = __fadd_rn(s / 2.0f, phi[t]);
+            en[t] = E;
+            if (!isfinite(E)) bad = true;
+            if (E >= 0.0f) ++nu_loc;
+        }
+        atomicAdd(&sh[0], nu_loc);
+        if (__syncthreads_or(bad)) { st = -1; break; }
+        const int nu = (int)sh[0];
+        if (nu == 0) {
+            if (deleted == 0) { st = 0; break; }
+            break;  // refresh the potential
+        }
+        int nb = (int)ceil(0.02 * (double)np_bound);
+        nb = nb < 1 ? 1 : nb;
+        nb = nb > nu ? nu : nb;
+        // radix select of the nb-th largest energy key among unbound members
+        uint32_t prefix = 0, mask = 0;
+        uint32_t need = (uint32_t)nb;
+        for (int byte = 3; byte >= 0; --byte) {
+            for (int i = tid; i < 256; i += kB) hist[i] = 0;
+            __syncthreads();
+            for (uint32_t t = s0 + tid; t < e0; t += kB) {
+                if (!keep[t]) continue;
+                const float E = en[t];
+                if (!(E >= 0.0f)) continue;
+                const uint32_t k = ekey(E);
+                if ((k & mask) == prefix) atomicAdd(&hist[(k >> (8 * byte)) & 255u], 1u);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t rem = need;
+                int dsel = 0;
+                for (int d = 255; d >= 0; --d) {
+                    if (hist[d] >= rem) { dsel = d; break; }
+                    rem -= hist[d];
+                }
+                sh[1] = (uint32_t)dsel;
+                sh[2] = rem;
+            }
+            __syncthreads();
+            prefix |= sh[1] << (8 * byte);
+            mask |= 255u << (8 * byte);
+            need = sh[2];
+            __syncthreads();
+        }
+        // delete keys > T, and the `need` smallest indices with key == T
+        const uint32_t T = prefix;
+        uint32_t taken = 0;  // running count of key == T seen in index order
+        for (uint32_t base = s0; base < e0; base += kB) {
+            const uint32_t t = base + tid;
+            bool del = false, tie = false;
+            if (t < e0 && keep[t]) {
+                const float E = en[t];
+                if (E >= 0.0f) {
+                    const uint32_t k = ekey(E);
+                    if (k > T) del = true;
+                    else if (k == T) tie = true;
+                }
+            }
+            // rank of this tie inside the 256-wide chunk
+            const uint64_t bal = __ballot(tie);
+            const uint32_t in_wave = __popcll(bal & lanemask_lt());
+            if (__lane_id() == 0) sh[4 + w] = __popcll(bal);
+            __syncthreads();
+            uint32_t before = taken;
+            for (int k = 0; k < w; ++k) before += sh[4 + k];
+            if (tie && before + in_wave < need) del = true;
+            if (del) keep[t] = 0;
+            uint32_t chunk = 0;
+            for (int k = 0; k < kB / kWave; ++k) chunk += sh[4 + k];
+            taken += chunk;
+            __syncthreads();
+        }
+        np_bound -= nb;
+        deleted += nb;
+        if (np_bound < P.np_min) { st = 1; break; }
+        if ((double)deleted >= 0.2 * (double)np_bound) break;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        npb[h] = np_bound;
+        if (st == -1) {
+            atomicCAS(err, 0, 1);
+            status[h] = 5;
+            active[h] = 0;
+        } else if (st >= 0) {
+            status[h] = st;
+            active[h] = 0;
+        } else if (last_pass) {
+            status[h] = 5;
+            active[h] = 0;
+        }
+    }
+    // kept mass for the next potential (zero-mode term)
+    double m = 0.0;
+    for (uint32_t t = s0 + tid; t < e0; t += kB)
+        if (keep[t]) m += (double)qw[t].w;
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_down(m, o, kWave);
+    if (__lane_id() == 0) red[0][w] = m;
+    __syncthreads();
+    if (tid == 0) {
+        double sm = 0.0;
+        for (int j = 0; j < kB / kWave; ++j) sm += red[0][j];
+        mkeep[h] = sm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// write-back (fof_fortran.f90:313-351)
+
+__global__ void k_valid_flags(const int32_t* __restrict__ status, const int32_t* __restrict__ npb,
+                              uint32_t nh, int32_t np_min, uint32_t* __restrict__ flag) {
+    const uint32_t h = blockIdx.x * kB + threadIdx.x;
+    if (h < nh) flag[h] = (status[h] == 0 && npb[h] >= np_min) ? 1u : 0u;
+}
+
+__global__ void k_bound_write(const uint32_t* __restrict__ members, const uint32_t* __restrict__ mhalo,
+                              const uint8_t* __restrict__ keep, const uint32_t* __restrict__ vflag,
+                              const uint32_t* __restrict__ vidx, uint32_t nmem,
+                              int32_t* __restrict__ in_halo, int32_t* __restrict__ halo_id,
+                              uint8_t* __restrict__ keep_out) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    const bool ok = vflag[h] && keep[t];
+    const uint32_t i = members[t];
+    in_halo[i] = ok ? 1 : 0;
+    halo_id[i] = ok ? (int32_t)vidx[h] + 1 : 0;
+    keep_out[t] = ok ? 1 : 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host
+
+namespace {
+
+struct SelCtx {
+    int ngrid, bn[4], n_max, m_max;
+    double s15, spacing_max;
+    int fgl_nlev[4];
+    double fgl_sp[4][32], fgl_cov[4][32];
+    int ntot, tab_gid[64], tab_lev[64];
+    double tab_cov[64];
+};
+
+// bound_set_context + level tables (bound_fortran.f90:291-330, 1854-1955)
+void sel_init(SelCtx& c, double s15, int64_t ng) {
+    static const int cand[4] = {48, 64, 128, 256};
+    memset(&c, 0, sizeof(c));
+    const int ngi = (int)ng;
+    c.ngrid = 1 + (ngi >= 64) + (ngi >= 128) + (ngi >= 256);
+    for (int g = 0; g < 4; ++g) c.bn[g] = std::min(cand[g], ngi);
+    c.n_max = std::min(256, ngi);
+    c.m_max = c.n_max / 2 - 3;
+    c.s15 = s15;
+    c.spacing_max = s15 / (double)c.m_max;
+    for (int g = 1; g <= c.ngrid; ++g) {
+        const int m = c.bn[g - 1] / 2 - 3;
+        int nl = 0;
+        double sp = 1.0 / 32.0;
+        while ((double)m * sp < s15 && nl < 32) { c.fgl_sp[g - 1][nl++] = sp; sp = 2.0 * sp; }
+        if (g == c.ngrid && nl < 32) c.fgl_sp[g - 1][nl++] = c.spacing_max;
+        c.fgl_nlev[g - 1] = nl;
+        for (int l = 0; l < nl; ++l) c.fgl_cov[g - 1][l] = (double)m * c.fgl_sp[g - 1][l];
+    }
+    int nt = 0;
+    for (int g = 1; g <= c.ngrid; ++g)
+        for (int l = 0; l < c.fgl_nlev[g - 1]; ++l) {
+            c.tab_gid[nt] = g; c.tab_lev[nt] = l + 1; c.tab_cov[nt] = c.fgl_cov[g - 1][l]; ++nt;
+        }
+    for (int j = 0; j < nt - 1; ++j) {  // selection sort by coverage (bound_build_ladder)
+        int k = j;
+        for (int i = j + 1; i < nt; ++i) if (c.tab_cov[i] < c.tab_cov[k]) k = i;
+        if (k != j) {
+            std::swap(c.tab_gid[j], c.tab_gid[k]);
+            std::swap(c.tab_lev[j], c.tab_lev[k]);
+            std::swap(c.tab_cov[j], c.tab_cov[k]);
+        }
+    }
+    c.ntot = nt;
+}
+
+// bound_select_grid_and_level (bound_fortran.f90:1963-2019); 0 ok, 2 none
+int sel_grid(const SelCtx& c, int64_t np, double span, int& gid, double& sp_out) {
+    static const int ncand[3] = {11, 10, 8};
+    static const int cg[3][12] = {{1, 2, 2, 3, 3, 3, 4, 4, 4, 4, 4},
+                                  {3, 3, 4, 4, 4, 4, 4, 4, 4, 4},
+                                  {4, 4, 4, 4, 4, 4, 4, 4}};
+    static const double cs[3][12] = {
+        {0.125, 0.125, 0.25, 0.125, 0.25, 0.5, 0.0625, 0.125, 0.25, 0.5, -1.0},
+        {0.0625, 0.125, 0.0625, 0.125, 0.25, 0.5, 1.0, 2.0, 4.0, -1.0},
+        {0.0625, 0.125, 0.25, 0.5, 1.0, 2.0, 4.0, -1.0}};
+    gid = 0;
+    sp_out = 0.0;
+    if (np <= 1000) return 0;
+    const int band = np <= 5000 ? 0 : (np <= 20000 ? 1 : 2);
+    for (int s = 0; s < ncand[band]; ++s) {
+        const int g = cg[band][s];
+        if (g < 1 || g > c.ngrid) continue;
+        double sp = cs[band][s];
+        if (sp == -1.0) sp = c.spacing_max;
+        int l = 0;
+        for (int k = 0; k < c.fgl_nlev[g - 1]; ++k)
+            if (fabs(c.fgl_sp[g - 1][k] - sp) <= 1e-12 * std::max(1.0, fabs(sp))) { l = k + 1; break; }
+        if (l == 0) continue;
+        if (c.fgl_cov[g - 1][l - 1] < span) continue;
+        gid = g;
+        sp_out = c.fgl_sp[g - 1][l - 1];
+        return 0;
+    }
+    if (span <= c.s15) {
+        for (int i = 0; i < c.ntot; ++i)
+            if (c.tab_cov[i] >= span) {
+                gid = c.tab_gid[i];
+                sp_out = c.fgl_sp[gid - 1][c.tab_lev[i] - 1];
+                return 0;
+            }
+    }
+    return 2;
+}
+
+GreenCache& green_for(Ctx& C, int n, hipStream_t s) {
+    GreenCache*& gc = C.greens[n];
+    if (!gc) gc = new GreenCache();
+    if (gc->ready) return *gc;
+    gc->n = n;
+    const uint64_t tot = (uint64_t)n * n * n;
+    float* G = gc->G.as<float>(tot);
+    float2* tw = gc->tw.as<float2>(n);
+    fft_make_plan(gc->plan, n, tw, s);
+    k_green<<<blocks(tot), kB, 0, s>>>(G, n);
+    PYFOF_LAUNCH_CHECK();
+    // self kernel: inverse transform of G, / n^3, at the 5^3 corner
+    DevBuf tmp;
+    float2* g = tmp.as<float2>(tot);
+    int32_t* slot0 = C.buf("green_slot").as<int32_t>(1);
+    PYFOF_HIP(hipMemsetAsync(slot0, 0, sizeof(int32_t), s));
+    k_green_to_complex<<<blocks(tot), kB, 0, s>>>(G, g, tot);
+    PYFOF_LAUNCH_CHECK();
+    fft3d(gc->plan, g, slot0, 1, true, s);
+    float* self = gc->self.as<float>(125);
+    k_self_extract<<<1, 128, 0, s>>>(g, n, self);
+    PYFOF_LAUNCH_CHECK();
+    PYFOF_HIP(hipStreamSynchronize(s));
+    tmp.release();
+    gc->ready = true;
+    return *gc;
+}
+
+}  // namespace
+
+int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& res) {
+    hipStream_t s = C.stream;
+    res = BoundResult();
+    const uint32_t nh = A.nh, M = A.nmem;
+    if (nh == 0 || M == 0) return IERR_OK;
+    SelCtx sc;
+    sel_init(sc, A.s15, A.ng);
+    // bound_fortran.f90:1559-1563, 1586-1590
+    if (sc.ngrid != 4) return IERR_INVALID;
+    if (!(A.a > 0.0) || !std::isfinite(A.a) || !std::isfinite(A.b)) return IERR_INVALID;
+    Params P{A.app, A.a, A.b, A.L, A.Lh, A.np_min};
+
+    float4* qw = C.buf("qw").as<float4>(M);
+    float4* vv = C.buf("vv").as<float4>(M);
+    k_bound_pack<<<blocks(M), kB, 0, s>>>(A.members, A.mhalo, A.anchor, A.xp, A.vp, A.mp, M, A.L, A.Lh, qw, vv);
+    PYFOF_LAUNCH_CHECK();
+    float* d_lohi = C.buf("lohi").as<float>(6 * (size_t)nh);
+    k_bound_span<<<nh, kB, 0, s>>>(qw, A.hoff, d_lohi);
+    PYFOF_LAUNCH_CHECK();
+    std::vector<float> lohi(6 * (size_t)nh);
+    std::vector<uint32_t> hoff(nh);
+    PYFOF_HIP(hipMemcpyAsync(lohi.data(), d_lohi, 4 * lohi.size(), hipMemcpyDeviceToHost, s));
+    PYFOF_HIP(hipMemcpyAsync(hoff.data(), A.hoff, 4 * (size_t)nh, hipMemcpyDeviceToHost, s));
+    PYFOF_HIP(hipStreamSynchronize(s));
+
+    // classification in candidate order (bound_fortran.f90:2940-3055)
+    std::vector<int32_t> pm_of_halo(nh, -1), dlist;
+    std::vector<PmHalo> pm;
+    std::vector<int32_t> pm_halo;
+    for (uint32_t h = 0; h < nh; ++h) {
+        const int64_t n = (int64_t)hoff[h] - (h ? (int64_t)hoff[h - 1] : 0);
+        const float* lh = &lohi[6 * (size_t)h];
+        float span = lh[3] - lh[0];
+        span = std::max(span, lh[4] - lh[1]);
+        span = std::max(span, lh[5] - lh[2]);
+        if (!std::isfinite(span)) return IERR_INVALID;
+        if (n > 1000 && span > (float)A.s15) return IERR_SPAN;
+        int gid;
+        double sp;
+        if (sel_grid(sc, n, (double)span, gid, sp) != 0) return IERR_INVALID;
+        if (gid == 0) {
+            if (n > kDirectMax) return IERR_INVALID;
+            dlist.push_back((int32_t)h);
+            continue;
+        }
+        const float sp32 = (float)sp;
+        if ((double)(sc.bn[gid - 1] / 2 - 3) * (double)sp32 < (double)span) return IERR_INVALID;
+        PmHalo H;
+        memset(&H, 0, sizeof(H));
+        H.origin = make_float4(lh[0] - sp32, lh[1] - sp32, lh[2] - sp32, sp32);
+        H.cls = gid - 1;
+        H.n = sc.bn[gid - 1];
+        pm_of_halo[h] = (int32_t)pm.size();
+        pm.push_back(H);
+        pm_halo.push_back((int32_t)h);
+    }
+    res.n_direct = (uint32_t)dlist.size();
+
+    int32_t* status = C.buf("bstatus").as<int32_t>(nh);
+    int32_t* npb = C.buf("bnpb").as<int32_t>(nh);
+    uint8_t* keep = C.buf("bkeep").as<uint8_t>(M);
+    int32_t* err = C.buf("berr").as<int32_t>(4);
+    PYFOF_HIP(hipMemsetAsync(err, 0, 16, s));
+    PYFOF_HIP(hipMemsetAsync(keep, 1, M, s));
+    {
+        std::vector<int32_t> npb0(nh), st0(nh, 5);
+        for (uint32_t h = 0; h < nh; ++h) npb0[h] = (int32_t)(hoff[h] - (h ? hoff[h - 1] : 0));
+        PYFOF_HIP(hipMemcpyAsync(npb, npb0.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
+        PYFOF_HIP(hipMemcpyAsync(status, st0.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
+    }
+    // ---- direct-summation halos
+    if (!dlist.empty()) {
+        int32_t* d_dl = C.buf("dlist").as<int32_t>(dlist.size());
+        PYFOF_HIP(hipMemcpyAsync(d_dl, dlist.data(), 4 * dlist.size(), hipMemcpyHostToDevice, s));
+        k_direct_solve<<<(uint32_t)dlist.size(), kB, 0, s>>>(d_dl, A.hoff, qw, vv, keep, status, npb, err, P);
+        PYFOF_LAUNCH_CHECK();
+    }
+    // ---- particle-mesh halos
+    const uint32_t npm = (uint32_t)pm.size();
+    int32_t passes = 0;
+    if (npm > 0) {
+        // grid slots per class, Green functions and self kernels
+        int nclass[4] = {0, 0, 0, 0};
+        for (auto& H : pm) H.slot = nclass[H.cls]++;
+        for (int g = 0; g < 4; ++g) res.n_pm[g] = (uint32_t)nclass[g];
+        float* selfs = C.buf("selfs").as<float>(4 * 125);
+        const float* Gfull[4] = {nullptr, nullptr, nullptr, nullptr};
+        const FftPlan* plans[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int g = 0; g < 4; ++g) {
+            if (!nclass[g]) continue;
+            GreenCache& gc = green_for(C, sc.bn[g], s);
+            Gfull[g] = gc.G.as<float>(1);
+            plans[g] = &gc.plan;
+            PYFOF_HIP(hipMemcpyAsync(selfs + 125 * g, gc.self.as<float>(125), 125 * 4, hipMemcpyDeviceToDevice, s));
+        }
+        // C2 cell grids per PM halo (bound_fortran.f90:3125-3145)
+        std::vector<int32_t> pm_list(npm);
+        for (uint32_t k = 0; k < npm; ++k) pm_list[k] = (int32_t)k;
+        int32_t* d_pmlist = C.buf("pmlist").as<int32_t>(npm);
+        int32_t* d_pmhalo = C.buf("pmhalo").as<int32_t>(npm);
+        PmHalo* d_pm = C.buf("pm").as<PmHalo>(npm);
+        int32_t* d_pmof = C.buf("pmof").as<int32_t>(nh);
+        PYFOF_HIP(hipMemcpyAsync(d_pmlist, pm_list.data(), 4 * (size_t)npm, hipMemcpyHostToDevice, s));
+        PYFOF_HIP(hipMemcpyAsync(d_pmhalo, pm_halo.data(), 4 * (size_t)npm, hipMemcpyHostToDevice, s));
+        PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
+        PYFOF_HIP(hipMemcpyAsync(d_pmof, pm_of_halo.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
+        double* uext = C.buf("uext").as<double>(6 * (size_t)npm);
+        k_c2_extent<<<npm, kB, 0, s>>>(qw, A.hoff, d_pmlist, d_pmhalo, d_pm, uext);
+        PYFOF_LAUNCH_CHECK();
+        std::vector<double> ue(6 * (size_t)npm);
+        PYFOF_HIP(hipMemcpyAsync(ue.data(), uext, 8 * ue.size(), hipMemcpyDeviceToHost, s));
+        PYFOF_HIP(hipStreamSynchronize(s));
+        uint64_t c2cells = 0;
+        for (uint32_t k = 0; k < npm; ++k) {
+            PmHalo& H = pm[k];
+            for (int d = 0; d < 3; ++d) {
+                const int nc = (int)floor((ue[6 * k + 3 + d] - ue[6 * k + d]) / 3.5) + 1;
+                H.nc[d] = nc < 1 ? 1 : nc;
+                H.umin[d] = ue[6 * k + d];
+            }
+            H.c2off = (int32_t)c2cells;
+            c2cells += (uint64_t)H.nc[0] * H.nc[1] * H.nc[2];
+
+        }
+        if (c2cells >= 0x7fffffffull) return IERR_TOO_LARGE;
+        PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
+        // member -> C2 cell; members sorted by (cell, member index)
+        uint32_t* ck = C.buf("c2k").as<uint32_t>(M);
+        uint32_t* cv = C.buf("c2v").as<uint32_t>(M);
+        uint32_t* ck2 = C.buf("c2k2").as<uint32_t>(M);
+        uint32_t* cv2 = C.buf("c2v2").as<uint32_t>(M);
+        k_c2_keys<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, M, (uint32_t)c2cells, ck, cv);
+        PYFOF_LAUNCH_CHECK();
+        void* tmp = C.buf("c2tmp").get(std::max(radix_temp_bytes(M), scan_temp_bytes(c2cells + 1)) + 256);
+        const bool alt = radix_sort_pairs(ck, cv, ck2, cv2, M, bits_for(c2cells), tmp, s);
+        const uint32_t* skeys = alt ? ck2 : ck;
+        const uint32_t* clist = alt ? cv2 : cv;
+        uint32_t* cell_of = C.buf("c2cellof").as<uint32_t>(M);
+        uint32_t* cstart = C.buf("c2start").as<uint32_t>(c2cells + 1);
+        k_fill32<<<blocks(c2cells + 1), kB, 0, s>>>(cstart, c2cells + 1, 0u);
+        PYFOF_LAUNCH_CHECK();
+        k_c2_cells<<<blocks(M), kB, 0, s>>>(skeys, clist, M, (uint32_t)c2cells, cell_of, cstart + 1);
+        PYFOF_LAUNCH_CHECK();
+        inclusive_max_scan_u32(cstart + 1, cstart + 1, c2cells, tmp, s);
+
+        // TSC setup
+        int4* first = C.buf("first").as<int4>(M);
+        float* wt = C.buf("wt").as<float>(9 * (size_t)M);
+        float* selfv = C.buf("selfv").as<float>(M);
+        k_tsc_setup<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, selfs, M, first, wt, selfv, err);
+        PYFOF_LAUNCH_CHECK();
+        // grids per class
+        unsigned long long* acc[4] = {nullptr, nullptr, nullptr, nullptr};
+        float2* grid[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int g = 0; g < 4; ++g) {
+            if (!nclass[g]) continue;
+            const uint64_t gsz = (uint64_t)sc.bn[g] * sc.bn[g] * sc.bn[g] * (uint64_t)nclass[g];
+            acc[g] = C.buf("acc" + std::to_string(g)).as<unsigned long long>(gsz);
+            grid[g] = C.buf("grid" + std::to_string(g)).as<float2>(gsz);
+        }
+        // halo of each grid slot, per class
+        std::vector<std::vector<int32_t>> slot_halo(4);
+        for (int g = 0; g < 4; ++g) slot_halo[g].assign(nclass[g], 0);
+        for (uint32_t k = 0; k < npm; ++k) slot_halo[pm[k].cls][pm[k].slot] = pm_halo[k];
+        int32_t* d_slot_halo[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int g = 0; g < 4; ++g) {
+            if (!nclass[g]) continue;
+            d_slot_halo[g] = C.buf("slothalo" + std::to_string(g)).as<int32_t>(nclass[g]);
+            PYFOF_HIP(hipMemcpyAsync(d_slot_halo[g], slot_halo[g].data(), 4 * (size_t)nclass[g], hipMemcpyHostToDevice, s));
+        }
+        double* fscale = C.buf("fscale").as<double>(nh);
+        float* phi_c1 = C.buf("phic1").as<float>(M);
+        float* phi_c2 = C.buf("phic2").as<float>(M);
+        float* phi = C.buf("phi").as<float>(M);
+        float* en = C.buf("energy").as<float>(M);
+        uint8_t* active = C.buf("active").as<uint8_t>(nh);
+        double* mkeep = C.buf("mkeep").as<double>(nh);
+        std::vector<uint8_t> act_host(nh, 0);
+        for (uint32_t k = 0; k < npm; ++k) act_host[pm_halo[k]] = 1;
+        PYFOF_HIP(hipMemcpyAsync(active, act_host.data(), nh, hipMemcpyHostToDevice, s));
+        k_halo_mass<<<npm, kB, 0, s>>>(qw, A.hoff, d_pmhalo, mkeep, fscale);
+        PYFOF_LAUNCH_CHECK();
+
+        std::vector<int32_t> act_list = pm_halo;
+        int32_t* d_alist = C.buf("alist").as<int32_t>(npm);
+        int32_t* d_slots = C.buf("slots").as<int32_t>(npm);
+        for (int pass = 1; pass <= 20 && !act_list.empty(); ++pass) {
+            passes = pass;
+            for (int g = 0; g < 4; ++g) {
+                std::vector<int32_t> sl;
+                for (int32_t h : act_list) {
+                    const PmHalo& H = pm[pm_of_halo[h]];
+                    if (H.cls == g) sl.push_back(H.slot);
+                }
+                if (sl.empty()) continue;
+                const int n = sc.bn[g];
+                const uint64_t gsz = (uint64_t)n * n * n;
+                PYFOF_HIP(hipMemcpyAsync(d_slots, sl.data(), 4 * sl.size(), hipMemcpyHostToDevice, s));
+                const dim3 gg(blocks(gsz), (uint32_t)sl.size());
+                k_zero_u64<<<gg, kB, 0, s>>>(acc[g], d_slots, gsz);
+                PYFOF_LAUNCH_CHECK();
+                k_deposit<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, active, fscale,
+                                                   g, M, acc[g]);
+                PYFOF_LAUNCH_CHECK();
+                k_fix_to_complex<<<gg, kB, 0, s>>>(acc[g], grid[g], d_slots, d_slot_halo[g], fscale, n);
+                PYFOF_LAUNCH_CHECK();
+                fft3d(*plans[g], grid[g], d_slots, (int32_t)sl.size(), false, s);
+                k_mul_green<<<gg, kB, 0, s>>>(grid[g], Gfull[g], d_slots, gsz);
+                PYFOF_LAUNCH_CHECK();
+                fft3d(*plans[g], grid[g], d_slots, (int32_t)sl.size(), true, s);
+                k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
+                                                  mkeep, grid[g], g, M, A.a, phi_c1);
+                PYFOF_LAUNCH_CHECK();
+            }
+            k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
+                                               clist, M, A.a, A.app, phi_c2);
+            PYFOF_LAUNCH_CHECK();
+            k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, keep, active, npb, phi_c1, phi_c2, M,
+                                                   phi, err);
+            PYFOF_LAUNCH_CHECK();
+            PYFOF_HIP(hipMemcpyAsync(d_alist, act_list.data(), 4 * act_list.size(), hipMemcpyHostToDevice, s));
+            k_pm_inner<<<(uint32_t)act_list.size(), kB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, en, keep,
+                                                                status, npb, active, mkeep, pass == 20,
+                                                                err, P);
+            PYFOF_LAUNCH_CHECK();
+            int32_t herr = 0;
+            PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipMemcpyAsync(act_host.data(), active, nh, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipStreamSynchronize(s));
+            if (herr) break;
+            std::vector<int32_t> next;
+            for (int32_t h : act_list) if (act_host[h]) next.push_back(h);
+            act_list.swap(next);
+        }
+    }
+    res.passes = passes;
+    int32_t herr = 0;
+    PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    PYFOF_HIP(hipStreamSynchronize(s));
+    if (herr) return herr;
+    // ---- write-back: valid halos renumbered in candidate order
+    uint32_t* vflag = C.buf("vflag").as<uint32_t>(nh);
+    uint32_t* vidx = C.buf("vidx").as<uint32_t>(nh);
+    uint32_t* vtot = C.buf("vtot").as<uint32_t>(1);
+    k_valid_flags<<<blocks(nh), kB, 0, s>>>(status, npb, nh, A.np_min, vflag);
+    PYFOF_LAUNCH_CHECK();
+    void* stmp = C.buf("vtmp").get(scan_temp_bytes(nh) + 256);
+    exclusive_scan_u32(vflag, vidx, nh, vtot, stmp, s);
+    k_bound_write<<<blocks(M), kB, 0, s>>>(A.members, A.mhalo, keep, vflag, vidx, M, A.in_halo, A.halo_id, keep_out);
+    PYFOF_LAUNCH_CHECK();
+    std::vector<int32_t> st_h(nh);
+    PYFOF_HIP(hipMemcpyAsync(&res.nvalid, vtot, 4, hipMemcpyDeviceToHost, s));
+    PYFOF_HIP(hipMemcpyAsync(st_h.data(), status, 4 * (size_t)nh, hipMemcpyDeviceToHost, s));
+    PYFOF_HIP(hipStreamSynchronize(s));
+    for (uint32_t h = 0; h < nh; ++h) {
+        if (st_h[h] == 0) ++res.n_ok;
+        else if (st_h[h] == 1) ++res.n_too_few;
+        else ++res.n_max_iter;
+    }
+    return IERR_OK;
+}
+
+}  // namespace pyfof

@@ -1,0 +1,120 @@
+// Per-device context: stream, events and the reusable HBM workspace.
+#pragma once
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+
+
+namespace pyfof {
+
+// Uploads the constant tables of the FoF kernels to the current device.
+void fof_init_device_tables();
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (bytes > cap) {
+            if (p) PYFOF_HIP(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            const size_t want = bytes + bytes / 8 + 256;
+            hipError_t e = hipMalloc(&p, want);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                p = nullptr;
+                throw HipError(hipErrorOutOfMemory, "device allocation failed");
+            }
+            cap = want;
+        }
+        return p;
+    }
+    template <class T>
+    T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Outcome of the last unbinding (mirrors BoundResult).
+struct BoundStats {
+    uint32_t nvalid = 0;
+    uint32_t n_ok = 0, n_too_few = 0, n_max_iter = 0;
+    uint32_t n_direct = 0, n_pm[4] = {0, 0, 0, 0};
+    int32_t passes = 0;
+};
+
+// Green function of one isolated grid size (depends on n only), cached.
+struct GreenCache {
+    int n = 0;
+    DevBuf G;      // full-spectrum real G, n^3 floats [z][y][x]
+    DevBuf tw;     // FFT twiddles
+    DevBuf self;   // 5^3 self kernel (device)
+    FftPlan plan;
+    bool ready = false;
+};
+
+enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvCat, kEvBound, kNumEv };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[kNumEv] = {};
+    bool ev_recorded[kNumEv] = {};
+    DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
+        xc, parent, label, size, flag, cidx, aux, dense, sub_ptr, sub_list, subs, ncomp, croot, needs_sub, temp, out_in, out_id,
+        out_root, anchor, hregion, scal, cat_keys, cat_vals, cat_keys2, cat_vals2, hoff, am_h, am,
+        pin;
+    // last call's sizes and catalog
+    uint64_t last_n = 0, last_ncell = 0, last_ndense = 0;
+    bool want_catalog = false;
+    BoundStats last_bound;  // build the fof_with_info catalog on every call
+    uint32_t cat_nh = 0;
+    uint64_t cat_nmem = 0;
+    uint32_t* cat_members = nullptr;  // device, grouped by halo
+    uint32_t* cat_hoff = nullptr;     // device, inclusive ends
+    std::mutex mu;
+    std::map<std::string, DevBuf> named;  // workspace of the unbinding solver
+    std::map<int, GreenCache*> greens;
+    DevBuf& buf(const std::string& name) { return named[name]; }
+
+    explicit Ctx(int dev) : device(dev) {
+        PYFOF_HIP(hipSetDevice(dev));
+        fof_init_device_tables();
+        PYFOF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
+    }
+    ~Ctx() {
+        (void)hipSetDevice(device);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
+                          &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
+                          &sub_ptr, &sub_list, &subs, &ncomp, &croot, &needs_sub, &temp, &out_in, &out_id, &out_root, &anchor,
+                          &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
+                          &am_h, &am, &pin})
+            b->release();
+        for (auto& kv : named) kv.second.release();
+        for (auto& kv : greens) {
+            kv.second->G.release();
+            kv.second->tw.release();
+            kv.second->self.release();
+            delete kv.second;
+        }
+    }
+    void mark(int e) {
+        PYFOF_HIP(hipEventRecord(ev[e], stream));
+        ev_recorded[e] = true;
+    }
+};
+
+
+}  // namespace pyfof

@@ -1,0 +1,28 @@
+// Batched 3-D complex FFT on n^3 grids, n = 2^a 3^b (48, 64, 128, 216, 256).
+//
+// One kernel per axis: a workgroup loads 8 lines of the axis into LDS
+// (coalesced along x for every axis), runs a Stockham autosort mixed-radix
+// (4, 2, 3) transform on them, and stores them back. Twiddles come from a
+// per-size table exp(-2 pi i k / n) computed in double on the host.
+#pragma once
+#include "common.hpp"
+
+namespace pyfof {
+
+struct FftPlan {
+    int n = 0;
+    int nrad = 0;
+    int radix[16] = {};
+    float2* tw = nullptr;  // device table, n entries
+};
+
+// Builds the plan (radix sequence and twiddle table in `tw_dev`, n float2).
+void fft_make_plan(FftPlan& p, int n, float2* tw_dev, hipStream_t s);
+
+// In-place forward (inverse = false: exp(-i)) or inverse (exp(+i),
+// unnormalised) transform of the grids whose slot numbers are listed in
+// `slots` (nslots entries); grid g starts at data + slot * n^3.
+void fft3d(const FftPlan& p, float2* data, const int32_t* slots, int32_t nslots, bool inverse,
+           hipStream_t s);
+
+}  // namespace pyfof

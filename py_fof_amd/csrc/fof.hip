@@ -24,6 +24,8 @@
 #include "fof_geom.hpp"
 #include "fof_kernels.hpp"
 #include "prims.hpp"
+#include "ctx.hpp"
+#include "bound.hpp"
 
 namespace pyfof {
 
@@ -850,6 +852,16 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     }
 }
 
+// anchor and region of each valid halo under its new (bound) number
+__global__ void k_remap_valid(const uint32_t* __restrict__ vflag, const uint32_t* __restrict__ vidx,
+                              const uint32_t* __restrict__ anchor, const int32_t* __restrict__ hreg,
+                              uint32_t nh, uint32_t* __restrict__ anchor2, int32_t* __restrict__ hreg2) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh || !vflag[h]) return;
+    anchor2[vidx[h]] = anchor[h];
+    hreg2[vidx[h]] = hreg[h];
+}
+
 // catalog: in-halo particles compacted in index order, keyed by halo id - 1
 __global__ void k_cat_flags(const int32_t* __restrict__ in_halo, uint32_t n, uint32_t* __restrict__ f) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -960,57 +972,7 @@ __global__ void __launch_bounds__(kBlk) k_region_angmom(const double* __restrict
 // ---------------------------------------------------------------------------
 // host side
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    void* get(size_t bytes) {
-        if (bytes == 0) bytes = 16;
-        if (bytes > cap) {
-            if (p) PYFOF_HIP(hipFree(p));
-            p = nullptr;
-            cap = 0;
-            const size_t want = bytes + bytes / 8 + 256;
-            hipError_t e = hipMalloc(&p, want);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                p = nullptr;
-                throw HipError(hipErrorOutOfMemory, "device allocation failed");
-            }
-            cap = want;
-        }
-        return p;
-    }
-    template <class T>
-    T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvCat, kEvBound, kNumEv };
-
-struct Ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[kNumEv] = {};
-    bool ev_recorded[kNumEv] = {};
-    DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
-        xc, parent, label, size, flag, cidx, aux, dense, sub_ptr, sub_list, subs, ncomp, croot, needs_sub, temp, out_in, out_id,
-        out_root, anchor, hregion, scal, cat_keys, cat_vals, cat_keys2, cat_vals2, hoff, am_h, am,
-        pin;
-    // last call's sizes and catalog
-    uint64_t last_n = 0, last_ncell = 0, last_ndense = 0;
-    bool want_catalog = false;  // build the fof_with_info catalog on every call
-    uint32_t cat_nh = 0;
-    uint64_t cat_nmem = 0;
-    uint32_t* cat_members = nullptr;  // device, grouped by halo
-    uint32_t* cat_hoff = nullptr;     // device, inclusive ends
-    std::mutex mu;
-
-    explicit Ctx(int dev) : device(dev) {
-        PYFOF_HIP(hipSetDevice(dev));
+void fof_init_device_tables() {
         // forward offsets of [-2,2]^3, nearest first (|o|^2 = 1, 2, 3, 4, ...)
         int8_t off[62][3];
         int n = 0;
@@ -1024,25 +986,7 @@ struct Ctx {
                 ++n;
             }
         PYFOF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fwd_off), off, sizeof(off)));
-        PYFOF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
-    }
-    ~Ctx() {
-        (void)hipSetDevice(device);
-        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-        for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
-                          &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
-                          &sub_ptr, &sub_list, &subs, &ncomp, &croot, &needs_sub, &temp, &out_in, &out_id, &out_root, &anchor,
-                          &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
-                          &am_h, &am, &pin})
-            b->release();
-    }
-    void mark(int e) {
-        PYFOF_HIP(hipEventRecord(ev[e], stream));
-        ev_recorded[e] = true;
-    }
-};
+}
 
 static thread_local std::string g_last_error;
 
@@ -1060,6 +1004,8 @@ struct CallArgs {
     int32_t np_min;
     int32_t bound_check;
     int32_t verbose;
+    double softening, a_val, b_val, s_15;
+    int64_t ng;
     int32_t total_np;
     int32_t* in_halo;
     int32_t* halo_id;
@@ -1078,6 +1024,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
     C.cat_nmem = 0;
     C.cat_hoff = nullptr;
     C.cat_members = nullptr;
+    C.last_bound = BoundStats();
     C.mark(kEvStart);
 
     // inputs in HBM
@@ -1088,12 +1035,12 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         double* d = C.xp.as<double>(3 * (size_t)N);
         PYFOF_HIP(hipMemcpyAsync(d, A.xp, 24 * (size_t)N, hipMemcpyHostToDevice, s));
         xp = d;
-        if (A.vp && A.ang_mom) {
+        if (A.vp && (A.ang_mom || A.bound_check)) {
             double* dv = C.vp.as<double>(3 * (size_t)N);
             PYFOF_HIP(hipMemcpyAsync(dv, A.vp, 24 * (size_t)N, hipMemcpyHostToDevice, s));
             vp = dv;
         }
-        if (A.mp && A.ang_mom) {
+        if (A.mp && (A.ang_mom || A.bound_check)) {
             double* dm = C.mp.as<double>((size_t)N);
             PYFOF_HIP(hipMemcpyAsync(dm, A.mp, 8 * (size_t)N, hipMemcpyHostToDevice, s));
             mp = dm;
@@ -1326,42 +1273,83 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipMemcpyAsync(&nh, scal + 1, 4, hipMemcpyDeviceToHost, s));
         C.mark(kEvOut);
 
-        // ---- catalog (members grouped by halo, ascending index) + ang. mom.
+        // ---- catalog (members grouped by halo, ascending index)
         PYFOF_HIP(hipStreamSynchronize(s));
         C.cat_nh = nh;
-        const bool need_groups = C.want_catalog || (A.ang_mom && vp);
-        if (nh > 0 && need_groups) {
+        struct Cat {
+            uint32_t nmem = 0;
+            uint32_t* members = nullptr;
+            uint32_t* mkeys = nullptr;
+            uint32_t* hoff = nullptr;
+        };
+        auto build_catalog = [&](uint32_t nhalo) {
+            Cat c;
             uint32_t* cf = C.flag.as<uint32_t>(N);
             k_cat_flags<<<nblk(N), kBlk, 0, s>>>(o_in, N, cf);
             PYFOF_LAUNCH_CHECK();
             exclusive_scan_u32(cf, cf, N, scal + 2, tmp, s);
-            uint32_t nmem = 0;
-            PYFOF_HIP(hipMemcpyAsync(&nmem, scal + 2, 4, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipMemcpyAsync(&c.nmem, scal + 2, 4, hipMemcpyDeviceToHost, s));
             PYFOF_HIP(hipStreamSynchronize(s));
-            uint32_t* ck = C.cat_keys.as<uint32_t>(nmem);
-            uint32_t* cv = C.cat_vals.as<uint32_t>(nmem);
-            uint32_t* ck2 = C.cat_keys2.as<uint32_t>(nmem);
-            uint32_t* cv2 = C.cat_vals2.as<uint32_t>(nmem);
+            uint32_t* ck = C.cat_keys.as<uint32_t>(c.nmem);
+            uint32_t* cv = C.cat_vals.as<uint32_t>(c.nmem);
+            uint32_t* ck2 = C.cat_keys2.as<uint32_t>(c.nmem);
+            uint32_t* cv2 = C.cat_vals2.as<uint32_t>(c.nmem);
             k_cat_compact<<<nblk(N), kBlk, 0, s>>>(o_id, cf, N, ck, cv);
             PYFOF_LAUNCH_CHECK();
-            const bool calt = radix_sort_pairs(ck, cv, ck2, cv2, nmem, bits_for(nh - 1), tmp, s);
-            uint32_t* mkeys = calt ? ck2 : ck;
-            uint32_t* members = calt ? cv2 : cv;
-            uint32_t* hoff = C.hoff.as<uint32_t>(nh);
-            k_fill_u32<<<nblk(nh), kBlk, 0, s>>>(hoff, nh, 0u);
+            const bool calt = radix_sort_pairs(ck, cv, ck2, cv2, c.nmem, bits_for(nhalo - 1), tmp, s);
+            c.mkeys = calt ? ck2 : ck;
+            c.members = calt ? cv2 : cv;
+            c.hoff = C.hoff.as<uint32_t>(nhalo);
+            k_fill_u32<<<nblk(nhalo), kBlk, 0, s>>>(c.hoff, nhalo, 0u);
             PYFOF_LAUNCH_CHECK();
-            k_run_ends<<<nblk(nmem), kBlk, 0, s>>>(mkeys, nmem, hoff);
+            k_run_ends<<<nblk(c.nmem), kBlk, 0, s>>>(c.mkeys, c.nmem, c.hoff);
             PYFOF_LAUNCH_CHECK();
-            inclusive_max_scan_u32(hoff, hoff, nh, tmp, s);
-            C.cat_nmem = nmem;
-            C.cat_members = members;
-            C.cat_hoff = hoff;
+            inclusive_max_scan_u32(c.hoff, c.hoff, nhalo, tmp, s);
+            return c;
+        };
+        uint32_t nh_out = nh;  // halos in the output numbering
+        if (A.bound_check && nh > 0) {
+            // ---- unbinding of every candidate (fof_fortran.f90:253-356)
+            Cat cand = build_catalog(nh);
+            BoundArgs B;
+            B.app = A.softening; B.a = A.a_val; B.b = A.b_val; B.s15 = A.s_15; B.ng = A.ng;
+            B.np_min = A.np_min; B.verbose = A.verbose; B.L = A.L; B.Lh = P.Lh;
+            B.xp = xp; B.vp = vp; B.mp = mp; B.N = N;
+            B.nh = nh; B.nmem = cand.nmem; B.hoff = cand.hoff; B.members = cand.members;
+            B.mhalo = cand.mkeys; B.anchor = anchor; B.in_halo = o_in; B.halo_id = o_id;
+            uint8_t* bkeep = C.buf("bkeep_out").as<uint8_t>(std::max<uint32_t>(cand.nmem, 1));
+            BoundResult br;
+            const int32_t berr = run_bound(C, B, bkeep, br);
+            C.last_bound = br;
+            if (berr != IERR_OK) {
+                set_error("unbinding failed");
+                return berr;
+            }
+            // anchors and regions of the valid halos, in their new numbering
+            k_remap_valid<<<nblk(nh), kBlk, 0, s>>>(C.buf("vflag").as<uint32_t>(nh), C.buf("vidx").as<uint32_t>(nh),
+                                                    anchor, hreg, nh, C.buf("anchor2").as<uint32_t>(nh),
+                                                    C.buf("hreg2").as<int32_t>(nh));
+            PYFOF_LAUNCH_CHECK();
+            PYFOF_HIP(hipMemcpyAsync(C.anchor.as<uint32_t>(N), C.buf("anchor2").as<uint32_t>(nh), 4 * (size_t)nh,
+                                     hipMemcpyDeviceToDevice, s));
+            PYFOF_HIP(hipMemcpyAsync(C.hregion.as<int32_t>(N), C.buf("hreg2").as<int32_t>(nh), 4 * (size_t)nh,
+                                     hipMemcpyDeviceToDevice, s));
+            nh_out = br.nvalid;
+            C.cat_nh = nh_out;
+            C.mark(kEvBound);
+        }
+        const bool need_groups = C.want_catalog || (A.ang_mom && vp);
+        if (nh_out > 0 && need_groups) {
+            Cat cat = build_catalog(nh_out);
+            C.cat_nmem = cat.nmem;
+            C.cat_members = cat.members;
+            C.cat_hoff = cat.hoff;
             if (A.ang_mom && vp) {
-                double* am_h = C.am_h.as<double>(3 * (size_t)nh);
+                double* am_h = C.am_h.as<double>(3 * (size_t)nh_out);
                 double* am = dev_mem ? A.ang_mom : C.am.as<double>(3 * (size_t)H);
-                k_halo_angmom<<<nh, kBlk, 0, s>>>(xp, vp, mp, hoff, members, anchor, am_h, A.L, P.Lh);
+                k_halo_angmom<<<nh_out, kBlk, 0, s>>>(xp, vp, mp, cat.hoff, cat.members, anchor, am_h, A.L, P.Lh);
                 PYFOF_LAUNCH_CHECK();
-                k_region_angmom<<<H, kBlk, 0, s>>>(am_h, hreg, nh, am);
+                k_region_angmom<<<H, kBlk, 0, s>>>(am_h, hreg, nh_out, am);
                 PYFOF_LAUNCH_CHECK();
                 if (!dev_mem)
                     PYFOF_HIP(hipMemcpyAsync(A.ang_mom, am, 24 * (size_t)H, hipMemcpyDeviceToHost, s));
@@ -1430,8 +1418,8 @@ static int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
         set_error("L_Box and linking_length must be positive");
         return IERR_INVALID;
     }
-    if (A.bound_check) {
-        set_error("bound_check is not available in this build of the library");
+    if (A.bound_check && !(A.softening > 0.0)) {
+        set_error("softening must be positive for bound_check");
         return IERR_INVALID;
     }
     std::lock_guard<std::mutex> lk(C->mu);
@@ -1490,10 +1478,11 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
                       double a_val, double b_val, double s_15_val, int32_t n_threads,
                       int32_t total_np, int64_t ng_val, double ng_global_val, int32_t* in_halo,
                       int32_t* halo_id, double* ang_mom, int32_t* root_out, int32_t mem_kind) {
-    (void)softening; (void)a_val; (void)b_val; (void)s_15_val; (void)ng_val; (void)ng_global_val;
     if (!ctx) { set_error("null context"); return IERR_INVALID; }
     CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
-               verbose, total_np, in_halo, halo_id, ang_mom, root_out, mem_kind};
+               verbose, softening, a_val, b_val, s_15_val, ng_val, total_np, in_halo, halo_id,
+               ang_mom, root_out, mem_kind};
+    (void)ng_global_val;
     return fof_entry(ctx->c, A, n_threads);
 }
 
@@ -1503,7 +1492,6 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
                   double b_val, double s_15_val, int32_t n_threads, int32_t total_np,
                   int64_t ng_val, double ng_global_val, int32_t* in_halo, int32_t* halo_id,
                   double* ang_mom, int32_t mem_kind) {
-    (void)softening; (void)a_val; (void)b_val; (void)s_15_val; (void)ng_val; (void)ng_global_val;
     Ctx* C = nullptr;
     try {
         C = default_ctx();
@@ -1512,7 +1500,9 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
         return IERR_HIP;
     }
     CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
-               verbose, total_np, in_halo, halo_id, ang_mom, nullptr, mem_kind};
+               verbose, softening, a_val, b_val, s_15_val, ng_val, total_np, in_halo, halo_id,
+               ang_mom, nullptr, mem_kind};
+    (void)ng_global_val;
     return fof_entry(C, A, n_threads);
 }
 
@@ -1569,10 +1559,14 @@ int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
 int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n) {
     if (!ctx) return 0;
     const Ctx& C = *ctx->c;
-    const int64_t v[5] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
-                          (int64_t)C.cat_nh, (int64_t)C.cat_nmem};
+    const BoundStats& b = C.last_bound;
+    const int64_t v[15] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
+                           (int64_t)C.cat_nh, (int64_t)C.cat_nmem, (int64_t)b.nvalid,
+                           (int64_t)b.n_ok, (int64_t)b.n_too_few, (int64_t)b.n_max_iter,
+                           (int64_t)b.n_direct, (int64_t)b.n_pm[0], (int64_t)b.n_pm[1],
+                           (int64_t)b.n_pm[2], (int64_t)b.n_pm[3], (int64_t)b.passes};
     int32_t w = 0;
-    for (; w < n && w < 5; ++w) out[w] = v[w];
+    for (; w < n && w < 15; ++w) out[w] = v[w];
     return w;
 }
 

@@ -22,6 +22,8 @@ __all__ = [
     "isotropic",
     "clumpy_plummer",
     "clumpy_nfw",
+    "halo_sizes",
+    "isolated_halos",
 ]
 
 
@@ -159,3 +161,47 @@ def clumpy_nfw(n=134217728, L=512.0, n_clumps=None, frac=0.5, c=8.0, seed=2,
     xp[:, n_cl:] = rng.uniform(0.0, L, (3, n - n_cl))
     xp[:, :n_cl] = _wrap(xp[:, :n_cl], L)
     return xp
+
+
+def halo_sizes(rng, k, alpha=1.9, nmin=20, nmax=1_000_000):
+    """k sizes from dN/dn ~ n^-alpha on [nmin, nmax] (inverse-CDF sampling)."""
+    u = rng.uniform(0.0, 1.0, k)
+    e = 1.0 - alpha
+    n = (nmin ** e + u * (nmax ** e - nmin ** e)) ** (1.0 / e)
+    return np.maximum(nmin, np.floor(n).astype(np.int64))
+
+
+def isolated_halos(n_halos=20_000, seed=3, alpha=1.9, nmin=20, nmax=1_000_000, L=512.0,
+                   interlopers=0.1, a_val=1.0, sizes=None):
+    """Isolated NFW halos for unbinding, one input region per halo.
+
+    Each halo: NFW profile, c ~ U[4, 15], r_vir = 0.6 (n/100)^(1/3) cube
+    units, isotropic Gaussian velocities with sigma^2 = G M / (2 r_vir) and
+    G = a / (6 pi) (the cube-unit constant of the potential), unit masses.
+    A fraction `interlopers` of the members are replaced by particles uniform
+    in a sphere of radius 2 r_vir moving with dispersion 3 sigma.
+    Returns (xp_list, vp_list) of (3, n) float64 Fortran-order arrays.
+    """
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = halo_sizes(rng, n_halos, alpha, nmin, nmax)
+    xs, vs = [], []
+    for n in sizes:
+        n = int(n)
+        c = rng.uniform(4.0, 15.0)
+        rvir = 0.6 * (n / 100.0) ** (1.0 / 3.0)
+        sigma = np.sqrt(a_val / (6.0 * np.pi) * n / (2.0 * rvir))
+        centre = rng.uniform(0.25 * L, 0.75 * L, 3)
+        n_int = int(round(interlopers * n))
+        n_h = n - n_int
+        x = np.empty((3, n))
+        v = np.empty((3, n))
+        x[:, :n_h] = nfw_radii(rng, n_h, c, rvir) * isotropic(rng, n_h)
+        v[:, :n_h] = sigma * rng.standard_normal((3, n_h))
+        r_i = 2.0 * rvir * rng.uniform(0.0, 1.0, n_int) ** (1.0 / 3.0)
+        x[:, n_h:] = r_i * isotropic(rng, n_int)
+        v[:, n_h:] = 3.0 * sigma * rng.standard_normal((3, n_int))
+        perm = rng.permutation(n)
+        xs.append(np.asfortranarray(x[:, perm] + centre[:, None]))
+        vs.append(np.asfortranarray(v[:, perm]))
+    return xs, vs
