@@ -40,65 +40,85 @@
 
 typedef struct { double re, im; } cpx;
 
-static void fft_rec(cpx *x, cpx *tmp, int n, int stride, int sign)
+/* twiddle table exp(-2 pi i k / N), k < N, in double */
+static cpx *fft_table(int N)
 {
-    /* in-place recursive decimation in time on x[0..n) with step `stride` */
+    cpx *t = (cpx *)malloc(sizeof(cpx) * (size_t)N);
+    for (int k = 0; k < N; ++k) {
+        double ang = -2.0 * BPI * (double)k / (double)N;
+        t[k].re = cos(ang);
+        t[k].im = sin(ang);
+    }
+    return t;
+}
+
+/* exp(sign 2 pi i k / N) from the table (sign -1 forward, +1 inverse) */
+static inline cpx tw_at(const cpx *t, int N, long k, int sign)
+{
+    cpx w = t[k % N];
+    if (sign > 0) w.im = -w.im;
+    return w;
+}
+
+static void fft_rec(cpx *x, cpx *tmp, int n, int sign, const cpx *t, int N)
+{
+    /* in-place recursive decimation in time on x[0..n), n | N */
     if (n == 1) return;
     int r = (n % 4 == 0) ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : n));
     int m = n / r;
-    /* gather sub-sequences */
     for (int q = 0; q < r; ++q)
-        for (int k = 0; k < m; ++k) tmp[q * m + k] = x[(size_t)(k * r + q) * stride];
-    for (int q = 0; q < r; ++q) {
-        for (int k = 0; k < m; ++k) x[(size_t)(q * m + k) * stride] = tmp[q * m + k];
-    }
-    for (int q = 0; q < r; ++q) fft_rec(x + (size_t)q * m * stride, tmp, m, stride, sign);
-    /* combine */
+        for (int k = 0; k < m; ++k) tmp[q * m + k] = x[k * r + q];
+    memcpy(x, tmp, sizeof(cpx) * (size_t)n);
+    for (int q = 0; q < r; ++q) fft_rec(x + q * m, tmp, m, sign, t, N);
+    const int step = N / n, rstep = N / r;
     for (int k = 0; k < m; ++k) {
         cpx v[4];
         for (int q = 0; q < r; ++q) {
-            cpx a = x[(size_t)(q * m + k) * stride];
-            double ang = sign * 2.0 * BPI * (double)(q * k) / (double)n;
-            double c = cos(ang), s = sin(ang);
-            v[q].re = a.re * c - a.im * s;
-            v[q].im = a.re * s + a.im * c;
+            cpx a = x[q * m + k];
+            cpx w = tw_at(t, N, (long)q * k * step, sign);
+            v[q].re = a.re * w.re - a.im * w.im;
+            v[q].im = a.re * w.im + a.im * w.re;
         }
         for (int j = 0; j < r; ++j) {
             cpx acc = {0.0, 0.0};
             for (int q = 0; q < r; ++q) {
-                double ang = sign * 2.0 * BPI * (double)(q * j) / (double)r;
-                double c = cos(ang), s = sin(ang);
-                acc.re += v[q].re * c - v[q].im * s;
-                acc.im += v[q].re * s + v[q].im * c;
+                cpx w = tw_at(t, N, (long)q * j * rstep, sign);
+                acc.re += v[q].re * w.re - v[q].im * w.im;
+                acc.im += v[q].re * w.im + v[q].im * w.re;
             }
             tmp[j * m + k] = acc;
         }
     }
-    for (int j = 0; j < r; ++j)
-        for (int k = 0; k < m; ++k) x[(size_t)(j * m + k) * stride] = tmp[j * m + k];
+    memcpy(x, tmp, sizeof(cpx) * (size_t)n);
 }
 
 /* 3-D complex FFT of an n^3 grid (x fastest). sign -1 forward, +1 inverse
- * (unnormalised). */
+ * (unnormalised). Lines are independent: OpenMP over them. */
 static void fft3(cpx *g, int n, int sign)
 {
-    cpx *line = (cpx *)malloc(sizeof(cpx) * n);
-    cpx *tmp = (cpx *)malloc(sizeof(cpx) * n);
+    cpx *t = fft_table(n);
     for (int axis = 0; axis < 3; ++axis) {
-        size_t st = axis == 0 ? 1 : (axis == 1 ? (size_t)n : (size_t)n * n);
-        for (int a = 0; a < n; ++a)
-            for (int b = 0; b < n; ++b) {
+        const size_t st = axis == 0 ? 1 : (axis == 1 ? (size_t)n : (size_t)n * n);
+        #pragma omp parallel
+        {
+            cpx *line = (cpx *)malloc(sizeof(cpx) * n);
+            cpx *tmp = (cpx *)malloc(sizeof(cpx) * n);
+            #pragma omp for schedule(static)
+            for (long ab = 0; ab < (long)n * n; ++ab) {
+                const int a = (int)(ab % n), b = (int)(ab / n);
                 size_t base;
                 if (axis == 0) base = (size_t)n * (a + (size_t)n * b);
                 else if (axis == 1) base = (size_t)a + (size_t)n * n * b;
                 else base = (size_t)a + (size_t)n * b;
                 for (int i = 0; i < n; ++i) line[i] = g[base + st * i];
-                fft_rec(line, tmp, n, 1, sign);
+                fft_rec(line, tmp, n, sign, t, n);
                 for (int i = 0; i < n; ++i) g[base + st * i] = line[i];
             }
+            free(line);
+            free(tmp);
+        }
     }
-    free(line);
-    free(tmp);
+    free(t);
 }
 
 /* ------------------------------------------------------------------ */
@@ -260,6 +280,7 @@ void oracle_green(int32_t n, float *G, float *self125)
 {
     const double apm_soft = 3.5, radius = 0.5 * (double)n;
     const int nh = n / 2 + 1;
+    #pragma omp parallel for schedule(dynamic)
     for (int k = 1; k <= n; ++k)
         for (int j = 1; j <= n; ++j)
             for (int i = 1; i <= nh; ++i) {
@@ -618,14 +639,15 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
             for (size_t g = 0; g < nn; ++g) { grid[g].re = 0.0; grid[g].im = 0.0; }
             /* deposit in float into a float grid (kept as double storage of float values) */
             float *rho = (float *)calloc(nn, sizeof(float));
+            /* stencil nodes nearest..nearest+2 (bound_fortran.f90:2660-2668) */
             for (int i = 0; i < n; ++i) {
                 if (!keep[i]) continue;
                 for (int sz = 0; sz < 3; ++sz) {
-                    int iz = first[3 * i + 2] - 1 + sz;
+                    int iz = first[3 * i + 2] + sz;
                     for (int sy = 0; sy < 3; ++sy) {
-                        int iy = first[3 * i + 1] - 1 + sy;
+                        int iy = first[3 * i + 1] + sy;
                         for (int sx = 0; sx < 3; ++sx) {
-                            int ix = first[3 * i] - 1 + sx;
+                            int ix = first[3 * i] + sx;
                             size_t gi = (size_t)ix + (size_t)ng * (iy + (size_t)ng * iz);
                             rho[gi] = rho[gi] + w[i] * wt[9 * i + sx] * wt[9 * i + 3 + sy] * wt[9 * i + 6 + sz];
                         }
@@ -657,11 +679,11 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
                 float dz0 = (float)fmax(0.0, mkeep - (double)w[i]) / radius_r;
                 float raw = 0.0f;
                 for (int sz = 0; sz < 3; ++sz) {
-                    int iz = first[3 * i + 2] - 1 + sz;
+                    int iz = first[3 * i + 2] + sz;
                     for (int sy = 0; sy < 3; ++sy) {
-                        int iy = first[3 * i + 1] - 1 + sy;
+                        int iy = first[3 * i + 1] + sy;
                         for (int sx = 0; sx < 3; ++sx) {
-                            int ix = first[3 * i] - 1 + sx;
+                            int ix = first[3 * i] + sx;
                             raw = raw + rho[(size_t)ix + (size_t)ng * (iy + (size_t)ng * iz)] *
                                             wt[9 * i + sx] * wt[9 * i + 3 + sy] * wt[9 * i + 6 + sz];
                         }
@@ -827,15 +849,17 @@ int32_t oracle_fof_bound(const double *xp, const double *vp, const double *mp,
             }
             mem_pid[hc] = mp2;
         }
-    green_t greens[4];
-    memset(greens, 0, sizeof(greens));
+    /* Green functions depend on n only: computed once per process and size */
+    static green_t greens_cache[4];
+    green_t *greens = greens_cache;
     int32_t valid = 0;
     int64_t phi_off = 0;
     if (ierr == 0) {
         for (int32_t h = 0; h < nc; ++h) {
             int gid = task[h].gid;
-            if (gid >= 1 && !greens[gid - 1].G) {
+            if (gid >= 1 && (!greens[gid - 1].G || greens[gid - 1].n != ctx.bn[gid - 1])) {
                 int n = ctx.bn[gid - 1];
+                free(greens[gid - 1].G);
                 greens[gid - 1].n = n;
                 greens[gid - 1].G = (float *)malloc(sizeof(float) * (size_t)(n / 2 + 1) * n * n);
                 oracle_green(n, greens[gid - 1].G, greens[gid - 1].self);
@@ -914,7 +938,6 @@ int32_t oracle_fof_bound(const double *xp, const double *vp, const double *mp,
         for (int32_t i = 0; i < total; ++i) { in_halo[i] = 0; halo_id[i] = 0; }
         for (int32_t i = 0; i < 3 * nhalo; ++i) ang_mom[i] = 0.0;
     }
-    for (int g = 0; g < 4; ++g) free(greens[g].G);
     for (int32_t h = 0; h < nc; ++h) free(mem_pid[h]);
     for (int32_t h = 0; h < nhalo; ++h) { free(rc[h].cand_np); free(rc[h].member_ptr); free(rc[h].members); }
     free(rc); free(task); free(mem_pid); free(cand_reg); free(cand_n); free(cand_anchor);

@@ -441,12 +441,14 @@ __global__ void k_deposit(const float4* __restrict__ qw, const uint32_t* __restr
     const int4 f = first[t];
     const float* w9 = wt + 9 * (size_t)t;
     unsigned long long* g = acc + (size_t)H.slot * n * n * n;
+    // stencil nodes nearest..nearest+2 per axis (bound_fortran.f90:2660-2668);
+    // k_tsc_setup's guard t in [0, n/2 - 1) keeps them inside [0, n/2 + 1]
     for (int sz = 0; sz < 3; ++sz) {
-        const int iz = f.z - 1 + sz;
+        const int iz = f.z + sz;
         for (int sy = 0; sy < 3; ++sy) {
-            const int iy = f.y - 1 + sy;
+            const int iy = f.y + sy;
             for (int sx = 0; sx < 3; ++sx) {
-                const int ix = f.x - 1 + sx;
+                const int ix = f.x + sx;
                 const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
                 const long long fx = llrint((double)v * fscale[h]);
                 atomicAdd(&g[(size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz)], (unsigned long long)fx);
@@ -503,11 +505,11 @@ __global__ void k_interp(const float4* __restrict__ qw, const uint32_t* __restri
     const float* w9 = wt + 9 * (size_t)t;
     float raw = 0.0f;
     for (int sz = 0; sz < 3; ++sz) {
-        const int iz = f.z - 1 + sz;
+        const int iz = f.z + sz;
         for (int sy = 0; sy < 3; ++sy) {
-            const int iy = f.y - 1 + sy;
+            const int iy = f.y + sy;
             for (int sx = 0; sx < 3; ++sx) {
-                const int ix = f.x - 1 + sx;
+                const int ix = f.x + sx;
                 const float rho = __fdiv_rn(gg[(size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz)].x, norm);
                 raw = __fadd_rn(raw, __fmul_rn(__fmul_rn(__fmul_rn(rho, w9[sx]), w9[3 + sy]), w9[6 + sz]));
             }
@@ -1156,6 +1158,14 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         float* selfv = C.buf("selfv").as<float>(M);
         k_tsc_setup<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, selfs, M, first, wt, selfv, err);
         PYFOF_LAUNCH_CHECK();
+        {
+            // a member outside the source domain leaves its stencil unset:
+            // stop here (ierr 1, bound_fortran.f90:2571-2575) before any deposit
+            int32_t herr = 0;
+            PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipStreamSynchronize(s));
+            if (herr) return herr;
+        }
         // grids per class
         unsigned long long* acc[4] = {nullptr, nullptr, nullptr, nullptr};
         float2* grid[4] = {nullptr, nullptr, nullptr, nullptr};
