@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark: plain FoF (binning + linking + relabel) on one MI355X.
+"""Benchmark: FoF + unbinding on one MI355X.
 
-Workload (default): 134,217,728 particles, float64 (3, N), L_Box = 512,
-50 % in 250,000 NFW clumps (c = 8, Pareto(0.9) counts in [20, 2e6],
-r_vir = 0.6 (n/100)^(1/3)), 50 % uniform; linking_length 0.2,
-np_halo_min 10, bound_check False. Positions are generated on the host
-(seeded), copied to HBM once, and every timed step is one full fof() call
-through the C-ABI on device-resident input and outputs.
+Headline (``value``): plain FoF (binning + linking + relabel), 134,217,728
+particles, float64 (3, N), L_Box = 512, 50 % in 250,000 NFW clumps (c = 8,
+Pareto(0.9) counts in [20, 2e6], r_vir = 0.6 (n/100)^(1/3)), 50 % uniform;
+linking_length 0.2, np_halo_min 10, bound_check False. Positions are
+generated on the host (seeded), copied to HBM once, and every timed step is
+one full fof() call through the C-ABI on device-resident input and outputs.
 
-Prints one JSON line (see the keys below). The CPU baseline is the oracle
-(the C restatement of the reference's FoF) timed on a bounded sample: the
-same generator at 2,097,152 particles in a 128^3 box (same density), with
-the reference's thread rule (2 linking threads for one region).
+``unbinding`` (after the headline steps): 20,000 isolated NFW halos, sizes
+from dN/dn ~ n^-1.9 on [20, 1e6] (seed 3: 5,803,808 particles), 10 %
+interlopers, cube units, softening 0.1, bound_check True; every step is one
+fof(bound_check=True) call over the 20,000 regions on device-resident
+input. Reported as bound halos/s and particles/s.
+
+Prints one JSON line (see the keys below). The CPU baselines are the oracle
+(the C restatement of the reference) timed on bounded samples: FoF on the
+same generator at 2,097,152 particles in a 128^3 box (same density) with the
+reference's thread rule (2 linking threads for one region); unbinding on the
+first halos of the same halo set.
 """
 import argparse
 import json
@@ -26,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FoF Mparticles/s; bound halos/s batched vs plain at stated err; p99 latency"
 HBM_PEAK_GBS = 8000.0
+PHASES = ["bbox", "binning", "linking", "relabel", "unbinding", "catalog"]
 
 
 def parse():
@@ -37,6 +45,12 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-n", type=int, default=2_097_152, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bound", action="store_true", help="skip the unbinding leg")
+    ap.add_argument("--bound-halos", type=int, default=20_000)
+    ap.add_argument("--bound-steps", type=int, default=3)
+    ap.add_argument("--bound-cpu-halos", type=int, default=2_000)
+    ap.add_argument("--only", choices=["all", "fof", "bound"], default="all",
+                    help="run one leg only (profiling); the headline needs 'all' or 'fof'")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -68,6 +82,89 @@ def cpu_baseline(args):
     }
 
 
+def bound_workload(n_halos, seed=3):
+    from py_fof_amd.synth import isolated_halos
+    xs, vs = isolated_halos(n_halos=n_halos, seed=seed)
+    np_halo = np.array([x.shape[1] for x in xs], np.int32)
+    xp = np.asfortranarray(np.hstack(xs))
+    vp = np.asfortranarray(np.hstack(vs))
+    # s_15 covers the widest region (interlopers reach 2 r_vir), as the
+    # caller would set it through box / ng_global
+    span = max(float((x.max(1) - x.min(1)).max()) for x in xs)
+    s15 = max(17.152, float(np.ceil(1.02 * span)))
+    return xp, vp, np_halo, s15
+
+
+def bench_bound(args, torch, _native, ctx):
+    L, b, app = 512.0, 0.2, 0.1
+    log(args, f"unbinding: generating {args.bound_halos} isolated halos")
+    xp, vp, np_halo, s15 = bound_workload(args.bound_halos)
+    n = int(np_halo.sum())
+    dev = torch.device("cuda:0")
+    x_d = torch.from_numpy(xp.T.copy()).to(dev)
+    v_d = torch.from_numpy(vp.T.copy()).to(dev)
+    in_h = torch.empty(n, dtype=torch.int32, device=dev)
+    hid = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        err = ctx.fof(x_d, v_d, None, np_halo, L, b, 10, 1, 0, app, 1.0, 0.0, s15, 0, n,
+                      256, 512.0, in_h, hid, None, None, _native.MEM_DEVICE)
+        if err:
+            raise RuntimeError(f"pyfof ierr={err}: {_native.last_error()}")
+
+    step()
+    log(args, f"unbinding warmup done: phases ms {[round(v, 2) for v in ctx.timings()]}")
+    ref = in_h.clone()
+    torch.cuda.synchronize()
+    phases = np.zeros(6)
+    t0 = time.perf_counter()
+    for _ in range(args.bound_steps):
+        step()
+        phases += np.array(ctx.timings()[:6])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.bound_steps
+    phases /= args.bound_steps
+    st = ctx.stats()
+    out = {
+        "workload": f"{args.bound_halos} isolated NFW halos, n ~ n^-1.9 on [20, 1e6], seed 3, "
+                    f"{n} particles, 10% interlopers, cube units, b {b}, softening {app}, "
+                    f"s_15 {s15:g}, bound_check True",
+        "bound_halos_per_s": round(st["nvalid"] / dt, 1),
+        "candidates_per_s": round((st["n_ok"] + st["n_too_few"] + st["n_max_iter"]) / dt, 1),
+        "Mparticles_per_s": round(n / dt / 1e6, 3),
+        "ms_per_call": round(dt * 1e3, 3),
+        "phases_ms": {k: round(float(v), 3) for k, v in zip(
+            PHASES, phases)},
+        "candidates": st["n_ok"] + st["n_too_few"] + st["n_max_iter"],
+        "bound_halos": st["nvalid"], "status_ok": st["n_ok"], "status_too_few": st["n_too_few"],
+        "status_max_iter": st["n_max_iter"], "direct_sum_halos": st["n_direct"],
+        "pm_halos": {"48": st["n_pm48"], "64": st["n_pm64"], "128": st["n_pm128"],
+                     "256": st["n_pm256"]},
+        "pm_passes": st["passes"],
+        "form": "direct sum: FP64 per-halo workgroup; PM: f32 batched per grid class",
+        "deterministic": bool(torch.equal(ref, in_h)),
+    }
+    if not args.no_cpu and args.bound_cpu_halos > 0:
+        from oracle import oracle_fof_bound
+        k = min(args.bound_cpu_halos, len(np_halo))
+        nk = int(np_halo[:k].sum())
+        log(args, f"unbinding cpu baseline on the first {k} halos ({nk} particles)")
+        t0 = time.perf_counter()
+        r = oracle_fof_bound(xp[:, :nk], np_halo[:k], L, b, 10, vp[:, :nk], app=app, s15=s15)
+        dtc = time.perf_counter() - t0
+        nvalid = int(len(np.unique(r["halo_id"][r["halo_id"] > 0])))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {
+            "value": round(nvalid / dtc, 2), "unit": "bound halos/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FoF + unbinding on the first {k} halos of the same set "
+                      f"({nk} particles, {nvalid} bound halos); FoF linking and direct sums "
+                      f"on 1 thread, FFTs and Green functions on {threads}; {dtc:.2f} s",
+        }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -78,6 +175,12 @@ def main():
 
     from py_fof_amd import _native
     from py_fof_amd.synth import clumpy_nfw
+
+    if args.only == "bound":
+        ctx = _native.Context(0)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1,
+                          "unbinding": bench_bound(args, torch, _native, ctx)}), flush=True)
+        return
 
     n = args.n
     L = box_for(n)
@@ -152,7 +255,7 @@ def main():
             "kernel": "binning + linking (k_bbox..k_link), algorithmic bytes N*64 + cells*12",
         },
         "phases_ms": {k: round(float(v), 3) for k, v in zip(
-            ["bbox", "binning", "linking", "relabel", "catalog", "unbinding"], phases)},
+            PHASES, phases)},
         "groups": ngroups,
         "cells": st["ncell"],
         "dense_cells": st["ndense"],
@@ -162,6 +265,10 @@ def main():
     if not args.no_cpu:
         log(args, "cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args)
+    if not args.no_bound and args.only == "all":
+        del x_d, in_h, hid
+        torch.cuda.empty_cache()
+        out["unbinding"] = bench_bound(args, torch, _native, ctx)
     print(json.dumps(out), flush=True)
 
 

@@ -78,7 +78,7 @@ int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, i
 
 /* Device time in milliseconds of the phases of the last call on ctx
  * (0: bbox+geometry, 1: binning, 2: linking, 3: relabel/outputs,
- * 4: catalog+angular momentum, 5: unbinding), measured with HIP events on
+ * 4: unbinding, 5: catalog+angular momentum), measured with HIP events on
  * the context's stream. Returns the number of phases written. */
 int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
 

@@ -628,14 +628,18 @@ __global__ void __launch_bounds__(kB) k_halo_mass(const float4* __restrict__ qw,
 
 // Full C2 residual over kept members (bound_fortran.f90:3403-3483), as a
 // gather: member i sums over the kept members j of its 27 neighbour cells.
+// Threads take members in cell order (clist), so the lanes of a wavefront
+// share neighbour cells: equal trip counts and broadcast loads.
 __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
                           const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
                           const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
                           const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cstart,
                           const uint32_t* __restrict__ clist, uint32_t nmem, double a, double app,
                           float* __restrict__ phi_c2) {
-    const uint32_t t = blockIdx.x * kB + threadIdx.x;
-    if (t >= nmem || !keep[t]) return;
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    if (ps >= nmem) return;
+    const uint32_t t = clist[ps];
+    if (!keep[t]) return;
     const uint32_t h = mhalo[t];
     const int32_t k = pm_of_halo[h];
     if (k < 0 || !active[h]) return;
@@ -673,6 +677,88 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
         }
     }
     phi_c2[t] = acc;
+}
+
+// Incremental C2 (bound_fortran.f90:3490-3552): members deleted since the
+// last refresh stop contributing; their residual is taken off every kept
+// neighbour. Deleted members are listed per C2 cell and each kept member
+// gathers from its 27 neighbour cells (fixed order, no atomics).
+__global__ void k_c2_del_flags(const uint32_t* __restrict__ mhalo, const int32_t* __restrict__ pm_of_halo,
+                               const uint8_t* __restrict__ active, const uint8_t* __restrict__ keep,
+                               const uint8_t* __restrict__ keep_c2, uint32_t nmem,
+                               uint32_t* __restrict__ flag) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    flag[t] = (pm_of_halo[h] >= 0 && active[h] && keep_c2[t] && !keep[t]) ? 1u : 0u;
+}
+
+__global__ void k_c2_del_compact(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ idx,
+                                 const uint32_t* __restrict__ cell_of, uint32_t nmem,
+                                 uint32_t* __restrict__ dkeys, uint32_t* __restrict__ dvals) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !flag[t]) return;
+    dkeys[idx[t]] = cell_of[t];
+    dvals[idx[t]] = t;
+}
+
+__global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t* __restrict__ ends) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = skeys[i];
+    if (i + 1 == n || skeys[i + 1] != k) ends[k] = i + 1;
+}
+
+__global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                            const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                            const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
+                            const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ dstart,
+                            const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ clist,
+                            uint32_t nmem, double a, double app, float* __restrict__ phi_c2) {
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    if (ps >= nmem) return;
+    const uint32_t t = clist[ps];
+    if (!keep[t]) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = pm_of_halo[h];
+    if (k < 0 || !active[h]) return;
+    const PmHalo H = pm[k];
+    const float sp = H.origin.w;
+    const double ht = (double)(float)app / (double)sp, hs = 3.5;
+    const double r2cut = 12.25 * (double)sp * (double)sp;
+    const float pref = (float)(a / (6.0 * kPi * (double)sp));
+    const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
+    const int cx = (int)(lc % (uint32_t)H.nc[0]);
+    const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
+    const int cz = (int)(lc / (uint32_t)(H.nc[0] * H.nc[1]));
+    const float4 qi = qw[t];
+    float acc = phi_c2[t];
+    for (int z = cz - 1; z <= cz + 1; ++z) {
+        if (z < 0 || z >= H.nc[2]) continue;
+        for (int y = cy - 1; y <= cy + 1; ++y) {
+            if (y < 0 || y >= H.nc[1]) continue;
+            for (int x = cx - 1; x <= cx + 1; ++x) {
+                if (x < 0 || x >= H.nc[0]) continue;
+                const uint32_t c = (uint32_t)H.c2off + (uint32_t)x + (uint32_t)H.nc[0] * ((uint32_t)y + (uint32_t)H.nc[1] * (uint32_t)z);
+                for (uint32_t p = dstart[c]; p < dstart[c + 1]; ++p) {
+                    const float4 qj = qw[dlist[p]];
+                    const double dx = (double)(qi.x - qj.x), dy = (double)(qi.y - qj.y), dz = (double)(qi.z - qj.z);
+                    const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                    if (r2 < r2cut) {
+                        const double rs = sqrt(r2) / (double)sp;
+                        const float dphi = __fmul_rn(-pref, (float)c2_delta_depth(rs, ht, hs));
+                        acc = __fsub_rn(acc, __fmul_rn(qj.w, dphi));
+                    }
+                }
+            }
+        }
+    }
+    phi_c2[t] = acc;
+}
+
+__global__ void k_copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // phi = C1 + C2 (bound_fortran.f90:2739-2754); zero for a single member
@@ -1140,7 +1226,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         uint32_t* cv2 = C.buf("c2v2").as<uint32_t>(M);
         k_c2_keys<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, M, (uint32_t)c2cells, ck, cv);
         PYFOF_LAUNCH_CHECK();
-        void* tmp = C.buf("c2tmp").get(std::max(radix_temp_bytes(M), scan_temp_bytes(c2cells + 1)) + 256);
+        void* tmp = C.buf("c2tmp").get(std::max({radix_temp_bytes(M), scan_temp_bytes(c2cells + 1), scan_temp_bytes(M)}) + 256);
         const bool alt = radix_sort_pairs(ck, cv, ck2, cv2, M, bits_for(c2cells), tmp, s);
         const uint32_t* skeys = alt ? ck2 : ck;
         const uint32_t* clist = alt ? cv2 : cv;
@@ -1198,6 +1284,17 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         k_halo_mass<<<npm, kB, 0, s>>>(qw, A.hoff, d_pmhalo, mkeep, fscale);
         PYFOF_LAUNCH_CHECK();
 
+        // incremental C2 state: keep flags at the last C2 evaluation, the
+        // deleted members since then listed per cell
+        uint8_t* keep_c2 = C.buf("keepc2").as<uint8_t>(M);
+        uint32_t* dflag = C.buf("c2dflag").as<uint32_t>(M);
+        uint32_t* didx = C.buf("c2didx").as<uint32_t>(M);
+        uint32_t* dtot = C.buf("c2dtot").as<uint32_t>(1);
+        uint32_t* dk = C.buf("c2dk").as<uint32_t>(M);
+        uint32_t* dv = C.buf("c2dv").as<uint32_t>(M);
+        uint32_t* dk2 = C.buf("c2dk2").as<uint32_t>(M);
+        uint32_t* dv2 = C.buf("c2dv2").as<uint32_t>(M);
+        uint32_t* dstart = C.buf("c2dstart").as<uint32_t>(c2cells + 1);
         std::vector<int32_t> act_list = pm_halo;
         int32_t* d_alist = C.buf("alist").as<int32_t>(npm);
         int32_t* d_slots = C.buf("slots").as<int32_t>(npm);
@@ -1229,8 +1326,34 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                                                   mkeep, grid[g], g, M, A.a, phi_c1);
                 PYFOF_LAUNCH_CHECK();
             }
-            k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
-                                               clist, M, A.a, A.app, phi_c2);
+            if (pass == 1) {
+                k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
+                                                   clist, M, A.a, A.app, phi_c2);
+                PYFOF_LAUNCH_CHECK();
+            } else {
+                k_c2_del_flags<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, active, keep, keep_c2, M, dflag);
+                PYFOF_LAUNCH_CHECK();
+                exclusive_scan_u32(dflag, didx, M, dtot, tmp, s);
+                uint32_t nd = 0;
+                PYFOF_HIP(hipMemcpyAsync(&nd, dtot, 4, hipMemcpyDeviceToHost, s));
+                PYFOF_HIP(hipStreamSynchronize(s));
+                if (nd > 0) {
+                    k_c2_del_compact<<<blocks(M), kB, 0, s>>>(dflag, didx, cell_of, M, dk, dv);
+                    PYFOF_LAUNCH_CHECK();
+                    const bool dalt = radix_sort_pairs(dk, dv, dk2, dv2, nd, bits_for(c2cells), tmp, s);
+                    const uint32_t* dks = dalt ? dk2 : dk;
+                    const uint32_t* dvs = dalt ? dv2 : dv;
+                    k_fill32<<<blocks(c2cells + 1), kB, 0, s>>>(dstart, c2cells + 1, 0u);
+                    PYFOF_LAUNCH_CHECK();
+                    k_run_ends<<<blocks(nd), kB, 0, s>>>(dks, nd, dstart + 1);
+                    PYFOF_LAUNCH_CHECK();
+                    inclusive_max_scan_u32(dstart + 1, dstart + 1, c2cells, tmp, s);
+                    k_c2_update<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, dstart,
+                                                         dvs, clist, M, A.a, A.app, phi_c2);
+                    PYFOF_LAUNCH_CHECK();
+                }
+            }
+            k_copy_u8<<<blocks(M), kB, 0, s>>>(keep, keep_c2, M);
             PYFOF_LAUNCH_CHECK();
             k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, keep, active, npb, phi_c1, phi_c2, M,
                                                    phi, err);

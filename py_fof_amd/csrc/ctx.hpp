@@ -62,7 +62,7 @@ struct GreenCache {
     bool ready = false;
 };
 
-enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvCat, kEvBound, kNumEv };
+enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvBound, kEvCat, kNumEv };
 
 struct Ctx {
     int device = 0;

@@ -1336,8 +1336,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                      hipMemcpyDeviceToDevice, s));
             nh_out = br.nvalid;
             C.cat_nh = nh_out;
-            C.mark(kEvBound);
         }
+        C.mark(kEvBound);
         const bool need_groups = C.want_catalog || (A.ang_mom && vp);
         if (nh_out > 0 && need_groups) {
             Cat cat = build_catalog(nh_out);
