@@ -44,36 +44,38 @@ def _load_from(path):
     if not os.path.exists(path):
         raise NativeLibraryError(
             f"{path} not found: build it with `python -m py_fof_amd.build`")
-    if True:
+    try:
         lib = ctypes.CDLL(path)
-        d, i32, i64, vp = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
-        lib.pyfof_fof.restype = i32
-        lib.pyfof_fof.argtypes = [vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d, i32, i32,
-                                  i64, d, vp, vp, vp, i32]
-        lib.pyfof_fof_ctx.restype = i32
-        lib.pyfof_fof_ctx.argtypes = [vp, vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d,
-                                      i32, i32, i64, d, vp, vp, vp, vp, i32]
-        lib.pyfof_ctx_create.restype = vp
-        lib.pyfof_ctx_create.argtypes = [i32]
-        lib.pyfof_ctx_destroy.restype = None
-        lib.pyfof_ctx_destroy.argtypes = [vp]
-        lib.pyfof_catalog_size.restype = i32
-        lib.pyfof_catalog_size.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i64)]
-        lib.pyfof_catalog_copy.restype = i32
-        lib.pyfof_catalog_copy.argtypes = [vp, vp, vp, i32]
-        lib.pyfof_ctx_timings.restype = i32
-        lib.pyfof_ctx_timings.argtypes = [vp, ctypes.POINTER(d), i32]
-        lib.pyfof_ctx_set_option.restype = i32
-        lib.pyfof_ctx_set_option.argtypes = [vp, i32, i64]
-        lib.pyfof_ctx_stats.restype = i32
-        lib.pyfof_ctx_stats.argtypes = [vp, ctypes.POINTER(i64), i32]
-        lib.pyfof_ctx_stream.restype = vp
-        lib.pyfof_ctx_stream.argtypes = [vp]
-        lib.pyfof_last_error.restype = ctypes.c_char_p
-        lib.pyfof_last_error.argtypes = []
-        lib.pyfof_version.restype = ctypes.c_char_p
-        lib.pyfof_version.argtypes = []
-        return lib
+    except OSError as e:
+        raise NativeLibraryError(f"cannot load {path}: {e}") from e
+    d, i32, i64, vp = ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    lib.pyfof_fof.restype = i32
+    lib.pyfof_fof.argtypes = [vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d, i32, i32,
+                              i64, d, vp, vp, vp, i32]
+    lib.pyfof_fof_ctx.restype = i32
+    lib.pyfof_fof_ctx.argtypes = [vp, vp, vp, vp, vp, i32, d, d, i32, i32, i32, d, d, d, d,
+                                  i32, i32, i64, d, vp, vp, vp, vp, i32]
+    lib.pyfof_ctx_create.restype = vp
+    lib.pyfof_ctx_create.argtypes = [i32]
+    lib.pyfof_ctx_destroy.restype = None
+    lib.pyfof_ctx_destroy.argtypes = [vp]
+    lib.pyfof_catalog_size.restype = i32
+    lib.pyfof_catalog_size.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i64)]
+    lib.pyfof_catalog_copy.restype = i32
+    lib.pyfof_catalog_copy.argtypes = [vp, vp, vp, i32]
+    lib.pyfof_ctx_timings.restype = i32
+    lib.pyfof_ctx_timings.argtypes = [vp, ctypes.POINTER(d), i32]
+    lib.pyfof_ctx_set_option.restype = i32
+    lib.pyfof_ctx_set_option.argtypes = [vp, i32, i64]
+    lib.pyfof_ctx_stats.restype = i32
+    lib.pyfof_ctx_stats.argtypes = [vp, ctypes.POINTER(i64), i32]
+    lib.pyfof_ctx_stream.restype = vp
+    lib.pyfof_ctx_stream.argtypes = [vp]
+    lib.pyfof_last_error.restype = ctypes.c_char_p
+    lib.pyfof_last_error.argtypes = []
+    lib.pyfof_version.restype = ctypes.c_char_p
+    lib.pyfof_version.argtypes = []
+    return lib
 
 
 def last_error():
