@@ -93,6 +93,59 @@ int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n);
 /* HIP stream (hipStream_t) the context launches on. */
 void* pyfof_ctx_stream(pyfof_ctx* ctx);
 
+/*
+ * Request server (continuous batching on one GPU). The reference has no
+ * counterpart: its single entry point (fof_fortran.f90:10-58) keeps module
+ * state and must be called serially (bound_fortran.f90:1524, 1564-1568).
+ * A server runs `slots` workers, each with its own context (stream and
+ * HBM-resident workspace). Submitted jobs queue in arrival order; the head
+ * starts when a worker is free and its estimated HBM footprint
+ * (pyfof_job_footprint) fits in hbm_budget_bytes beside the running jobs.
+ * Jobs on different workers overlap on the device. Each job has its own ierr
+ * and message. The arrays a job points to (xp, vp, mp, outputs) must stay
+ * valid until pyfof_poll/pyfof_wait report it done; np_halo is copied.
+ */
+typedef struct pyfof_server pyfof_server;
+
+/* One job: the arguments of pyfof_fof, by name. */
+typedef struct pyfof_job_desc {
+    const double* xp;
+    const double* vp;
+    const double* mp;
+    const int32_t* np_halo;
+    int32_t* in_halo;
+    int32_t* halo_id;
+    double* ang_mom;
+    double L_Box, linking_length, softening, a_val, b_val, s_15_val, ng_global_val;
+    int64_t ng_val;
+    int32_t nhalo, np_halo_min, bound_check, verbose, n_threads, total_np, mem_kind, reserved;
+} pyfof_job_desc;
+
+/* slots in [1, 16]; NULL on failure (pyfof_server_last_error). Destroying a
+ * server waits for every queued and running job. */
+pyfof_server* pyfof_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t slots);
+void pyfof_server_destroy(pyfof_server* server);
+
+/* Estimated device bytes of a job, as used for admission. */
+int64_t pyfof_job_footprint(const pyfof_job_desc* job);
+
+/* Queues a job; returns its id (> 0), or -1 (message in
+ * pyfof_server_last_error) when the job is malformed or its footprint
+ * exceeds the whole budget. */
+int64_t pyfof_submit(pyfof_server* server, const pyfof_job_desc* job);
+
+/* pyfof_poll: 1 done (ierr, message and times written, the id released),
+ * 0 still queued or running, -1 unknown id. pyfof_wait blocks until done.
+ * times (optional, 2 values): ms queued before start, ms running. */
+int32_t pyfof_poll(pyfof_server* server, int64_t id, int32_t* ierr, char* msg, int32_t msg_cap,
+                   double* times);
+int32_t pyfof_wait(pyfof_server* server, int64_t id, int32_t* ierr, char* msg, int32_t msg_cap,
+                   double* times);
+
+/* [0] queued jobs, [1] running jobs, [2] admitted footprint, [3] budget. */
+int32_t pyfof_server_status(pyfof_server* server, int64_t* out, int32_t n);
+const char* pyfof_server_last_error(void);
+
 /* Message of the last error on this thread ("" if none). */
 const char* pyfof_last_error(void);
 

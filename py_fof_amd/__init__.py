@@ -28,17 +28,20 @@ def _error_for(ierr):
     return RuntimeError(f"FOF Fortran computation failed with error code ierr={ierr}")
 
 
-def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
-              np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
-              n_threads=None, mp_list=None, softening=None, z=0.0,
-              box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
-              units='cube', include_hubble_flow=False,
-              ng=None, ng_global=None, _ctx=None, _want_catalog=False, _want_roots=False):
-    """Validation, unit conversion and the native call (py_fof/__init__.py:168-351).
+class _Prepared:
+    """Native arguments of one call plus what the result needs afterwards."""
+    __slots__ = ("empty", "xp", "vp", "mp", "np_halo", "nhalo", "total_np", "L", "link",
+                 "np_min", "bound_check", "verbose", "softening", "a", "b", "s15", "n_threads",
+                 "ng", "ng_global", "need_am", "ang_mom_scale")
 
-    Returns (in_halo bool[N], halo_id int32[N], ang_mom (3, nhalo) f64) and,
-    when asked, the device catalog and the partition roots.
-    """
+
+def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
+             np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
+             n_threads=None, mp_list=None, softening=None, z=0.0,
+             box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
+             units='cube', include_hubble_flow=False, ng=None, ng_global=None):
+    """Validation and unit conversion of py_fof/__init__.py:168-326, in the
+    same order and with the same messages. Returns a _Prepared."""
     if units not in ('cube', 'physical'):
         raise ValueError(f"units must be 'cube' or 'physical', got '{units}'")
 
@@ -64,16 +67,12 @@ def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
     np_halo = np.array([xp.shape[1] for xp in xp_list_norm], dtype=np.int32)
     total_np = int(np.sum(np_halo))
 
-    extras = {}
-    if total_np == 0:
-        in_halo = np.zeros(0, dtype=bool)
-        halo_id = np.zeros(0, dtype=np.int32)
-        ang_mom = np.zeros((3, nhalo), dtype=np.float64)
-        if _want_catalog:
-            extras["catalog"] = (np.zeros(1, np.int64), np.zeros(0, np.int32))
-        if _want_roots:
-            extras["roots"] = np.zeros(0, np.int32)
-        return (in_halo, halo_id, ang_mom, extras) if (_want_catalog or _want_roots) else (in_halo, halo_id, ang_mom)
+    P = _Prepared()
+    P.nhalo = nhalo
+    P.total_np = total_np
+    P.empty = total_np == 0
+    if P.empty:
+        return P
 
     xp_halo = np.asfortranarray(np.hstack(xp_list_norm), dtype=np.float64)
 
@@ -184,29 +183,70 @@ def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
 
         ang_mom_scale = m_grid * dx / v_scale
 
-    ng_val = int(ng) if ng is not None else 256
-    ng_global_val = float(ng_global) if ng_global is not None else 512.0
-
-    in_halo_out = np.zeros(total_np, dtype=np.int32)
-    halo_id_out = np.zeros(total_np, dtype=np.int32)
-    ang_mom_out = np.zeros((3, nhalo), dtype=np.float64, order='F')
-    roots = np.zeros(total_np, dtype=np.int32) if _want_roots else None
+    P.xp, P.vp, P.mp, P.np_halo = xp_code, vp_code, mp_code, np_halo
+    P.L, P.link, P.np_min = L_Box_code, link_code, int(np_halo_min)
+    P.bound_check, P.verbose = int(bool(bound_check)), int(verbose)
+    P.softening, P.a, P.b, P.s15 = softening_grid, a_val, b_val, s_15_val
+    P.n_threads = n_threads_val
+    P.ng = int(ng) if ng is not None else 256
+    P.ng_global = float(ng_global) if ng_global is not None else 512.0
     # the moment is only observable with return_ang_mom; with zero velocities it is 0
-    need_am = bool(return_ang_mom) and vp_code is not None
+    P.need_am = bool(return_ang_mom) and vp_code is not None
+    P.ang_mom_scale = ang_mom_scale
+    return P
+
+
+def _outputs(P):
+    return (np.zeros(P.total_np, dtype=np.int32), np.zeros(P.total_np, dtype=np.int32),
+            np.zeros((3, P.nhalo), dtype=np.float64, order='F'))
+
+
+def _finish(P, ierr, in_halo_out, halo_id_out, ang_mom_out, message=""):
+    """ierr -> exception (py_fof/__init__.py:338-348), then the scaled outputs."""
+    if ierr != 0:
+        if P.verbose:
+            print(f"[py_fof_amd] ierr={ierr}: {message}")
+        raise _error_for(ierr)
+    return in_halo_out.astype(bool), halo_id_out, ang_mom_out * P.ang_mom_scale
+
+
+def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
+              np_halo_min=10, bound_check=False, verbose=0, return_ang_mom=False,
+              n_threads=None, mp_list=None, softening=None, z=0.0,
+              box=300.0, h0=0.67, omega_m=0.3, omega_l=0.7, n_sim=None,
+              units='cube', include_hubble_flow=False,
+              ng=None, ng_global=None, _ctx=None, _want_catalog=False, _want_roots=False):
+    """py_fof/__init__.py:168-351: validation, units, the native call, errors.
+
+    Returns (in_halo bool[N], halo_id int32[N], ang_mom (3, nhalo) f64) and,
+    when asked, the device catalog and the partition roots.
+    """
+    P = _prepare(xp_list, vp_list=vp_list, L_Box=L_Box, linking_length=linking_length,
+                 np_halo_min=np_halo_min, bound_check=bound_check, verbose=verbose,
+                 return_ang_mom=return_ang_mom, n_threads=n_threads, mp_list=mp_list,
+                 softening=softening, z=z, box=box, h0=h0, omega_m=omega_m, omega_l=omega_l,
+                 n_sim=n_sim, units=units, include_hubble_flow=include_hubble_flow, ng=ng,
+                 ng_global=ng_global)
+    extras = {}
+    if P.empty:
+        in_halo = np.zeros(0, dtype=bool)
+        halo_id = np.zeros(0, dtype=np.int32)
+        ang_mom = np.zeros((3, P.nhalo), dtype=np.float64)
+        if _want_catalog:
+            extras["catalog"] = (np.zeros(1, np.int64), np.zeros(0, np.int32))
+        if _want_roots:
+            extras["roots"] = np.zeros(0, np.int32)
+        return (in_halo, halo_id, ang_mom, extras) if (_want_catalog or _want_roots) else (in_halo, halo_id, ang_mom)
+
+    in_halo_out, halo_id_out, ang_mom_out = _outputs(P)
+    roots = np.zeros(P.total_np, dtype=np.int32) if _want_roots else None
     ctx = _ctx if _ctx is not None else _native.default_context()
     ctx.set_option(_native.OPT_CATALOG, 1 if _want_catalog else 0)
-    ierr = ctx.fof(xp_code, vp_code, mp_code, np_halo, L_Box_code, link_code, int(np_halo_min),
-                   int(bool(bound_check)), int(verbose), softening_grid, a_val, b_val, s_15_val,
-                   n_threads_val, total_np, ng_val, ng_global_val, in_halo_out, halo_id_out,
-                   ang_mom_out if need_am else None, roots, MEM_HOST)
-
-    if ierr != 0:
-        if verbose:
-            print(f"[py_fof_amd] ierr={ierr}: {_native.last_error()}")
-        raise _error_for(ierr)
-
-    ang_mom_scaled = ang_mom_out * ang_mom_scale
-    out = (in_halo_out.astype(bool), halo_id_out, ang_mom_scaled)
+    ierr = ctx.fof(P.xp, P.vp, P.mp, P.np_halo, P.L, P.link, P.np_min, P.bound_check, P.verbose,
+                   P.softening, P.a, P.b, P.s15, P.n_threads, P.total_np, P.ng, P.ng_global,
+                   in_halo_out, halo_id_out, ang_mom_out if P.need_am else None, roots, MEM_HOST)
+    out = _finish(P, ierr, in_halo_out, halo_id_out, ang_mom_out,
+                  _native.last_error() if ierr else "")
     if _want_catalog or _want_roots:
         if _want_catalog:
             extras["catalog"] = ctx.catalog()

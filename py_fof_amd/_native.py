@@ -21,6 +21,9 @@ EXPORTS = [
     "pyfof_catalog_size", "pyfof_catalog_copy", "pyfof_ctx_timings", "pyfof_ctx_stats",
     "pyfof_ctx_stream", "pyfof_ctx_set_option",
     "pyfof_last_error", "pyfof_version",
+    # request server (py_fof_amd.serve)
+    "pyfof_server_create", "pyfof_server_destroy", "pyfof_job_footprint", "pyfof_submit",
+    "pyfof_poll", "pyfof_wait", "pyfof_server_status", "pyfof_server_last_error",
 ]
 
 _lib = None

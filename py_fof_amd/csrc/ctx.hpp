@@ -117,4 +117,30 @@ struct Ctx {
 };
 
 
+// Arguments of one FoF (+ unbinding) call, as pyfof_fof takes them.
+struct CallArgs {
+    const double* xp;
+    const double* vp;
+    const double* mp;
+    const int32_t* np_halo;
+    int32_t nhalo;
+    double L, b;
+    int32_t np_min;
+    int32_t bound_check;
+    int32_t verbose;
+    double softening, a_val, b_val, s_15;
+    int64_t ng;
+    int32_t total_np;
+    int32_t* in_halo;
+    int32_t* halo_id;
+    double* ang_mom;
+    int32_t* root_out;
+    int32_t mem_kind;
+};
+
+// Runs one call on C (validation, then the pipeline); returns ierr. The
+// message of a failure is last_error_message() on the calling thread.
+int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads);
+const std::string& last_error_message();
+
 }  // namespace pyfof

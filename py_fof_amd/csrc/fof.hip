@@ -973,6 +973,14 @@ __global__ void __launch_bounds__(kBlk) k_region_angmom(const double* __restrict
 // host side
 
 void fof_init_device_tables() {
+        // once per device (contexts may be created from several threads)
+        static std::mutex mu;
+        static std::vector<bool> done;
+        std::lock_guard<std::mutex> lk(mu);
+        int dev = 0;
+        PYFOF_HIP(hipGetDevice(&dev));
+        if ((int)done.size() <= dev) done.resize(dev + 1, false);
+        if (done[dev]) return;
         // forward offsets of [-2,2]^3, nearest first (|o|^2 = 1, 2, 3, 4, ...)
         int8_t off[62][3];
         int n = 0;
@@ -986,33 +994,17 @@ void fof_init_device_tables() {
                 ++n;
             }
         PYFOF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fwd_off), off, sizeof(off)));
+        done[dev] = true;
 }
 
 static thread_local std::string g_last_error;
 
 static void set_error(const std::string& m) { g_last_error = m; }
 
+const std::string& last_error_message() { return g_last_error; }
+
 static inline uint32_t nblk(uint64_t n) { return div_up(n == 0 ? 1 : n, kBlk); }
 
-struct CallArgs {
-    const double* xp;
-    const double* vp;
-    const double* mp;
-    const int32_t* np_halo;
-    int32_t nhalo;
-    double L, b;
-    int32_t np_min;
-    int32_t bound_check;
-    int32_t verbose;
-    double softening, a_val, b_val, s_15;
-    int64_t ng;
-    int32_t total_np;
-    int32_t* in_halo;
-    int32_t* halo_id;
-    double* ang_mom;
-    int32_t* root_out;
-    int32_t mem_kind;
-};
 
 static int32_t run_fof(Ctx& C, const CallArgs& A) {
     hipStream_t s = C.stream;
@@ -1388,7 +1380,7 @@ static Ctx* default_ctx() {
     return g_default_ctx[dev];
 }
 
-static int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
+int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
     set_error("");
     // fof_fortran.f90:110-123 -- outputs cleared, thread bound, empty input
     const bool dev_mem = A.mem_kind == PYFOF_MEM_DEVICE;

@@ -68,3 +68,20 @@ def test_shape_errors():
 def test_empty_input_needs_no_device():
     in_h, am = py_fof_amd.fof(np.zeros((3, 0)), return_ang_mom=True)
     assert in_h.shape == (0,) and in_h.dtype == bool and am.shape == (3, 1)
+
+
+def test_job_descriptor_layout_matches_header():
+    # pyfof_job_desc: 7 pointers, 7 doubles, 1 int64, 8 int32 -> 128 bytes
+    from py_fof_amd.serve import _JobDesc
+    assert ctypes.sizeof(_JobDesc) == 7 * 8 + 7 * 8 + 8 + 8 * 4
+    assert _JobDesc.ng_val.offset == 112 and _JobDesc.nhalo.offset == 120
+
+
+def test_server_argument_errors_raise_before_submission():
+    # validation happens in the shared front end, before any device call
+    from py_fof_amd import serve
+    srv = serve.Server.__new__(serve.Server)  # no device: only the front end is used
+    with pytest.raises(ValueError, match="units must be"):
+        srv.submit(np.zeros((3, 4)), units="bogus")
+    with pytest.raises(ValueError, match="n_threads cannot exceed 256"):
+        srv.submit(np.zeros((3, 4)), n_threads=300)
