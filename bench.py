@@ -49,7 +49,13 @@ def parse():
     ap.add_argument("--bound-halos", type=int, default=20_000)
     ap.add_argument("--bound-steps", type=int, default=3)
     ap.add_argument("--bound-cpu-halos", type=int, default=2_000)
-    ap.add_argument("--only", choices=["all", "fof", "bound"], default="all",
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--serve-pool", type=int, default=32, help="distinct job inputs, reused")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="Poisson arrival rate, jobs/s (0: 0.9 x the sequential rate)")
+    ap.add_argument("--only", choices=["all", "fof", "bound", "serve"], default="all",
                     help="run one leg only (profiling); the headline needs 'all' or 'fof'")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
@@ -165,6 +171,115 @@ def bench_bound(args, torch, _native, ctx):
     return out
 
 
+def serve_pool(n_pool, seed=5):
+    """Job inputs of the serving stream: periodic sub-boxes of N particles,
+    N log-uniform in [1e4, 2e7], the NFW-clump / uniform 50/50 mixture at one
+    particle per cube cell (L = N^(1/3)), Gaussian velocities (sigma 0.5),
+    30 % with bound_check (s_15 from ng_global 2048)."""
+    from py_fof_amd.synth import clumpy_nfw
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(2e7), n_pool)).astype(np.int64)
+    bound = rng.uniform(size=n_pool) < 0.3
+    jobs = []
+    for k, (n, b) in enumerate(zip(ns, bound)):
+        L = float(n) ** (1.0 / 3.0)
+        xp = clumpy_nfw(n=int(n), L=L, seed=seed * 1000 + k)
+        vp = np.asfortranarray(rng.normal(0.0, 0.5, (3, int(n))))
+        kw = dict(L_Box=L, linking_length=0.2, np_halo_min=10, bound_check=bool(b),
+                  ng_global=2048.0)
+        jobs.append((xp, vp, kw))
+    return jobs
+
+
+def _pct(v, q):
+    return round(float(np.percentile(np.asarray(v), q)), 3)
+
+
+def bench_serve(args):
+    import threading
+    from py_fof_amd.serve import Server
+    log(args, f"serving: generating a pool of {args.serve_pool} jobs")
+    pool = serve_pool(args.serve_pool)
+    sizes = [j[0].shape[1] for j in pool]
+    with Server(slots=1, hbm_budget=200 << 30) as srv:
+        # sequential plain calls: one job at a time, warm workspace
+        for xp, vp, kw in pool[:2]:
+            srv.submit(xp, vp, **kw).result()
+        seq = []
+        t0 = time.perf_counter()
+        for xp, vp, kw in pool:
+            t = time.perf_counter()
+            srv.submit(xp, vp, **kw).result()
+            seq.append(time.perf_counter() - t)
+        seq_wall = time.perf_counter() - t0
+    seq_rate = len(pool) / seq_wall
+    log(args, f"serving: sequential {seq_rate:.2f} jobs/s over the pool")
+    # capacity: every job of 3 passes over the pool submitted at once
+    with Server(slots=args.serve_slots, hbm_budget=200 << 30) as srv:
+        for xp, vp, kw in pool[:args.serve_slots]:
+            srv.submit(xp, vp, **kw).result()
+        t0 = time.perf_counter()
+        hs = [srv.submit(xp, vp, **kw) for _ in range(3) for xp, vp, kw in pool]
+        for h in hs:
+            h.result()
+        cap_rate = len(hs) / (time.perf_counter() - t0)
+    log(args, f"serving: capacity with {args.serve_slots} slots {cap_rate:.2f} jobs/s")
+    rate = args.serve_rate if args.serve_rate > 0 else 0.9 * seq_rate
+    rng = np.random.default_rng(55)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, args.serve_jobs))
+    lat = [None] * args.serve_jobs
+    handles = [None] * args.serve_jobs
+    with Server(slots=args.serve_slots, hbm_budget=200 << 30) as srv:
+        for xp, vp, kw in pool[:args.serve_slots]:
+            srv.submit(xp, vp, **kw).result()
+        done = threading.Event()
+
+        def collect():
+            for i in range(args.serve_jobs):
+                while handles[i] is None:
+                    time.sleep(0.0005)
+                h, t_arr = handles[i]
+                h.result()
+                lat[i] = time.perf_counter() - t_arr
+            done.set()
+
+        th = threading.Thread(target=collect, daemon=True)
+        th.start()
+        t0 = time.perf_counter()
+        last_log = t0
+        for i in range(args.serve_jobs):
+            wait = t0 + arrivals[i] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            xp, vp, kw = pool[i % len(pool)]
+            t_arr = t0 + arrivals[i]
+            handles[i] = (srv.submit(xp, vp, **kw), t_arr)
+            if time.perf_counter() - last_log > 20.0:
+                last_log = time.perf_counter()
+                log(args, f"serving: {i + 1}/{args.serve_jobs} jobs submitted")
+        done.wait()
+        wall = time.perf_counter() - t0
+    lat_ms = [x * 1e3 for x in lat]
+    nb = sum(1 for j in pool if j[2]["bound_check"])
+    return {
+        "workload": f"{args.serve_jobs} jobs, Poisson arrivals at {rate:.2f} jobs/s, inputs "
+                    f"from a pool of {len(pool)} periodic sub-boxes (N log-uniform in [1e4, 2e7], "
+                    f"NFW clumps 50% + uniform, 1 particle per cell, sigma_v 0.5; {nb} of "
+                    f"{len(pool)} with bound_check), host inputs, {args.serve_slots} slots",
+        "arrival_rate_jobs_per_s": round(rate, 3),
+        "jobs_per_s": round(args.serve_jobs / wall, 3),
+        "p50_ms": _pct(lat_ms, 50),
+        "p99_ms": _pct(lat_ms, 99),
+        "mean_ms": round(float(np.mean(lat_ms)), 3),
+        "capacity_jobs_per_s": round(cap_rate, 3),
+        "sequential_jobs_per_s": round(seq_rate, 3),
+        "sequential_p50_ms": _pct([x * 1e3 for x in seq], 50),
+        "sequential_p99_ms": _pct([x * 1e3 for x in seq], 99),
+        "pool_particles": int(sum(sizes)),
+        "latency": "arrival to result, including queueing; sequential: one call at a time",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -180,6 +295,10 @@ def main():
         ctx = _native.Context(0)
         print(json.dumps({"metric": METRIC, "n_gpus": 1,
                           "unbinding": bench_bound(args, torch, _native, ctx)}), flush=True)
+        return
+    if args.only == "serve":
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "serving": bench_serve(args)}),
+              flush=True)
         return
 
     n = args.n
@@ -269,6 +388,8 @@ def main():
         del x_d, in_h, hid
         torch.cuda.empty_cache()
         out["unbinding"] = bench_bound(args, torch, _native, ctx)
+    if not args.no_serve and args.only == "all":
+        out["serving"] = bench_serve(args)
     print(json.dumps(out), flush=True)
 
 

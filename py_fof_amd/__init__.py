@@ -15,6 +15,17 @@ __all__ = ["fof", "fof_with_info", "Context", "NativeLibraryError"]
 __version__ = "0.1.0"
 
 
+def _pack(arrays):
+    """(3, N) f64 Fortran-order concatenation (py_fof/__init__.py:210); a single
+    array that already has that layout is used as is, without a copy."""
+    if len(arrays) == 1:
+        a = arrays[0]
+        if a.dtype == np.float64 and a.flags.f_contiguous:
+            return a
+        return np.asfortranarray(a, dtype=np.float64)
+    return np.asfortranarray(np.hstack(arrays), dtype=np.float64)
+
+
 def _error_for(ierr):
     # py_fof/__init__.py:338-348
     if ierr == 1:
@@ -74,7 +85,7 @@ def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
     if P.empty:
         return P
 
-    xp_halo = np.asfortranarray(np.hstack(xp_list_norm), dtype=np.float64)
+    xp_halo = _pack(xp_list_norm)
 
     if bound_check and vp_list is None:
         raise ValueError("vp_list is required when bound_check=True")
@@ -92,7 +103,7 @@ def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
         for i, vp in enumerate(vp_list_norm):
             if not isinstance(vp, np.ndarray) or vp.ndim != 2 or vp.shape != xp_list_norm[i].shape:
                 raise ValueError(f"vp_list[{i}] shape {getattr(vp, 'shape', None)} must match xp_list[{i}] shape {xp_list_norm[i].shape}")
-        vp_halo = np.asfortranarray(np.hstack(vp_list_norm), dtype=np.float64)
+        vp_halo = _pack(vp_list_norm)
     # a missing vp_list means zero velocities (py_fof/__init__.py:233); the
     # library treats a null vp that way, so no zero array is materialised.
 

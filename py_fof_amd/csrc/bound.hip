@@ -457,31 +457,14 @@ __global__ void k_deposit(const float4* __restrict__ qw, const uint32_t* __restr
     }
 }
 
-__global__ void k_fix_to_complex(const unsigned long long* __restrict__ acc, float2* __restrict__ g,
-                                 const int32_t* __restrict__ slots, const int32_t* __restrict__ slot_halo,
-                                 const double* __restrict__ fscale, int32_t n) {
+// zero the deposit nodes [0, mc)^3 of each listed slot
+__global__ void k_zero_region(unsigned long long* __restrict__ acc, const int32_t* __restrict__ slots,
+                              int32_t n, uint32_t mc) {
+    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (id >= (uint64_t)mc * mc * mc) return;
+    const uint32_t x = (uint32_t)(id % mc), y = (uint32_t)((id / mc) % mc), z = (uint32_t)(id / ((uint64_t)mc * mc));
     const uint64_t grid = (uint64_t)n * n * n;
-    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
-    const uint32_t sl = blockIdx.y;
-    if (id >= grid) return;
-    const size_t off = (size_t)slots[sl] * grid + id;
-    const double inv = 1.0 / fscale[slot_halo[slots[sl]]];
-    g[off] = make_float2((float)((double)(long long)acc[off] * inv), 0.0f);
-}
-
-__global__ void k_zero_u64(unsigned long long* __restrict__ acc, const int32_t* __restrict__ slots,
-                           uint64_t grid) {
-    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
-    if (id < grid) acc[(size_t)slots[blockIdx.y] * grid + id] = 0ull;
-}
-
-__global__ void k_mul_green(float2* __restrict__ g, const float* __restrict__ G,
-                            const int32_t* __restrict__ slots, uint64_t grid) {
-    const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
-    if (id >= grid) return;
-    float2& v = g[(size_t)slots[blockIdx.y] * grid + id];
-    const float gk = G[id];
-    v = make_float2(v.x * gk, v.y * gk);
+    acc[(size_t)slots[blockIdx.y] * grid + x + (uint64_t)n * (y + (uint64_t)n * z)] = 0ull;
 }
 
 // C1 potential (bound_fortran.f90:2680-2703)
@@ -780,23 +763,53 @@ __global__ void k_phi_combine(const uint32_t* __restrict__ mhalo, const int32_t*
 // Removal loop of one PM halo per workgroup (bound_fortran.f90:2756-2856):
 // repeated 2 % batches by energy until no unbound member is left or 20 % of
 // the kept members have gone since the potential was computed.
+// Per batch: one pass for the f64 centre of mass and bulk velocity, one pass
+// for the f32 energies that also lists the unbound members in index order
+// (ul, with their energy keys uk); the nb-th largest key is then found by a
+// radix select over that short list, and deletion walks it in index order
+// (ties at the threshold: smaller index first).
+constexpr int kPB = 1024;  // threads of k_pm_inner
+constexpr int kPW = kPB / kWave;
+
 __device__ __forceinline__ uint32_t ekey(float e) { return e > 0.0f ? __float_as_uint(e) : 0u; }
 
-__global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ alist,
-                                                 const uint32_t* __restrict__ hoff,
-                                                 const float4* __restrict__ qw,
-                                                 const float4* __restrict__ vv,
-                                                 const float* __restrict__ phi,
-                                                 float* __restrict__ en, uint8_t* __restrict__ keep,
-                                                 int32_t* __restrict__ status,
-                                                 int32_t* __restrict__ npb,
-                                                 uint8_t* __restrict__ active,
-                                                 double* __restrict__ mkeep, int32_t last_pass,
-                                                 int32_t* __restrict__ err, Params P) {
-    __shared__ double red[7][kB / kWave];
+// exclusive rank of `flag` among the workgroup's threads in thread order,
+// and the workgroup total; wcnt holds kPW counters in LDS
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* wcnt, uint32_t& total) {
+    const int w = threadIdx.x / kWave;
+    const uint64_t bal = __ballot(flag);
+    const uint32_t in_wave = __popcll(bal & lanemask_lt());
+    if (__lane_id() == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int k = 0; k < kPW; ++k) {
+        const uint32_t c = wcnt[k];
+        before += k < w ? c : 0u;
+        total += c;
+    }
+    __syncthreads();
+    return before + in_wave;
+}
+
+__global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ alist,
+                                                  const uint32_t* __restrict__ hoff,
+                                                  const float4* __restrict__ qw,
+                                                  const float4* __restrict__ vv,
+                                                  const float* __restrict__ phi,
+                                                  uint32_t* __restrict__ ul, uint32_t* __restrict__ uk,
+                                                  uint8_t* __restrict__ keep,
+                                                  int32_t* __restrict__ status,
+                                                  int32_t* __restrict__ npb,
+                                                  uint8_t* __restrict__ active,
+                                                  double* __restrict__ mkeep, int32_t last_pass,
+                                                  int32_t* __restrict__ err, Params P) {
+    __shared__ double red[7][kPW];
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t sh[8];
+    __shared__ uint32_t wcnt[kPW];
+    __shared__ uint32_t sh[4];
     __shared__ double bc[7];
+    __shared__ int bad_any;
     const int32_t h = alist[blockIdx.x];
     const uint32_t s0 = h ? hoff[h - 1] : 0u, e0 = hoff[h];
     const int tid = threadIdx.x, w = tid / kWave;
@@ -804,8 +817,9 @@ __global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ ali
     int np_bound = npb[h];
     int deleted = 0, st = -2;  // -2: still iterating
     for (;;) {
+        // centre of mass and bulk velocity, f64, over kept members
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t t = s0 + tid; t < e0; t += kB) {
+        for (uint32_t t = s0 + tid; t < e0; t += kPB) {
             if (!keep[t]) continue;
             const float4 q = qw[t];
             const float4 u = vv[t];
@@ -817,56 +831,58 @@ __global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ ali
         for (int k = 0; k < 7; ++k)
             for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, kWave);
         if (__lane_id() == 0) for (int k = 0; k < 7; ++k) red[k][w] = v[k];
+        if (tid == 0) bad_any = 0;
         __syncthreads();
         if (tid < 7) {
-            double s = 0.0;
-            for (int j = 0; j < kB / kWave; ++j) s += red[tid][j];
-            bc[tid] = s;
+            double sacc = 0.0;
+            for (int j = 0; j < kPW; ++j) sacc += red[tid][j];
+            bc[tid] = sacc;
         }
-        if (tid == 0) sh[0] = 0;  // unbound count
         __syncthreads();
         const double ws = bc[0];
         const float qc32[3] = {(float)(bc[1] / ws), (float)(bc[2] / ws), (float)(bc[3] / ws)};
         const float vc32[3] = {(float)(bc[4] / ws), (float)(bc[5] / ws), (float)(bc[6] / ws)};
-        uint32_t nu_loc = 0;
-        bool bad = false;
-        for (uint32_t t = s0 + tid; t < e0; t += kB) {
-            if (!keep[t]) continue;
-            const float4 q = qw[t];
-            const float4 u = vv[t];
-            const float d0 = __fadd_rn(__fsub_rn(u.x, vc32[0]), __fmul_rn(b32, __fsub_rn(q.x, qc32[0])));
-            const float d1 = __fadd_rn(__fsub_rn(u.y, vc32[1]), __fmul_rn(b32, __fsub_rn(q.y, qc32[1])));
-            const float d2 = __fadd_rn(__fsub_rn(u.z, vc32[2]), __fmul_rn(b32, __fsub_rn(q.z, qc32[2])));
-            float s = __fmul_rn(d0, d0);
-            s = __fadd_rn(s, __fmul_rn(d1, d1));
-            s = __fadd_rn(s, __fmul_rn(d2, d2));
-            const float E = __fadd_rn(s / 2.0f, phi[t]);
-            en[t] = E;
-            if (!isfinite(E)) bad = true;
-            if (E >= 0.0f) ++nu_loc;
+        // energies; unbound members listed in index order
+        uint32_t nu = 0;
+        for (uint32_t base = s0; base < e0; base += kPB) {
+            const uint32_t t = base + tid;
+            bool unb = false;
+            uint32_t key = 0;
+            if (t < e0 && keep[t]) {
+                const float4 q = qw[t];
+                const float4 u = vv[t];
+                const float d0 = __fadd_rn(__fsub_rn(u.x, vc32[0]), __fmul_rn(b32, __fsub_rn(q.x, qc32[0])));
+                const float d1 = __fadd_rn(__fsub_rn(u.y, vc32[1]), __fmul_rn(b32, __fsub_rn(q.y, qc32[1])));
+                const float d2 = __fadd_rn(__fsub_rn(u.z, vc32[2]), __fmul_rn(b32, __fsub_rn(q.z, qc32[2])));
+                float sq = __fmul_rn(d0, d0);
+                sq = __fadd_rn(sq, __fmul_rn(d1, d1));
+                sq = __fadd_rn(sq, __fmul_rn(d2, d2));
+                const float E = __fadd_rn(sq / 2.0f, phi[t]);
+                if (!isfinite(E)) bad_any = 1;
+                else if (E >= 0.0f) { unb = true; key = ekey(E); }
+            }
+            uint32_t tot;
+            const uint32_t r = block_rank(unb, wcnt, tot);
+            if (unb) { ul[s0 + nu + r] = t; uk[s0 + nu + r] = key; }
+            nu += tot;
         }
-        atomicAdd(&sh[0], nu_loc);
-        if (__syncthreads_or(bad)) { st = -1; break; }
-        const int nu = (int)sh[0];
+        __syncthreads();
+        if (bad_any) { st = -1; break; }
         if (nu == 0) {
-            if (deleted == 0) { st = 0; break; }
-            break;  // refresh the potential
+            if (deleted == 0) st = 0;
+            break;  // status OK, or refresh the potential
         }
         int nb = (int)ceil(0.02 * (double)np_bound);
         nb = nb < 1 ? 1 : nb;
-        nb = nb > nu ? nu : nb;
-        // radix select of the nb-th largest energy key among unbound members
-        uint32_t prefix = 0, mask = 0;
-        uint32_t need = (uint32_t)nb;
+        nb = nb > (int)nu ? (int)nu : nb;
+        // radix select of the nb-th largest key in uk[s0, s0 + nu)
+        uint32_t prefix = 0, mask = 0, need = (uint32_t)nb;
         for (int byte = 3; byte >= 0; --byte) {
-            for (int i = tid; i < 256; i += kB) hist[i] = 0;
+            for (int i = tid; i < 256; i += kPB) hist[i] = 0;
             __syncthreads();
-            for (uint32_t t = s0 + tid; t < e0; t += kB) {
-                if (!keep[t]) continue;
-                const float E = en[t];
-                if (!(E >= 0.0f)) continue;
-                const uint32_t k = ekey(E);
-                if ((k & mask) == prefix) atomicAdd(&hist[(k >> (8 * byte)) & 255u], 1u);
+            for (uint32_t k = tid; k < nu; k += kPB) {
+                const uint32_t key = uk[s0 + k];
+                if ((key & mask) == prefix) atomicAdd(&hist[(key >> (8 * byte)) & 255u], 1u);
             }
             __syncthreads();
             if (tid == 0) {
@@ -885,34 +901,26 @@ __global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ ali
             need = sh[2];
             __syncthreads();
         }
-        // delete keys > T, and the `need` smallest indices with key == T
+        // delete keys > T, and the `need` first (smallest index) keys == T
         const uint32_t T = prefix;
-        uint32_t taken = 0;  // running count of key == T seen in index order
-        for (uint32_t base = s0; base < e0; base += kB) {
-            const uint32_t t = base + tid;
+        uint32_t taken = 0;
+        for (uint32_t base = 0; base < nu; base += kPB) {
+            const uint32_t k = base + tid;
             bool del = false, tie = false;
-            if (t < e0 && keep[t]) {
-                const float E = en[t];
-                if (E >= 0.0f) {
-                    const uint32_t k = ekey(E);
-                    if (k > T) del = true;
-                    else if (k == T) tie = true;
-                }
+            uint32_t t = 0;
+            if (k < nu) {
+                t = ul[s0 + k];
+                const uint32_t key = uk[s0 + k];
+                if (key > T) del = true;
+                else if (key == T) tie = true;
             }
-            // rank of this tie inside the 256-wide chunk
-            const uint64_t bal = __ballot(tie);
-            const uint32_t in_wave = __popcll(bal & lanemask_lt());
-            if (__lane_id() == 0) sh[4 + w] = __popcll(bal);
-            __syncthreads();
-            uint32_t before = taken;
-            for (int k = 0; k < w; ++k) before += sh[4 + k];
-            if (tie && before + in_wave < need) del = true;
+            uint32_t tot;
+            const uint32_t r = block_rank(tie, wcnt, tot);
+            if (tie && taken + r < need) del = true;
             if (del) keep[t] = 0;
-            uint32_t chunk = 0;
-            for (int k = 0; k < kB / kWave; ++k) chunk += sh[4 + k];
-            taken += chunk;
-            __syncthreads();
+            taken += tot;
         }
+        __syncthreads();
         np_bound -= nb;
         deleted += nb;
         if (np_bound < P.np_min) { st = 1; break; }
@@ -935,14 +943,14 @@ __global__ void __launch_bounds__(kB) k_pm_inner(const int32_t* __restrict__ ali
     }
     // kept mass for the next potential (zero-mode term)
     double m = 0.0;
-    for (uint32_t t = s0 + tid; t < e0; t += kB)
+    for (uint32_t t = s0 + tid; t < e0; t += kPB)
         if (keep[t]) m += (double)qw[t].w;
     for (int o = 32; o > 0; o >>= 1) m += __shfl_down(m, o, kWave);
     if (__lane_id() == 0) red[0][w] = m;
     __syncthreads();
     if (tid == 0) {
         double sm = 0.0;
-        for (int j = 0; j < kB / kWave; ++j) sm += red[0][j];
+        for (int j = 0; j < kPW; ++j) sm += red[0][j];
         mkeep[h] = sm;
     }
 }
@@ -1275,7 +1283,8 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         float* phi_c1 = C.buf("phic1").as<float>(M);
         float* phi_c2 = C.buf("phic2").as<float>(M);
         float* phi = C.buf("phi").as<float>(M);
-        float* en = C.buf("energy").as<float>(M);
+        uint32_t* ulist = C.buf("ulist").as<uint32_t>(M);
+        uint32_t* ukey = C.buf("ukey").as<uint32_t>(M);
         uint8_t* active = C.buf("active").as<uint8_t>(nh);
         double* mkeep = C.buf("mkeep").as<double>(nh);
         std::vector<uint8_t> act_host(nh, 0);
@@ -1308,20 +1317,17 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                 }
                 if (sl.empty()) continue;
                 const int n = sc.bn[g];
-                const uint64_t gsz = (uint64_t)n * n * n;
                 PYFOF_HIP(hipMemcpyAsync(d_slots, sl.data(), 4 * sl.size(), hipMemcpyHostToDevice, s));
-                const dim3 gg(blocks(gsz), (uint32_t)sl.size());
-                k_zero_u64<<<gg, kB, 0, s>>>(acc[g], d_slots, gsz);
+                // deposit nodes [0, m]^3, m = n/2 + 1 (k_tsc_setup's domain guard)
+                const uint32_t mc = (uint32_t)(n / 2 + 2);
+                const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)sl.size());
+                k_zero_region<<<gz, kB, 0, s>>>(acc[g], d_slots, n, mc);
                 PYFOF_LAUNCH_CHECK();
                 k_deposit<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, active, fscale,
                                                    g, M, acc[g]);
                 PYFOF_LAUNCH_CHECK();
-                k_fix_to_complex<<<gg, kB, 0, s>>>(acc[g], grid[g], d_slots, d_slot_halo[g], fscale, n);
-                PYFOF_LAUNCH_CHECK();
-                fft3d(*plans[g], grid[g], d_slots, (int32_t)sl.size(), false, s);
-                k_mul_green<<<gg, kB, 0, s>>>(grid[g], Gfull[g], d_slots, gsz);
-                PYFOF_LAUNCH_CHECK();
-                fft3d(*plans[g], grid[g], d_slots, (int32_t)sl.size(), true, s);
+                fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
+                           Gfull[g], s);
                 k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
                                                   mkeep, grid[g], g, M, A.a, phi_c1);
                 PYFOF_LAUNCH_CHECK();
@@ -1359,7 +1365,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                                                    phi, err);
             PYFOF_LAUNCH_CHECK();
             PYFOF_HIP(hipMemcpyAsync(d_alist, act_list.data(), 4 * act_list.size(), hipMemcpyHostToDevice, s));
-            k_pm_inner<<<(uint32_t)act_list.size(), kB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, en, keep,
+            k_pm_inner<<<(uint32_t)act_list.size(), kPB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, ulist, ukey, keep,
                                                                 status, npb, active, mkeep, pass == 20,
                                                                 err, P);
             PYFOF_LAUNCH_CHECK();

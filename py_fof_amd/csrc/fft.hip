@@ -63,34 +63,16 @@ __device__ __forceinline__ void dft_small(float2* v, int r) {
     }
 }
 
+// Stockham autosort stages on the kLines lines held in buf[start]; returns the
+// buffer holding the result. Every thread of the workgroup must call it.
 template <bool INV>
-__global__ void __launch_bounds__(kFftBlock) k_fft_axis(float2* __restrict__ data, int32_t n,
-                                                        const int32_t* __restrict__ slots,
-                                                        Radices rad, const float2* __restrict__ tw,
-                                                        AxisDesc ax) {
-    __shared__ float2 buf[2][kLines * kMaxN];
-    const int64_t grid = (int64_t)n * n * n;
-    const int32_t g = blockIdx.x;
-    const int32_t i0 = g % ax.c0;
-    const int32_t i1 = (g / ax.c0) % ax.c1;
-    const int32_t ib = g / (ax.c0 * ax.c1);
-    float2* base = data + (int64_t)slots[ib] * grid + (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
-    const int tid = threadIdx.x;
-    const int total = kLines * n;
-    // load
-    for (int idx = tid; idx < total; idx += kFftBlock) {
-        int c, e;
-        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
-        else { e = idx / kLines; c = idx - e * kLines; }
-        buf[0][c * n + e] = base[(int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
-    }
-    __syncthreads();
-    // stages
-    int cur = 0;
+__device__ __forceinline__ int stockham(float2 (*buf)[kLines * kMaxN], int start, int n,
+                                        const Radices& rad, const float2* __restrict__ tw) {
+    int cur = start;
     int ns = 1;
     const int tpl = kFftBlock / kLines;  // threads per line
-    const int c = tid / tpl;
-    const int lt = tid - c * tpl;
+    const int c = threadIdx.x / tpl;
+    const int lt = threadIdx.x - c * tpl;
     for (int st = 0; st < rad.nrad; ++st) {
         const int r = rad.r[st];
         const int nb = n / r;
@@ -113,12 +95,119 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_axis(float2* __restrict__ dat
         cur ^= 1;
         __syncthreads();
     }
+    return cur;
+}
+
+template <bool INV>
+__global__ void __launch_bounds__(kFftBlock) k_fft_axis(float2* __restrict__ data, int32_t n,
+                                                        const int32_t* __restrict__ slots,
+                                                        Radices rad, const float2* __restrict__ tw,
+                                                        AxisDesc ax) {
+    __shared__ float2 buf[2][kLines * kMaxN];
+    const int64_t grid = (int64_t)n * n * n;
+    const int32_t g = blockIdx.x;
+    const int32_t i0 = g % ax.c0;
+    const int32_t i1 = (g / ax.c0) % ax.c1;
+    const int32_t ib = g / (ax.c0 * ax.c1);
+    float2* base = data + (int64_t)slots[ib] * grid + (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
+    const int tid = threadIdx.x;
+    const int total = kLines * n;
+    // load
+    for (int idx = tid; idx < total; idx += kFftBlock) {
+        int c, e;
+        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+        else { e = idx / kLines; c = idx - e * kLines; }
+        buf[0][c * n + e] = base[(int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
+    }
+    __syncthreads();
+    const int cur = stockham<INV>(buf, 0, n, rad, tw);
     // store
     for (int idx = tid; idx < total; idx += kFftBlock) {
         int cc, e;
         if (ax.x_contig) { cc = idx / n; e = idx - cc * n; }
         else { e = idx / kLines; cc = idx - e * kLines; }
         base[(int64_t)cc * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][cc * n + e];
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Pruned convolution of zero-padded grids. Sources occupy nodes [0, m]^3 with
+// m = n/2 + 1 and only those nodes are read back, so each pass touches only
+// the lines that can be non-zero or are needed later, and zero-fills the rest
+// in LDS:
+//   x forward  lines y, z <= m, input x <= m read from the int64 deposit
+//   y forward  lines z <= m, input y <= m
+//   z          all lines: forward, x G(k), inverse; output z <= m
+//   y inverse  lines z <= m; output y <= m
+//   x inverse  lines y, z <= m; output x <= m
+
+enum { kConvXFwd = 0, kConvYFwd = 1, kConvZ = 2, kConvYInv = 3, kConvXInv = 4 };
+
+struct ConvDesc {
+    AxisDesc ax;
+    int32_t line_lim;  // x passes: lines y > line_lim are skipped (zero)
+    int32_t in_lim;    // elements e > in_lim load as zero
+    int32_t out_lim;   // elements e > out_lim are not stored
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ data,
+                                                        const unsigned long long* __restrict__ acc,
+                                                        int32_t n, const int32_t* __restrict__ slots,
+                                                        const int32_t* __restrict__ slot_halo,
+                                                        const double* __restrict__ fscale,
+                                                        const float* __restrict__ G, Radices rad,
+                                                        const float2* __restrict__ tw, ConvDesc d) {
+    __shared__ float2 buf[2][kLines * kMaxN];
+    const AxisDesc& ax = d.ax;
+    const int64_t grid = (int64_t)n * n * n;
+    const int32_t g = blockIdx.x;
+    const int32_t i0 = g % ax.c0;
+    const int32_t i1 = (g / ax.c0) % ax.c1;
+    const int32_t ib = g / (ax.c0 * ax.c1);
+    const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
+    const int64_t sbase = (int64_t)slots[ib] * grid;
+    const int line0 = ax.x_contig ? i0 * kLines : 0;  // y of line 0 in the x passes
+    double inv = 0.0;
+    if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
+    const int tid = threadIdx.x;
+    const int total = kLines * n;
+    for (int idx = tid; idx < total; idx += kFftBlock) {
+        int c, e;
+        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+        else { e = idx / kLines; c = idx - e * kLines; }
+        float2 v = make_float2(0.0f, 0.0f);
+        if (e <= d.in_lim && (!ax.x_contig || line0 + c <= d.line_lim)) {
+            const int64_t o = sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride;
+            if (MODE == kConvXFwd) v = make_float2((float)((double)(long long)acc[o] * inv), 0.0f);
+            else v = data[o];
+        }
+        buf[0][c * n + e] = v;
+    }
+    __syncthreads();
+    int cur;
+    if (MODE == kConvXFwd || MODE == kConvYFwd) {
+        cur = stockham<false>(buf, 0, n, rad, tw);
+    } else if (MODE == kConvZ) {
+        cur = stockham<false>(buf, 0, n, rad, tw);
+        for (int idx = tid; idx < total; idx += kFftBlock) {
+            const int e = idx / kLines, c = idx - e * kLines;
+            const float gk = G[goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
+            float2& v = buf[cur][c * n + e];
+            v = make_float2(v.x * gk, v.y * gk);
+        }
+        __syncthreads();
+        cur = stockham<true>(buf, cur, n, rad, tw);
+    } else {
+        cur = stockham<true>(buf, 0, n, rad, tw);
+    }
+    for (int idx = tid; idx < total; idx += kFftBlock) {
+        int c, e;
+        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+        else { e = idx / kLines; c = idx - e * kLines; }
+        if (e > d.out_lim || (ax.x_contig && line0 + c > d.line_lim)) continue;
+        data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * n + e];
     }
 }
 
@@ -166,6 +255,38 @@ void fft3d(const FftPlan& p, float2* data, const int32_t* slots, int32_t nslots,
             k_fft_axis<false><<<groups, kFftBlock, 0, s>>>(data, n, slots, rad, p.tw, ax[a]);
         PYFOF_LAUNCH_CHECK();
     }
+}
+
+void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
+                int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
+                hipStream_t s) {
+    if (nslots <= 0) return;
+    const int n = p.n;
+    const int m = n / 2 + 1;
+    const int mc = m + 1;
+    Radices rad;
+    rad.nrad = p.nrad;
+    for (int i = 0; i < 16; ++i) rad.r[i] = p.radix[i];
+    const int64_t n2 = (int64_t)n * n;
+    const int ygroups = (mc + kLines - 1) / kLines;
+    const AxisDesc ax_x{(int64_t)kLines * n, n2, ygroups, mc, n, 1, 1};
+    const AxisDesc ax_y{kLines, n2, n / kLines, mc, 1, n, 0};
+    const AxisDesc ax_z{kLines, n, n / kLines, n, 1, n2, 0};
+    const ConvDesc dxf{ax_x, m, m, n - 1}, dyf{ax_y, n, m, n - 1}, dz{ax_z, n, m, m},
+        dyi{ax_y, n, n - 1, m}, dxi{ax_x, m, n - 1, m};
+    const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
+    const uint32_t gy = (uint32_t)((n / kLines) * mc) * (uint32_t)nslots;
+    const uint32_t gz = (uint32_t)((n / kLines) * n) * (uint32_t)nslots;
+    k_fft_conv<kConvXFwd><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dxf);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvYFwd><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dyf);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvZ><<<gz, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dz);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvYInv><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dyi);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvXInv><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dxi);
+    PYFOF_LAUNCH_CHECK();
 }
 
 }  // namespace pyfof

@@ -25,4 +25,14 @@ void fft_make_plan(FftPlan& p, int n, float2* tw_dev, hipStream_t s);
 void fft3d(const FftPlan& p, float2* data, const int32_t* slots, int32_t nslots, bool inverse,
            hipStream_t s);
 
+// Potential of zero-padded sources by circular convolution with the real
+// spectrum G (n^3 f32): grid slot k <- IFFT(G x FFT(acc_k / fscale)) on the
+// nodes [0, n/2 + 1]^3, unnormalised (divide by n^3). acc holds each slot's
+// int64 fixed-point deposit, non-zero only on [0, n/2 + 1]^3; slot_halo maps a
+// slot to the halo whose scale fscale[] it uses. Other nodes of grid are
+// left with intermediate values.
+void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
+                int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
+                hipStream_t s);
+
 }  // namespace pyfof
