@@ -61,10 +61,35 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
                       int32_t total_np, int64_t ng_val, double ng_global_val, int32_t* in_halo,
                       int32_t* halo_id, double* ang_mom, int32_t* root_out, int32_t mem_kind);
 
-/* Context options. PYFOF_OPT_CATALOG (default 0): when non-zero, every call
- * also groups halo members for pyfof_catalog_* (fof_with_info). */
+/* Context options (no counterpart in the reference, whose solver form is
+ * fixed; the defaults reproduce it):
+ *   PYFOF_OPT_CATALOG (default 0): every call also groups halo members for
+ *     pyfof_catalog_* (fof_with_info).
+ *   PYFOF_OPT_DIRECT_MAX (default 1000, range [1000, 65536]): candidates of
+ *     up to this many members are unbound by direct summation; larger ones
+ *     by the particle mesh. 1000 is the reference's rule
+ *     (bound_fortran.f90:1976-1980).
+ *   PYFOF_OPT_DIRECT_FP32 (default 0): direct-summation pair terms and
+ *     potential sums in f32; centre-of-mass and bulk-velocity sums stay f64.
+ *     0 sums in FP64 exactly as the reference (bound_fortran.f90:2613-2647).
+ *   PYFOF_OPT_BATCHED (default 1): all candidates of a kind advance together,
+ *     one launch per step; 0 unbinds one candidate at a time (plain form).
+ *   PYFOF_OPT_RECORD_PHI (default 0): keep the first-pass potential of every
+ *     candidate member for pyfof_ctx_potential.
+ * Returns 0, or 1 for an unknown option or an out-of-range value. */
 #define PYFOF_OPT_CATALOG 1
+#define PYFOF_OPT_DIRECT_MAX 2
+#define PYFOF_OPT_DIRECT_FP32 3
+#define PYFOF_OPT_BATCHED 4
+#define PYFOF_OPT_RECORD_PHI 5
 int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value);
+
+/* First-pass potentials recorded by the last bound_check call on ctx (with
+ * PYFOF_OPT_RECORD_PHI): phi (cube units, f32) and the particle index of
+ * every FoF-candidate member, candidates in halo order, members ascending.
+ * With phi and index both NULL returns the count; otherwise cap must be at
+ * least the count. Returns the count, or -1 on error. */
+int64_t pyfof_ctx_potential(pyfof_ctx* ctx, float* phi, int32_t* index, int64_t cap, int32_t mem_kind);
 
 /*
  * Catalog of the last call on ctx, as fof_with_info builds it on the host
@@ -86,7 +111,8 @@ int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
  * [2] dense cells given sub-cells, [3] halos in the output, [4] members of
  * those halos; with bound_check: [5] valid bound halos, [6..8] candidates
  * ending OK / TOO_FEW / MAX_ITER, [9] direct-summation candidates,
- * [10..13] candidates on the 48/64/128/256 grids, [14] potential passes.
+ * [10..13] candidates on the 48/64/128/256 grids, [14] potential passes,
+ * [15] direct-summation candidates above 1000 members.
  * Returns the number of values written. */
 int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n);
 

@@ -15,11 +15,15 @@ LIB_PATH = os.path.join(HERE, "libpyfof_hip.so")
 MEM_HOST = 0
 MEM_DEVICE = 1
 OPT_CATALOG = 1
+OPT_DIRECT_MAX = 2
+OPT_DIRECT_FP32 = 3
+OPT_BATCHED = 4
+OPT_RECORD_PHI = 5
 
 EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",
     "pyfof_catalog_size", "pyfof_catalog_copy", "pyfof_ctx_timings", "pyfof_ctx_stats",
-    "pyfof_ctx_stream", "pyfof_ctx_set_option",
+    "pyfof_ctx_stream", "pyfof_ctx_set_option", "pyfof_ctx_potential",
     "pyfof_last_error", "pyfof_version",
     # request server (py_fof_amd.serve)
     "pyfof_server_create", "pyfof_server_destroy", "pyfof_job_footprint", "pyfof_submit",
@@ -72,6 +76,8 @@ def _load_from(path):
     lib.pyfof_ctx_set_option.argtypes = [vp, i32, i64]
     lib.pyfof_ctx_stats.restype = i32
     lib.pyfof_ctx_stats.argtypes = [vp, ctypes.POINTER(i64), i32]
+    lib.pyfof_ctx_potential.restype = i64
+    lib.pyfof_ctx_potential.argtypes = [vp, vp, vp, i64, i32]
     lib.pyfof_ctx_stream.restype = vp
     lib.pyfof_ctx_stream.argtypes = [vp]
     lib.pyfof_last_error.restype = ctypes.c_char_p
@@ -151,11 +157,25 @@ class Context:
 
     def stats(self):
         """dict of the last call's sizes (particles, cells, dense cells, halos, members)."""
-        buf = (ctypes.c_int64 * 15)()
-        n = self.lib.pyfof_ctx_stats(self.h, buf, 15)
+        buf = (ctypes.c_int64 * 16)()
+        n = self.lib.pyfof_ctx_stats(self.h, buf, 16)
         keys = ["n", "ncell", "ndense", "nhalo", "nmem", "nvalid", "n_ok", "n_too_few",
-                "n_max_iter", "n_direct", "n_pm48", "n_pm64", "n_pm128", "n_pm256", "passes"]
+                "n_max_iter", "n_direct", "n_pm48", "n_pm64", "n_pm128", "n_pm256", "passes",
+                "n_direct_large"]
         return {keys[i]: int(buf[i]) for i in range(n)}
+
+    def potential(self):
+        """(phi f32, particle index int32) of the last bound_check call's
+        candidate members, recorded with OPT_RECORD_PHI."""
+        n = self.lib.pyfof_ctx_potential(self.h, None, None, 0, MEM_HOST)
+        if n < 0:
+            raise RuntimeError(last_error())
+        phi = np.zeros(max(n, 1), np.float32)
+        idx = np.zeros(max(n, 1), np.int32)
+        if n > 0 and self.lib.pyfof_ctx_potential(self.h, phi.ctypes.data, idx.ctypes.data, n,
+                                                  MEM_HOST) < 0:
+            raise RuntimeError(last_error())
+        return phi[:n], idx[:n]
 
     def stream(self):
         return self.lib.pyfof_ctx_stream(self.h)

@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kB = 256;
 constexpr int kDirectMax = 1000;  // bound_fortran.f90:1976-1980
+constexpr int kDirectLargeMax = 65536;  // largest direct-summation threshold a context accepts
 constexpr double kPi = 3.141592653589793238462643383279502884197;
 
 inline uint32_t blocks(uint64_t n) { return div_up(n == 0 ? 1 : n, kB); }
@@ -50,6 +51,46 @@ __device__ double depth_continuous(double r, double h) {
     const double prim = e * e * (448.0 + e * (-280.0 + e * (56.0 + e * (14.0 + e * (-8.0 + e))))) -
                         224.0 * e - 12.0 / e;
     return (128.0 - prim) / (70.0 * h);
+}
+
+// the same kernel in f32 (mixed-precision direct summation)
+__device__ float depth_continuous_f(float r, float h) {
+    if (r <= 0.0f) return 208.0f / (70.0f * h);
+    if (r >= h) return 1.0f / r;
+    const float e = 2.0f * r / h;
+    if (e <= 1.0f) {
+        const float prim = e * e * (112.0f + e * e * (-56.0f + e * (14.0f + e * (8.0f - 3.0f * e))));
+        return (208.0f - prim) / (70.0f * h);
+    }
+    const float prim = e * e * (448.0f + e * (-280.0f + e * (56.0f + e * (14.0f + e * (-8.0f + e))))) -
+                       224.0f * e - 12.0f / e;
+    return (128.0f - prim) / (70.0f * h);
+}
+
+// one pair term w_j (a/6pi) D(|q_j - q_i|): the reference's form in f64 (f32
+// differences widened, bound_fortran.f90:2616-2628) or all in f32
+template <bool F32>
+__device__ __forceinline__ double pair_term(const float4& qi, const float4& qj, double Gd, float Gf,
+                                            double app, float appf) {
+    if (F32) {
+        const float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
+        const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+        return (double)__fmul_rn(qj.w, __fmul_rn(Gf, depth_continuous_f(__fsqrt_rn(r2), appf)));
+    }
+    const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
+    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+    return __dmul_rn((double)qj.w, __dmul_rn(Gd, depth_continuous(r, app)));
+}
+
+// the add-back term of later passes, (w_j (a/6pi)) D(r) in the reference's
+// association (bound_fortran.f90:2636-2645)
+template <bool F32>
+__device__ __forceinline__ double addback_term(const float4& qi, const float4& qj, double Gd, float Gf,
+                                               double app, float appf) {
+    if (F32) return pair_term<true>(qi, qj, Gd, Gf, app, appf);
+    const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
+    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+    return __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, app));
 }
 
 __device__ double c2_delta_depth(double r, double ht, double hs) {
@@ -141,6 +182,10 @@ __device__ __forceinline__ bool e_greater(float eb, int b, float ea, int a) {
     return eb > ea || (eb == ea && b < a);
 }
 
+// F32: pair terms and potential sums in f32 (the mixed-precision form);
+// otherwise FP64 exactly as the reference. phi_rec (optional) receives the
+// first pass's potential.
+template <bool F32>
 __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ hoff,
                                                      const float4* __restrict__ qw,
@@ -148,7 +193,8 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
                                                      uint8_t* __restrict__ keep_out,
                                                      int32_t* __restrict__ status,
                                                      int32_t* __restrict__ npb_out,
-                                                     int32_t* __restrict__ err, Params P) {
+                                                     int32_t* __restrict__ err,
+                                                     float* __restrict__ phi_rec, Params P) {
     __shared__ DirectLds S;
     const int32_t h = dlist[blockIdx.x];
     const uint32_t s0 = h ? hoff[h - 1] : 0u;
@@ -163,6 +209,7 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
     if (tid == 0) { S.cnt[0] = 0; S.cnt[1] = 0; }
     __syncthreads();
     const double Gd = P.a / (6.0 * kPi);
+    const float Gf = (float)Gd, appf = (float)P.app;
     const float b32 = (float)P.b;
     int np_bound = n, n_just = 0, st = 5;
     for (int pass = 1; pass <= 20; ++pass) {
@@ -171,16 +218,15 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
             for (int k = tid; k < n; k += kB) {
                 const float4 qk = S.q[k];
                 double acc = 0.0;
+                float accf = 0.0f;
                 for (int j = 0; j < n; ++j) {
                     if (j == k) continue;
                     const float4 qj = S.q[j];
-                    const double dx = (double)(qj.x - qk.x), dy = (double)(qj.y - qk.y),
-                                 dz = (double)(qj.z - qk.z);
-                    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
-                    const double dd = __dmul_rn(Gd, depth_continuous(r, P.app));
-                    acc = __dsub_rn(acc, __dmul_rn((double)qj.w, dd));
+                    const double t = pair_term<F32>(qk, qj, Gd, Gf, P.app, appf);
+                    if (F32) accf = __fsub_rn(accf, (float)t);
+                    else acc = __dsub_rn(acc, t);
                 }
-                S.phid[k] = acc;
+                S.phid[k] = F32 ? (double)accf : acc;
             }
         } else {
             // add back the sources deleted since the last refresh, in deletion order
@@ -188,15 +234,20 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
                 if (!S.keep[i]) continue;
                 const float4 qi = S.q[i];
                 double acc = S.phid[i];
+                float accf = (float)acc;
                 for (int k = 0; k < n_just; ++k) {
                     const int j = S.just[k];
                     const float4 qj = S.q[j];
-                    const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y),
-                                 dz = (double)(qj.z - qi.z);
-                    const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
-                    acc = __dadd_rn(acc, __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, P.app)));
+                    if (F32) {
+                        accf = __fadd_rn(accf, (float)pair_term<true>(qi, qj, Gd, Gf, P.app, appf));
+                    } else {
+                        const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y),
+                                     dz = (double)(qj.z - qi.z);
+                        const double r = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)));
+                        acc = __dadd_rn(acc, __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, P.app)));
+                    }
                 }
-                S.phid[i] = acc;
+                S.phid[i] = F32 ? (double)accf : acc;
             }
         }
         n_just = 0;
@@ -208,6 +259,7 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
             if (np_bound == 1) p = 0.0f;
             if (!isfinite(p)) bad = true;
             S.phi[i] = p;
+            if (pass == 1 && phi_rec) phi_rec[s0 + i] = p;
         }
         if (__syncthreads_or(bad)) { st = -4; break; }
         int deleted = 0;
@@ -291,6 +343,124 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
         status[h] = st < 0 ? 5 : st;
         npb_out[h] = np_bound;
     }
+}
+
+// ---------------------------------------------------------------------------
+// direct summation of halos above the LDS size (up to the context's direct
+// threshold): potentials in global memory, one workgroup per 256-member tile
+// of a halo, partners staged in LDS in ascending order; the removal loop is
+// k_pm_inner's. Pass 1 sums every partner; later passes add back the
+// members deleted since the previous potential, as the reference's direct
+// path does (bound_fortran.f90:2613-2647).
+
+template <bool F32>
+__global__ void __launch_bounds__(kB) k_dl_full(const int2* __restrict__ tiles,
+                                                const uint32_t* __restrict__ hoff,
+                                                const float4* __restrict__ qw,
+                                                double* __restrict__ phid, Params P) {
+    __shared__ float4 sq[kB];
+    const int2 tl = tiles[blockIdx.x];
+    const uint32_t s0 = tl.x ? hoff[tl.x - 1] : 0u;
+    const int n = (int)(hoff[tl.x] - s0);
+    const int i = tl.y + (int)threadIdx.x;
+    const bool valid = i < n;
+    const float4 qi = valid ? qw[s0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double Gd = P.a / (6.0 * kPi);
+    const float Gf = (float)Gd, appf = (float)P.app;
+    double acc = 0.0;
+    float accf = 0.0f;
+    for (int jb = 0; jb < n; jb += kB) {
+        __syncthreads();
+        if (jb + (int)threadIdx.x < n) sq[threadIdx.x] = qw[s0 + jb + threadIdx.x];
+        __syncthreads();
+        const int jn = min(kB, n - jb);
+        if (!valid) continue;
+        for (int jj = 0; jj < jn; ++jj) {
+            if (jb + jj == i) continue;
+            const double t = pair_term<F32>(qi, sq[jj], Gd, Gf, P.app, appf);
+            if (F32) accf = __fsub_rn(accf, (float)t);
+            else acc = __dsub_rn(acc, t);
+        }
+    }
+    if (valid) phid[s0 + i] = F32 ? (double)accf : acc;
+}
+
+// add back the deleted partners dj[dstart[h] .. dstart[h+1]) to kept members
+template <bool F32>
+__global__ void __launch_bounds__(kB) k_dl_update(const int2* __restrict__ tiles,
+                                                  const uint32_t* __restrict__ hoff,
+                                                  const float4* __restrict__ qw,
+                                                  const uint8_t* __restrict__ keep,
+                                                  const uint32_t* __restrict__ dstart,
+                                                  const uint32_t* __restrict__ dj,
+                                                  double* __restrict__ phid, Params P) {
+    __shared__ float4 sq[kB];
+    const int2 tl = tiles[blockIdx.x];
+    const uint32_t s0 = tl.x ? hoff[tl.x - 1] : 0u;
+    const int n = (int)(hoff[tl.x] - s0);
+    const int i = tl.y + (int)threadIdx.x;
+    const bool valid = i < n && keep[s0 + i];
+    const float4 qi = valid ? qw[s0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double Gd = P.a / (6.0 * kPi);
+    const float Gf = (float)Gd, appf = (float)P.app;
+    double acc = valid ? phid[s0 + i] : 0.0;
+    float accf = (float)acc;
+    const uint32_t d0 = dstart[tl.x], d1 = dstart[tl.x + 1];
+    for (uint32_t jb = d0; jb < d1; jb += kB) {
+        __syncthreads();
+        if (jb + threadIdx.x < d1) sq[threadIdx.x] = qw[dj[jb + threadIdx.x]];
+        __syncthreads();
+        const int jn = (int)min((uint32_t)kB, d1 - jb);
+        if (!valid) continue;
+        for (int jj = 0; jj < jn; ++jj) {
+            const double t = addback_term<F32>(qi, sq[jj], Gd, Gf, P.app, appf);
+            if (F32) accf = __fadd_rn(accf, (float)t);
+            else acc = __dadd_rn(acc, t);
+        }
+    }
+    if (valid) phid[s0 + i] = F32 ? (double)accf : acc;
+}
+
+// members of active large-direct halos deleted since the last potential
+__global__ void k_dl_del_flags(const uint32_t* __restrict__ mhalo, const int8_t* __restrict__ kind,
+                               const uint8_t* __restrict__ active, const uint8_t* __restrict__ keep,
+                               const uint8_t* __restrict__ keep_prev, uint32_t nmem,
+                               uint32_t* __restrict__ flag) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    flag[t] = (kind[h] == 1 && active[h] && keep_prev[t] && !keep[t]) ? 1u : 0u;
+}
+
+__global__ void k_dl_del_compact(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ idx,
+                                 const uint32_t* __restrict__ mhalo, uint32_t nmem,
+                                 uint32_t* __restrict__ dkeys, uint32_t* __restrict__ dvals) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !flag[t]) return;
+    dkeys[idx[t]] = mhalo[t];
+    dvals[idx[t]] = t;
+}
+
+// phi of kept members of active large-direct halos (0 for a single member)
+__global__ void k_dl_phi(const uint32_t* __restrict__ mhalo, const int8_t* __restrict__ kind,
+                         const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
+                         const int32_t* __restrict__ npb, const double* __restrict__ phid, uint32_t nmem,
+                         float* __restrict__ phi, int32_t* __restrict__ err) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const uint32_t h = mhalo[t];
+    if (kind[h] != 1 || !active[h]) return;
+    float p = (float)phid[t];
+    if (npb[h] == 1) p = 0.0f;
+    if (!isfinite(p)) atomicCAS(err, 0, 4);
+    phi[t] = p;
+}
+
+// first-pass potential of the members of active halos
+__global__ void k_record_phi(const uint32_t* __restrict__ mhalo, const uint8_t* __restrict__ active,
+                             const float* __restrict__ phi, uint32_t nmem, float* __restrict__ rec) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t < nmem && active[mhalo[t]]) rec[t] = phi[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -1111,6 +1281,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     // bound_fortran.f90:1559-1563, 1586-1590
     if (sc.ngrid != 4) return IERR_INVALID;
     if (!(A.a > 0.0) || !std::isfinite(A.a) || !std::isfinite(A.b)) return IERR_INVALID;
+    if (A.direct_max < kDirectMax || A.direct_max > kDirectLargeMax) return IERR_INVALID;
     Params P{A.app, A.a, A.b, A.L, A.Lh, A.np_min};
 
     float4* qw = C.buf("qw").as<float4>(M);
@@ -1125,27 +1296,36 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     PYFOF_HIP(hipMemcpyAsync(lohi.data(), d_lohi, 4 * lohi.size(), hipMemcpyDeviceToHost, s));
     PYFOF_HIP(hipMemcpyAsync(hoff.data(), A.hoff, 4 * (size_t)nh, hipMemcpyDeviceToHost, s));
     PYFOF_HIP(hipStreamSynchronize(s));
+    auto hsize = [&](uint32_t h) { return (int64_t)hoff[h] - (h ? (int64_t)hoff[h - 1] : 0); };
 
-    // classification in candidate order (bound_fortran.f90:2940-3055)
-    std::vector<int32_t> pm_of_halo(nh, -1), dlist;
+    // classification in candidate order (bound_fortran.f90:2940-3055).
+    // kind: 0 direct in LDS (<= 1000), 1 direct in global memory (up to the
+    // context's threshold; 1000 by default, as the reference), 2 mesh
+    std::vector<int8_t> kind(nh, 0);
+    std::vector<int32_t> pm_of_halo(nh, -1), dsmall, dlarge;
     std::vector<PmHalo> pm;
     std::vector<int32_t> pm_halo;
     for (uint32_t h = 0; h < nh; ++h) {
-        const int64_t n = (int64_t)hoff[h] - (h ? (int64_t)hoff[h - 1] : 0);
+        const int64_t n = hsize(h);
         const float* lh = &lohi[6 * (size_t)h];
         float span = lh[3] - lh[0];
         span = std::max(span, lh[4] - lh[1]);
         span = std::max(span, lh[5] - lh[2]);
         if (!std::isfinite(span)) return IERR_INVALID;
         if (n > 1000 && span > (float)A.s15) return IERR_SPAN;
+        if (n <= A.direct_max) {
+            if (n <= kDirectMax) {
+                dsmall.push_back((int32_t)h);
+            } else {
+                kind[h] = 1;
+                dlarge.push_back((int32_t)h);
+            }
+            continue;
+        }
         int gid;
         double sp;
         if (sel_grid(sc, n, (double)span, gid, sp) != 0) return IERR_INVALID;
-        if (gid == 0) {
-            if (n > kDirectMax) return IERR_INVALID;
-            dlist.push_back((int32_t)h);
-            continue;
-        }
+        if (gid == 0) return IERR_INVALID;  // n > 1000 always selects a grid
         const float sp32 = (float)sp;
         if ((double)(sc.bn[gid - 1] / 2 - 3) * (double)sp32 < (double)span) return IERR_INVALID;
         PmHalo H;
@@ -1153,11 +1333,12 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         H.origin = make_float4(lh[0] - sp32, lh[1] - sp32, lh[2] - sp32, sp32);
         H.cls = gid - 1;
         H.n = sc.bn[gid - 1];
+        kind[h] = 2;
         pm_of_halo[h] = (int32_t)pm.size();
         pm.push_back(H);
         pm_halo.push_back((int32_t)h);
     }
-    res.n_direct = (uint32_t)dlist.size();
+    res.n_direct = (uint32_t)(dsmall.size() + dlarge.size());
 
     int32_t* status = C.buf("bstatus").as<int32_t>(nh);
     int32_t* npb = C.buf("bnpb").as<int32_t>(nh);
@@ -1167,28 +1348,62 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     PYFOF_HIP(hipMemsetAsync(keep, 1, M, s));
     {
         std::vector<int32_t> npb0(nh), st0(nh, 5);
-        for (uint32_t h = 0; h < nh; ++h) npb0[h] = (int32_t)(hoff[h] - (h ? hoff[h - 1] : 0));
+        for (uint32_t h = 0; h < nh; ++h) npb0[h] = (int32_t)hsize(h);
         PYFOF_HIP(hipMemcpyAsync(npb, npb0.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
         PYFOF_HIP(hipMemcpyAsync(status, st0.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
     }
-    // ---- direct-summation halos
-    if (!dlist.empty()) {
-        int32_t* d_dl = C.buf("dlist").as<int32_t>(dlist.size());
-        PYFOF_HIP(hipMemcpyAsync(d_dl, dlist.data(), 4 * dlist.size(), hipMemcpyHostToDevice, s));
-        k_direct_solve<<<(uint32_t)dlist.size(), kB, 0, s>>>(d_dl, A.hoff, qw, vv, keep, status, npb, err, P);
+    int8_t* d_kind = C.buf("bkind").as<int8_t>(nh);
+    PYFOF_HIP(hipMemcpyAsync(d_kind, kind.data(), nh, hipMemcpyHostToDevice, s));
+    float* phi_rec = A.phi_rec;
+
+    auto direct_small = [&](const int32_t* list_dev, uint32_t count) {
+        if (A.direct_f32)
+            k_direct_solve<true><<<count, kB, 0, s>>>(list_dev, A.hoff, qw, vv, keep, status, npb, err, phi_rec, P);
+        else
+            k_direct_solve<false><<<count, kB, 0, s>>>(list_dev, A.hoff, qw, vv, keep, status, npb, err, phi_rec, P);
         PYFOF_LAUNCH_CHECK();
-    }
-    // ---- particle-mesh halos
+    };
+
+    // ---- state of the iterative halos (mesh and large direct)
     const uint32_t npm = (uint32_t)pm.size();
+    const uint32_t nit = npm + (uint32_t)dlarge.size();
     int32_t passes = 0;
+    float* phi = C.buf("phi").as<float>(M);
+    uint32_t* ulist = C.buf("ulist").as<uint32_t>(M);
+    uint32_t* ukey = C.buf("ukey").as<uint32_t>(M);
+    uint8_t* active = C.buf("active").as<uint8_t>(nh);
+    double* mkeep = C.buf("mkeep").as<double>(nh);
+    uint8_t* keep_prev = C.buf("keepc2").as<uint8_t>(M);
+    uint32_t* dflag = C.buf("c2dflag").as<uint32_t>(M);
+    uint32_t* didx = C.buf("c2didx").as<uint32_t>(M);
+    uint32_t* dtot = C.buf("c2dtot").as<uint32_t>(1);
+    uint32_t* dk = C.buf("c2dk").as<uint32_t>(M);
+    uint32_t* dv = C.buf("c2dv").as<uint32_t>(M);
+    uint32_t* dk2 = C.buf("c2dk2").as<uint32_t>(M);
+    uint32_t* dv2 = C.buf("c2dv2").as<uint32_t>(M);
+    int32_t* d_alist = C.buf("alist").as<int32_t>(std::max<uint32_t>(nit, 1));
+    PYFOF_HIP(hipMemsetAsync(active, 0, nh, s));
+
+    // mesh set-up: Green functions, C2 cells, TSC stencils, grids
+    int nclass[4] = {0, 0, 0, 0};
+    const float* Gfull[4] = {nullptr, nullptr, nullptr, nullptr};
+    const FftPlan* plans[4] = {nullptr, nullptr, nullptr, nullptr};
+    unsigned long long* acc[4] = {nullptr, nullptr, nullptr, nullptr};
+    float2* grid[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t* d_slot_halo[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t *d_pmof = nullptr, *d_slots = nullptr;
+    PmHalo* d_pm = nullptr;
+    uint64_t c2cells = 0;
+    const uint32_t *clist = nullptr;
+    uint32_t *cell_of = nullptr, *cstart = nullptr, *dstart = nullptr;
+    int4* first = nullptr;
+    float *wt = nullptr, *selfv = nullptr, *phi_c1 = nullptr, *phi_c2 = nullptr;
+    double* fscale = nullptr;
+    void* tmp = C.buf("c2tmp").get(std::max({radix_temp_bytes(M), scan_temp_bytes(M), scan_temp_bytes(nh + 1)}) + 256);
     if (npm > 0) {
-        // grid slots per class, Green functions and self kernels
-        int nclass[4] = {0, 0, 0, 0};
         for (auto& H : pm) H.slot = nclass[H.cls]++;
         for (int g = 0; g < 4; ++g) res.n_pm[g] = (uint32_t)nclass[g];
         float* selfs = C.buf("selfs").as<float>(4 * 125);
-        const float* Gfull[4] = {nullptr, nullptr, nullptr, nullptr};
-        const FftPlan* plans[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int g = 0; g < 4; ++g) {
             if (!nclass[g]) continue;
             GreenCache& gc = green_for(C, sc.bn[g], s);
@@ -1201,8 +1416,8 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         for (uint32_t k = 0; k < npm; ++k) pm_list[k] = (int32_t)k;
         int32_t* d_pmlist = C.buf("pmlist").as<int32_t>(npm);
         int32_t* d_pmhalo = C.buf("pmhalo").as<int32_t>(npm);
-        PmHalo* d_pm = C.buf("pm").as<PmHalo>(npm);
-        int32_t* d_pmof = C.buf("pmof").as<int32_t>(nh);
+        d_pm = C.buf("pm").as<PmHalo>(npm);
+        d_pmof = C.buf("pmof").as<int32_t>(nh);
         PYFOF_HIP(hipMemcpyAsync(d_pmlist, pm_list.data(), 4 * (size_t)npm, hipMemcpyHostToDevice, s));
         PYFOF_HIP(hipMemcpyAsync(d_pmhalo, pm_halo.data(), 4 * (size_t)npm, hipMemcpyHostToDevice, s));
         PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
@@ -1213,7 +1428,6 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         std::vector<double> ue(6 * (size_t)npm);
         PYFOF_HIP(hipMemcpyAsync(ue.data(), uext, 8 * ue.size(), hipMemcpyDeviceToHost, s));
         PYFOF_HIP(hipStreamSynchronize(s));
-        uint64_t c2cells = 0;
         for (uint32_t k = 0; k < npm; ++k) {
             PmHalo& H = pm[k];
             for (int d = 0; d < 3; ++d) {
@@ -1223,7 +1437,6 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
             }
             H.c2off = (int32_t)c2cells;
             c2cells += (uint64_t)H.nc[0] * H.nc[1] * H.nc[2];
-
         }
         if (c2cells >= 0x7fffffffull) return IERR_TOO_LARGE;
         PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
@@ -1234,12 +1447,13 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         uint32_t* cv2 = C.buf("c2v2").as<uint32_t>(M);
         k_c2_keys<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, M, (uint32_t)c2cells, ck, cv);
         PYFOF_LAUNCH_CHECK();
-        void* tmp = C.buf("c2tmp").get(std::max({radix_temp_bytes(M), scan_temp_bytes(c2cells + 1), scan_temp_bytes(M)}) + 256);
+        tmp = C.buf("c2tmp").get(std::max({radix_temp_bytes(M), scan_temp_bytes(c2cells + 1), scan_temp_bytes(M),
+                                           scan_temp_bytes(nh + 1)}) + 256);
         const bool alt = radix_sort_pairs(ck, cv, ck2, cv2, M, bits_for(c2cells), tmp, s);
         const uint32_t* skeys = alt ? ck2 : ck;
-        const uint32_t* clist = alt ? cv2 : cv;
-        uint32_t* cell_of = C.buf("c2cellof").as<uint32_t>(M);
-        uint32_t* cstart = C.buf("c2start").as<uint32_t>(c2cells + 1);
+        clist = alt ? cv2 : cv;
+        cell_of = C.buf("c2cellof").as<uint32_t>(M);
+        cstart = C.buf("c2start").as<uint32_t>(c2cells + 1);
         k_fill32<<<blocks(c2cells + 1), kB, 0, s>>>(cstart, c2cells + 1, 0u);
         PYFOF_LAUNCH_CHECK();
         k_c2_cells<<<blocks(M), kB, 0, s>>>(skeys, clist, M, (uint32_t)c2cells, cell_of, cstart + 1);
@@ -1247,9 +1461,9 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         inclusive_max_scan_u32(cstart + 1, cstart + 1, c2cells, tmp, s);
 
         // TSC setup
-        int4* first = C.buf("first").as<int4>(M);
-        float* wt = C.buf("wt").as<float>(9 * (size_t)M);
-        float* selfv = C.buf("selfv").as<float>(M);
+        first = C.buf("first").as<int4>(M);
+        wt = C.buf("wt").as<float>(9 * (size_t)M);
+        selfv = C.buf("selfv").as<float>(M);
         k_tsc_setup<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, selfs, M, first, wt, selfv, err);
         PYFOF_LAUNCH_CHECK();
         {
@@ -1260,110 +1474,145 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
             PYFOF_HIP(hipStreamSynchronize(s));
             if (herr) return herr;
         }
-        // grids per class
-        unsigned long long* acc[4] = {nullptr, nullptr, nullptr, nullptr};
-        float2* grid[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int g = 0; g < 4; ++g) {
             if (!nclass[g]) continue;
             const uint64_t gsz = (uint64_t)sc.bn[g] * sc.bn[g] * sc.bn[g] * (uint64_t)nclass[g];
             acc[g] = C.buf("acc" + std::to_string(g)).as<unsigned long long>(gsz);
             grid[g] = C.buf("grid" + std::to_string(g)).as<float2>(gsz);
         }
-        // halo of each grid slot, per class
         std::vector<std::vector<int32_t>> slot_halo(4);
         for (int g = 0; g < 4; ++g) slot_halo[g].assign(nclass[g], 0);
         for (uint32_t k = 0; k < npm; ++k) slot_halo[pm[k].cls][pm[k].slot] = pm_halo[k];
-        int32_t* d_slot_halo[4] = {nullptr, nullptr, nullptr, nullptr};
         for (int g = 0; g < 4; ++g) {
             if (!nclass[g]) continue;
             d_slot_halo[g] = C.buf("slothalo" + std::to_string(g)).as<int32_t>(nclass[g]);
             PYFOF_HIP(hipMemcpyAsync(d_slot_halo[g], slot_halo[g].data(), 4 * (size_t)nclass[g], hipMemcpyHostToDevice, s));
         }
-        double* fscale = C.buf("fscale").as<double>(nh);
-        float* phi_c1 = C.buf("phic1").as<float>(M);
-        float* phi_c2 = C.buf("phic2").as<float>(M);
-        float* phi = C.buf("phi").as<float>(M);
-        uint32_t* ulist = C.buf("ulist").as<uint32_t>(M);
-        uint32_t* ukey = C.buf("ukey").as<uint32_t>(M);
-        uint8_t* active = C.buf("active").as<uint8_t>(nh);
-        double* mkeep = C.buf("mkeep").as<double>(nh);
-        std::vector<uint8_t> act_host(nh, 0);
-        for (uint32_t k = 0; k < npm; ++k) act_host[pm_halo[k]] = 1;
-        PYFOF_HIP(hipMemcpyAsync(active, act_host.data(), nh, hipMemcpyHostToDevice, s));
+        fscale = C.buf("fscale").as<double>(nh);
+        phi_c1 = C.buf("phic1").as<float>(M);
+        phi_c2 = C.buf("phic2").as<float>(M);
+        dstart = C.buf("c2dstart").as<uint32_t>(c2cells + 1);
+        d_slots = C.buf("slots").as<int32_t>(npm);
         k_halo_mass<<<npm, kB, 0, s>>>(qw, A.hoff, d_pmhalo, mkeep, fscale);
         PYFOF_LAUNCH_CHECK();
+    }
+    double* phid = dlarge.empty() ? nullptr : C.buf("phid").as<double>(M);
+    uint32_t* hstart = dlarge.empty() ? nullptr : C.buf("dlstart").as<uint32_t>(nh + 1);
+    int2* d_tiles = nullptr;
 
-        // incremental C2 state: keep flags at the last C2 evaluation, the
-        // deleted members since then listed per cell
-        uint8_t* keep_c2 = C.buf("keepc2").as<uint8_t>(M);
-        uint32_t* dflag = C.buf("c2dflag").as<uint32_t>(M);
-        uint32_t* didx = C.buf("c2didx").as<uint32_t>(M);
-        uint32_t* dtot = C.buf("c2dtot").as<uint32_t>(1);
-        uint32_t* dk = C.buf("c2dk").as<uint32_t>(M);
-        uint32_t* dv = C.buf("c2dv").as<uint32_t>(M);
-        uint32_t* dk2 = C.buf("c2dk2").as<uint32_t>(M);
-        uint32_t* dv2 = C.buf("c2dv2").as<uint32_t>(M);
-        uint32_t* dstart = C.buf("c2dstart").as<uint32_t>(c2cells + 1);
-        std::vector<int32_t> act_list = pm_halo;
-        int32_t* d_alist = C.buf("alist").as<int32_t>(npm);
-        int32_t* d_slots = C.buf("slots").as<int32_t>(npm);
+    // the pass loop over a set of mesh / large-direct halos
+    // (bound_fortran.f90:2613-2864): potential, then the removal loop
+    auto iterate = [&](std::vector<int32_t> act_list) -> int32_t {
+        if (act_list.empty()) return IERR_OK;
+        std::vector<uint8_t> on(nh, 0);
+        for (int32_t h : act_list) on[h] = 1;
+        PYFOF_HIP(hipMemcpyAsync(active, on.data(), nh, hipMemcpyHostToDevice, s));
+        bool any_pm = false, any_dl = false;
+        for (int32_t h : act_list) (kind[h] == 2 ? any_pm : any_dl) = true;
         for (int pass = 1; pass <= 20 && !act_list.empty(); ++pass) {
-            passes = pass;
-            for (int g = 0; g < 4; ++g) {
-                std::vector<int32_t> sl;
+            passes = std::max(passes, pass);
+            // ---- mesh halos: C1 by class, then C2
+            if (any_pm) {
+                for (int g = 0; g < 4; ++g) {
+                    std::vector<int32_t> sl;
+                    for (int32_t h : act_list) {
+                        if (kind[h] != 2) continue;
+                        const PmHalo& H = pm[pm_of_halo[h]];
+                        if (H.cls == g) sl.push_back(H.slot);
+                    }
+                    if (sl.empty()) continue;
+                    const int n = sc.bn[g];
+                    PYFOF_HIP(hipMemcpyAsync(d_slots, sl.data(), 4 * sl.size(), hipMemcpyHostToDevice, s));
+                    // deposit nodes [0, m]^3, m = n/2 + 1 (k_tsc_setup's domain guard)
+                    const uint32_t mc = (uint32_t)(n / 2 + 2);
+                    const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)sl.size());
+                    k_zero_region<<<gz, kB, 0, s>>>(acc[g], d_slots, n, mc);
+                    PYFOF_LAUNCH_CHECK();
+                    k_deposit<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, active,
+                                                       fscale, g, M, acc[g]);
+                    PYFOF_LAUNCH_CHECK();
+                    fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
+                               Gfull[g], s);
+                    k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
+                                                      mkeep, grid[g], g, M, A.a, phi_c1);
+                    PYFOF_LAUNCH_CHECK();
+                }
+                if (pass == 1) {
+                    k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
+                                                       clist, M, A.a, A.app, phi_c2);
+                    PYFOF_LAUNCH_CHECK();
+                } else {
+                    k_c2_del_flags<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, active, keep, keep_prev, M, dflag);
+                    PYFOF_LAUNCH_CHECK();
+                    exclusive_scan_u32(dflag, didx, M, dtot, tmp, s);
+                    uint32_t nd = 0;
+                    PYFOF_HIP(hipMemcpyAsync(&nd, dtot, 4, hipMemcpyDeviceToHost, s));
+                    PYFOF_HIP(hipStreamSynchronize(s));
+                    if (nd > 0) {
+                        k_c2_del_compact<<<blocks(M), kB, 0, s>>>(dflag, didx, cell_of, M, dk, dv);
+                        PYFOF_LAUNCH_CHECK();
+                        const bool dalt = radix_sort_pairs(dk, dv, dk2, dv2, nd, bits_for(c2cells), tmp, s);
+                        const uint32_t* dks = dalt ? dk2 : dk;
+                        const uint32_t* dvs = dalt ? dv2 : dv;
+                        k_fill32<<<blocks(c2cells + 1), kB, 0, s>>>(dstart, c2cells + 1, 0u);
+                        PYFOF_LAUNCH_CHECK();
+                        k_run_ends<<<blocks(nd), kB, 0, s>>>(dks, nd, dstart + 1);
+                        PYFOF_LAUNCH_CHECK();
+                        inclusive_max_scan_u32(dstart + 1, dstart + 1, c2cells, tmp, s);
+                        k_c2_update<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of,
+                                                             dstart, dvs, clist, M, A.a, A.app, phi_c2);
+                        PYFOF_LAUNCH_CHECK();
+                    }
+                }
+                k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, keep, active, npb, phi_c1, phi_c2, M,
+                                                       phi, err);
+                PYFOF_LAUNCH_CHECK();
+            }
+            // ---- large direct halos
+            if (any_dl) {
+                std::vector<int2> tiles;
                 for (int32_t h : act_list) {
-                    const PmHalo& H = pm[pm_of_halo[h]];
-                    if (H.cls == g) sl.push_back(H.slot);
+                    if (kind[h] != 1) continue;
+                    for (int64_t i0 = 0; i0 < hsize(h); i0 += kB) tiles.push_back(make_int2(h, (int)i0));
                 }
-                if (sl.empty()) continue;
-                const int n = sc.bn[g];
-                PYFOF_HIP(hipMemcpyAsync(d_slots, sl.data(), 4 * sl.size(), hipMemcpyHostToDevice, s));
-                // deposit nodes [0, m]^3, m = n/2 + 1 (k_tsc_setup's domain guard)
-                const uint32_t mc = (uint32_t)(n / 2 + 2);
-                const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)sl.size());
-                k_zero_region<<<gz, kB, 0, s>>>(acc[g], d_slots, n, mc);
-                PYFOF_LAUNCH_CHECK();
-                k_deposit<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, active, fscale,
-                                                   g, M, acc[g]);
-                PYFOF_LAUNCH_CHECK();
-                fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
-                           Gfull[g], s);
-                k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
-                                                  mkeep, grid[g], g, M, A.a, phi_c1);
+                d_tiles = C.buf("dltiles").as<int2>(tiles.size());
+                PYFOF_HIP(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
+                if (pass == 1) {
+                    if (A.direct_f32) k_dl_full<true><<<(uint32_t)tiles.size(), kB, 0, s>>>(d_tiles, A.hoff, qw, phid, P);
+                    else k_dl_full<false><<<(uint32_t)tiles.size(), kB, 0, s>>>(d_tiles, A.hoff, qw, phid, P);
+                    PYFOF_LAUNCH_CHECK();
+                } else {
+                    k_dl_del_flags<<<blocks(M), kB, 0, s>>>(A.mhalo, d_kind, active, keep, keep_prev, M, dflag);
+                    PYFOF_LAUNCH_CHECK();
+                    exclusive_scan_u32(dflag, didx, M, dtot, tmp, s);
+                    uint32_t nd = 0;
+                    PYFOF_HIP(hipMemcpyAsync(&nd, dtot, 4, hipMemcpyDeviceToHost, s));
+                    PYFOF_HIP(hipStreamSynchronize(s));
+                    if (nd > 0) {
+                        // members are grouped by halo, so the list is too
+                        k_dl_del_compact<<<blocks(M), kB, 0, s>>>(dflag, didx, A.mhalo, M, dk, dv);
+                        PYFOF_LAUNCH_CHECK();
+                        k_fill32<<<blocks(nh + 1), kB, 0, s>>>(hstart, nh + 1, 0u);
+                        PYFOF_LAUNCH_CHECK();
+                        k_run_ends<<<blocks(nd), kB, 0, s>>>(dk, nd, hstart + 1);
+                        PYFOF_LAUNCH_CHECK();
+                        inclusive_max_scan_u32(hstart + 1, hstart + 1, nh, tmp, s);
+                        if (A.direct_f32)
+                            k_dl_update<true><<<(uint32_t)tiles.size(), kB, 0, s>>>(d_tiles, A.hoff, qw, keep, hstart, dv, phid, P);
+                        else
+                            k_dl_update<false><<<(uint32_t)tiles.size(), kB, 0, s>>>(d_tiles, A.hoff, qw, keep, hstart, dv, phid, P);
+                        PYFOF_LAUNCH_CHECK();
+                    }
+                }
+                k_dl_phi<<<blocks(M), kB, 0, s>>>(A.mhalo, d_kind, keep, active, npb, phid, M, phi, err);
                 PYFOF_LAUNCH_CHECK();
             }
-            if (pass == 1) {
-                k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
-                                                   clist, M, A.a, A.app, phi_c2);
+            k_copy_u8<<<blocks(M), kB, 0, s>>>(keep, keep_prev, M);
+            PYFOF_LAUNCH_CHECK();
+            if (pass == 1 && phi_rec) {
+                k_record_phi<<<blocks(M), kB, 0, s>>>(A.mhalo, active, phi, M, phi_rec);
                 PYFOF_LAUNCH_CHECK();
-            } else {
-                k_c2_del_flags<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, active, keep, keep_c2, M, dflag);
-                PYFOF_LAUNCH_CHECK();
-                exclusive_scan_u32(dflag, didx, M, dtot, tmp, s);
-                uint32_t nd = 0;
-                PYFOF_HIP(hipMemcpyAsync(&nd, dtot, 4, hipMemcpyDeviceToHost, s));
-                PYFOF_HIP(hipStreamSynchronize(s));
-                if (nd > 0) {
-                    k_c2_del_compact<<<blocks(M), kB, 0, s>>>(dflag, didx, cell_of, M, dk, dv);
-                    PYFOF_LAUNCH_CHECK();
-                    const bool dalt = radix_sort_pairs(dk, dv, dk2, dv2, nd, bits_for(c2cells), tmp, s);
-                    const uint32_t* dks = dalt ? dk2 : dk;
-                    const uint32_t* dvs = dalt ? dv2 : dv;
-                    k_fill32<<<blocks(c2cells + 1), kB, 0, s>>>(dstart, c2cells + 1, 0u);
-                    PYFOF_LAUNCH_CHECK();
-                    k_run_ends<<<blocks(nd), kB, 0, s>>>(dks, nd, dstart + 1);
-                    PYFOF_LAUNCH_CHECK();
-                    inclusive_max_scan_u32(dstart + 1, dstart + 1, c2cells, tmp, s);
-                    k_c2_update<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, dstart,
-                                                         dvs, clist, M, A.a, A.app, phi_c2);
-                    PYFOF_LAUNCH_CHECK();
-                }
             }
-            k_copy_u8<<<blocks(M), kB, 0, s>>>(keep, keep_c2, M);
-            PYFOF_LAUNCH_CHECK();
-            k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, keep, active, npb, phi_c1, phi_c2, M,
-                                                   phi, err);
-            PYFOF_LAUNCH_CHECK();
             PYFOF_HIP(hipMemcpyAsync(d_alist, act_list.data(), 4 * act_list.size(), hipMemcpyHostToDevice, s));
             k_pm_inner<<<(uint32_t)act_list.size(), kPB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, ulist, ukey, keep,
                                                                 status, npb, active, mkeep, pass == 20,
@@ -1371,12 +1620,37 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
             PYFOF_LAUNCH_CHECK();
             int32_t herr = 0;
             PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
-            PYFOF_HIP(hipMemcpyAsync(act_host.data(), active, nh, hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipMemcpyAsync(on.data(), active, nh, hipMemcpyDeviceToHost, s));
             PYFOF_HIP(hipStreamSynchronize(s));
-            if (herr) break;
+            if (herr) return herr;
             std::vector<int32_t> next;
-            for (int32_t h : act_list) if (act_host[h]) next.push_back(h);
+            for (int32_t h : act_list) if (on[h]) next.push_back(h);
             act_list.swap(next);
+        }
+        return IERR_OK;
+    };
+
+    int32_t* d_dl = C.buf("dlist").as<int32_t>(std::max<size_t>(dsmall.size(), 1));
+    if (!dsmall.empty())
+        PYFOF_HIP(hipMemcpyAsync(d_dl, dsmall.data(), 4 * dsmall.size(), hipMemcpyHostToDevice, s));
+    if (A.batched) {
+        // every halo of a kind in one launch per step
+        if (!dsmall.empty()) direct_small(d_dl, (uint32_t)dsmall.size());
+        std::vector<int32_t> it;
+        for (uint32_t h = 0; h < nh; ++h) if (kind[h] != 0) it.push_back((int32_t)h);
+        const int32_t e = iterate(it);
+        if (e != IERR_OK) return e;
+    } else {
+        // the plain form: one halo at a time, in candidate order
+        size_t k = 0;
+        for (uint32_t h = 0; h < nh; ++h) {
+            if (kind[h] == 0) {
+                direct_small(d_dl + k, 1);
+                ++k;
+            } else {
+                const int32_t e = iterate(std::vector<int32_t>{(int32_t)h});
+                if (e != IERR_OK) return e;
+            }
         }
     }
     res.passes = passes;
@@ -1403,6 +1677,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         else if (st_h[h] == 1) ++res.n_too_few;
         else ++res.n_max_iter;
     }
+    res.n_direct_large = (uint32_t)dlarge.size();
     return IERR_OK;
 }
 

@@ -25,6 +25,14 @@ struct BoundArgs {
     // outputs rewritten for bound members only
     int32_t* in_halo;
     int32_t* halo_id;
+    // solver form: halos of up to direct_max members use direct summation
+    // (1000 = the reference), in FP64 (reference) or f32 pair terms
+    // (direct_f32); all halos of a kind per launch (batched) or one halo at
+    // a time; phi_rec (optional, nmem floats) gets the first-pass potential
+    int32_t direct_max = 1000;
+    int32_t direct_f32 = 0;
+    int32_t batched = 1;
+    float* phi_rec = nullptr;
 };
 
 using BoundResult = BoundStats;

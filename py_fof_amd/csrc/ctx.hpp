@@ -50,6 +50,7 @@ struct BoundStats {
     uint32_t n_ok = 0, n_too_few = 0, n_max_iter = 0;
     uint32_t n_direct = 0, n_pm[4] = {0, 0, 0, 0};
     int32_t passes = 0;
+    uint32_t n_direct_large = 0;
 };
 
 // Green function of one isolated grid size (depends on n only), cached.
@@ -76,6 +77,13 @@ struct Ctx {
     // last call's sizes and catalog
     uint64_t last_n = 0, last_ncell = 0, last_ndense = 0;
     bool want_catalog = false;
+    // unbinding form (PYFOF_OPT_DIRECT_MAX / _DIRECT_FP32 / _BATCHED / _RECORD_PHI)
+    int32_t direct_max = 1000;
+    int32_t direct_f32 = 0;
+    int32_t batched = 1;
+    bool record_phi = false;
+    uint64_t rec_n = 0;  // recorded first-pass potentials of the last call
+    DevBuf rec_phi, rec_idx;
     BoundStats last_bound;  // build the fof_with_info catalog on every call
     uint32_t cat_nh = 0;
     uint64_t cat_nmem = 0;
@@ -103,6 +111,8 @@ struct Ctx {
                           &am_h, &am, &pin})
             b->release();
         for (auto& kv : named) kv.second.release();
+        rec_phi.release();
+        rec_idx.release();
         for (auto& kv : greens) {
             kv.second->G.release();
             kv.second->tw.release();

@@ -1309,6 +1309,16 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             B.xp = xp; B.vp = vp; B.mp = mp; B.N = N;
             B.nh = nh; B.nmem = cand.nmem; B.hoff = cand.hoff; B.members = cand.members;
             B.mhalo = cand.mkeys; B.anchor = anchor; B.in_halo = o_in; B.halo_id = o_id;
+            B.direct_max = C.direct_max;
+            B.direct_f32 = C.direct_f32;
+            B.batched = C.batched;
+            C.rec_n = 0;
+            if (C.record_phi) {
+                B.phi_rec = C.rec_phi.as<float>(std::max<uint32_t>(cand.nmem, 1));
+                uint32_t* ri = C.rec_idx.as<uint32_t>(std::max<uint32_t>(cand.nmem, 1));
+                PYFOF_HIP(hipMemcpyAsync(ri, cand.members, 4 * (size_t)cand.nmem, hipMemcpyDeviceToDevice, s));
+                C.rec_n = cand.nmem;
+            }
             uint8_t* bkeep = C.buf("bkeep_out").as<uint8_t>(std::max<uint32_t>(cand.nmem, 1));
             BoundResult br;
             const int32_t berr = run_bound(C, B, bkeep, br);
@@ -1540,25 +1550,60 @@ int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, i
 
 int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
     if (!ctx) return IERR_INVALID;
-    if (option == PYFOF_OPT_CATALOG) {
-        ctx->c->want_catalog = value != 0;
-        return IERR_OK;
+    Ctx& C = *ctx->c;
+    switch (option) {
+        case PYFOF_OPT_CATALOG: C.want_catalog = value != 0; return IERR_OK;
+        case PYFOF_OPT_DIRECT_MAX:
+            if (value < 1000 || value > 65536) {
+                set_error("direct-summation threshold must be in [1000, 65536]");
+                return IERR_INVALID;
+            }
+            C.direct_max = (int32_t)value;
+            return IERR_OK;
+        case PYFOF_OPT_DIRECT_FP32: C.direct_f32 = value != 0; return IERR_OK;
+        case PYFOF_OPT_BATCHED: C.batched = value != 0; return IERR_OK;
+        case PYFOF_OPT_RECORD_PHI: C.record_phi = value != 0; return IERR_OK;
+        default: break;
     }
     set_error("unknown option");
     return IERR_INVALID;
+}
+
+int64_t pyfof_ctx_potential(pyfof_ctx* ctx, float* phi, int32_t* index, int64_t cap, int32_t mem_kind) {
+    if (!ctx) return -1;
+    Ctx& C = *ctx->c;
+    const uint64_t n = C.rec_n;
+    if (!phi && !index) return (int64_t)n;
+    if ((uint64_t)cap < n) {
+        set_error("capacity below the number of recorded members");
+        return -1;
+    }
+    if (n == 0) return 0;
+    try {
+        PYFOF_HIP(hipSetDevice(C.device));
+        const hipMemcpyKind k = mem_kind == PYFOF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (phi) PYFOF_HIP(hipMemcpyAsync(phi, C.rec_phi.as<float>(n), 4 * n, k, C.stream));
+        if (index) PYFOF_HIP(hipMemcpyAsync(index, C.rec_idx.as<uint32_t>(n), 4 * n, k, C.stream));
+        PYFOF_HIP(hipStreamSynchronize(C.stream));
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return -1;
+    }
+    return (int64_t)n;
 }
 
 int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n) {
     if (!ctx) return 0;
     const Ctx& C = *ctx->c;
     const BoundStats& b = C.last_bound;
-    const int64_t v[15] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
+    const int64_t v[16] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
                            (int64_t)C.cat_nh, (int64_t)C.cat_nmem, (int64_t)b.nvalid,
                            (int64_t)b.n_ok, (int64_t)b.n_too_few, (int64_t)b.n_max_iter,
                            (int64_t)b.n_direct, (int64_t)b.n_pm[0], (int64_t)b.n_pm[1],
-                           (int64_t)b.n_pm[2], (int64_t)b.n_pm[3], (int64_t)b.passes};
+                           (int64_t)b.n_pm[2], (int64_t)b.n_pm[3], (int64_t)b.passes,
+                           (int64_t)b.n_direct_large};
     int32_t w = 0;
-    for (; w < n && w < 15; ++w) out[w] = v[w];
+    for (; w < n && w < 16; ++w) out[w] = v[w];
     return w;
 }
 
