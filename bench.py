@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--bound-halos", type=int, default=20_000)
     ap.add_argument("--bound-steps", type=int, default=3)
     ap.add_argument("--bound-cpu-halos", type=int, default=2_000)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the plain / f32 / tuned unbinding forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--serve-pool", type=int, default=32, help="distinct job inputs, reused")
@@ -113,44 +115,103 @@ def bench_bound(args, torch, _native, ctx):
     hid = torch.empty(n, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
+    def form(direct_max, f32, batched, record=False):
+        ctx.set_option(_native.OPT_DIRECT_MAX, direct_max)
+        ctx.set_option(_native.OPT_DIRECT_FP32, int(f32))
+        ctx.set_option(_native.OPT_BATCHED, int(batched))
+        ctx.set_option(_native.OPT_RECORD_PHI, int(record))
+
     def step():
         err = ctx.fof(x_d, v_d, None, np_halo, L, b, 10, 1, 0, app, 1.0, 0.0, s15, 0, n,
                       256, 512.0, in_h, hid, None, None, _native.MEM_DEVICE)
         if err:
             raise RuntimeError(f"pyfof ierr={err}: {_native.last_error()}")
 
-    step()
-    log(args, f"unbinding warmup done: phases ms {[round(v, 2) for v in ctx.timings()]}")
-    ref = in_h.clone()
-    torch.cuda.synchronize()
-    phases = np.zeros(6)
-    t0 = time.perf_counter()
-    for _ in range(args.bound_steps):
-        step()
-        phases += np.array(ctx.timings()[:6])
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.bound_steps
-    phases /= args.bound_steps
-    st = ctx.stats()
+    same = []
+
+    def timed(steps):
+        step()  # warm-up of this form
+        first = in_h.clone()
+        torch.cuda.synchronize()
+        ph = np.zeros(6)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            ph += np.array(ctx.timings()[:6])
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        same.append(bool(torch.equal(first, in_h)))
+        return dt, ph / steps, ctx.stats()
+
+    # the reference's form: crossover 1000, FP64 direct sums, batched
+    form(1000, False, True)
+    dt, phases, st = timed(args.bound_steps)
+    ref_in = in_h.clone()
+    log(args, f"unbinding (reference form): {dt * 1e3:.1f} ms/call")
+    ncand = st["n_ok"] + st["n_too_few"] + st["n_max_iter"]
     out = {
         "workload": f"{args.bound_halos} isolated NFW halos, n ~ n^-1.9 on [20, 1e6], seed 3, "
                     f"{n} particles, 10% interlopers, cube units, b {b}, softening {app}, "
                     f"s_15 {s15:g}, bound_check True",
         "bound_halos_per_s": round(st["nvalid"] / dt, 1),
-        "candidates_per_s": round((st["n_ok"] + st["n_too_few"] + st["n_max_iter"]) / dt, 1),
+        "candidates_per_s": round(ncand / dt, 1),
         "Mparticles_per_s": round(n / dt / 1e6, 3),
         "ms_per_call": round(dt * 1e3, 3),
-        "phases_ms": {k: round(float(v), 3) for k, v in zip(
-            PHASES, phases)},
-        "candidates": st["n_ok"] + st["n_too_few"] + st["n_max_iter"],
+        "phases_ms": {k: round(float(v), 3) for k, v in zip(PHASES, phases)},
+        "candidates": ncand,
         "bound_halos": st["nvalid"], "status_ok": st["n_ok"], "status_too_few": st["n_too_few"],
         "status_max_iter": st["n_max_iter"], "direct_sum_halos": st["n_direct"],
         "pm_halos": {"48": st["n_pm48"], "64": st["n_pm64"], "128": st["n_pm128"],
                      "256": st["n_pm256"]},
         "pm_passes": st["passes"],
-        "form": "direct sum: FP64 per-halo workgroup; PM: f32 batched per grid class",
-        "deterministic": bool(torch.equal(ref, in_h)),
+        "form": "reference: direct sum (<= 1000 members) FP64, PM f32, batched",
     }
+    if not args.no_forms:
+        from py_fof_amd.tune import tune_direct_max
+        # plain per-halo FP64 form (one candidate at a time)
+        form(1000, False, False)
+        dt_plain, _, st_p = timed(1)
+        log(args, f"unbinding (plain FP64 per halo): {dt_plain * 1e3:.1f} ms/call")
+        # autotuned crossover for the f32 form, then the batched f32 form
+        dmax, table = tune_direct_max(f32=True)
+        log(args, f"unbinding: tuned direct-summation threshold {dmax}")
+        form(dmax, True, True)
+        dt_mix, _, st_m = timed(args.bound_steps)
+        mix_in = in_h.clone()
+        log(args, f"unbinding (batched f32, crossover {dmax}): {dt_mix * 1e3:.1f} ms/call")
+        # potential error of the f32 form against FP64 at the same crossover
+        form(dmax, False, True, record=True)
+        step()
+        phi64, idx64 = ctx.potential()
+        form(dmax, True, True, record=True)
+        step()
+        phi32, idx32 = ctx.potential()
+        form(1000, False, True)
+        same_members = bool(np.array_equal(idx64, idx32))
+        rel = np.abs(phi32.astype(np.float64) - phi64) / np.abs(phi64.astype(np.float64))
+        diff = int((mix_in != ref_in).sum().item())
+        out["forms"] = {
+            "plain_fp64_per_halo": {"bound_halos_per_s": round(st_p["nvalid"] / dt_plain, 1),
+                                    "ms_per_call": round(dt_plain * 1e3, 3),
+                                    "crossover": 1000},
+            "batched_fp64": {"bound_halos_per_s": out["bound_halos_per_s"],
+                             "ms_per_call": out["ms_per_call"], "crossover": 1000},
+            "batched_f32_tuned": {"bound_halos_per_s": round(st_m["nvalid"] / dt_mix, 1),
+                                  "ms_per_call": round(dt_mix * 1e3, 3), "crossover": dmax,
+                                  "direct_sum_halos": st_m["n_direct"],
+                                  "direct_sum_above_1000": st_m["n_direct_large"]},
+            "speedup_batched_f32_vs_plain_fp64": round(dt_plain / dt_mix, 2),
+            "potential_error_f32_vs_fp64": {
+                "members": int(len(phi32)), "same_members": same_members,
+                "max_rel": float(f"{rel.max():.3e}"),
+                "rms_rel": float(f"{np.sqrt(np.mean(rel ** 2)):.3e}"),
+                "note": f"first-pass potential of every candidate member, crossover {dmax}; "
+                        "PM members are identical in both forms"},
+            "status_differs_vs_reference_form": {
+                "particles": diff, "fraction": float(f"{diff / n:.3e}")},
+            "tuning": table,
+        }
+    out["deterministic"] = all(same)  # repeated calls of each form gave identical in_halo
     if not args.no_cpu and args.bound_cpu_halos > 0:
         from oracle import oracle_fof_bound
         k = min(args.bound_cpu_halos, len(np_halo))
