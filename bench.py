@@ -96,11 +96,83 @@ def bound_workload(n_halos, seed=3):
     np_halo = np.array([x.shape[1] for x in xs], np.int32)
     xp = np.asfortranarray(np.hstack(xs))
     vp = np.asfortranarray(np.hstack(vs))
-    # s_15 covers the widest region (interlopers reach 2 r_vir), as the
-    # caller would set it through box / ng_global
-    span = max(float((x.max(1) - x.min(1)).max()) for x in xs)
-    s15 = max(17.152, float(np.ceil(1.02 * span)))
+    # the reference's default s_15 for a 512 cube with box 300, h0 0.67; it
+    # covers every FoF candidate of this set (largest span 16.6 cells)
+    s15 = 15.0 * 0.67 * 512.0 / 300.0
     return xp, vp, np_halo, s15
+
+
+def physical_workload(n_halos, seed=4):
+    """The isolated-halo generator in physical units: positions of the cube
+    generator (grid of n_sim^(1/3) = 1024 cells over box 300 Mpc/h) times
+    dx, velocities from the virial sigma with the cube-unit G at a = 2/3
+    (z = 0.5) converted to km/s, masses log-normal (0.3 dex) around
+    2e9 Msun/h."""
+    from py_fof_amd.synth import isolated_halos
+    box, ng, z, om = 300.0, 1024.0, 0.5, 0.3
+    a = 1.0 / (1.0 + z)
+    xs, vs = isolated_halos(n_halos=n_halos, seed=seed, L=ng, a_val=a)
+    dx = box / ng
+    v_scale = (a * ng) / (150.0 * np.sqrt(om) * box)
+    xp = np.asfortranarray(np.hstack(xs) * dx)
+    vp = np.asfortranarray(np.hstack(vs) / v_scale)
+    rng = np.random.default_rng(seed + 100)
+    mp = 2e9 * 10.0 ** rng.normal(0.0, 0.3, xp.shape[1])
+    np_halo = np.array([x.shape[1] for x in xs], np.int32)
+    return [xp[:, o:o + k] for o, k in zip(np.concatenate([[0], np.cumsum(np_halo)[:-1]]), np_halo)], \
+        [vp[:, o:o + k] for o, k in zip(np.concatenate([[0], np.cumsum(np_halo)[:-1]]), np_halo)], \
+        mp, dict(L_Box=box, linking_length=0.2 * dx, np_halo_min=10, bound_check=True,
+                 units="physical", box=box, n_sim=int(ng ** 3), z=z, omega_m=om,
+                 include_hubble_flow=True, softening=0.01)
+
+
+def bench_physical(args, _native, dmax):
+    """Physical-units unbinding: the reference form against the batched f32
+    form at the tuned crossover (host inputs through the Python front end)."""
+    import py_fof_amd as pf
+    log(args, f"physical units: generating {args.bound_halos} halos")
+    xs, vs, mp, kw = physical_workload(args.bound_halos)
+    n = int(sum(x.shape[1] for x in xs))
+    ctx = _native.Context(0)
+    res = {}
+    try:
+        for name, dm, f32 in (("reference_fp64", 1000, False), ("batched_f32_tuned", dmax, True)):
+            ctx.set_option(_native.OPT_DIRECT_MAX, dm)
+            ctx.set_option(_native.OPT_DIRECT_FP32, int(f32))
+            ctx.set_option(_native.OPT_RECORD_PHI, 0)
+            pf._fof_core(xs, vs, mp_list=mp, _ctx=ctx, **kw)  # warm-up
+            t0 = time.perf_counter()
+            inh, _, _ = pf._fof_core(xs, vs, mp_list=mp, _ctx=ctx, **kw)
+            dt = time.perf_counter() - t0
+            st = ctx.stats()
+            res[name] = {"bound_halos_per_s": round(st["nvalid"] / dt, 1),
+                         "ms_per_call": round(dt * 1e3, 3), "crossover": dm,
+                         "unbinding_ms": round(ctx.timings()[4], 3), "in_halo": inh}
+        # potential error of the f32 form against FP64 at the same crossover
+        phis = []
+        for f32 in (False, True):
+            ctx.set_option(_native.OPT_DIRECT_MAX, dmax)
+            ctx.set_option(_native.OPT_DIRECT_FP32, int(f32))
+            ctx.set_option(_native.OPT_RECORD_PHI, 1)
+            pf._fof_core(xs, vs, mp_list=mp, _ctx=ctx, **kw)
+            phis.append(ctx.potential())
+    finally:
+        ctx.close()
+    (p64, i64), (p32, i32) = phis
+    rel = np.abs(p32.astype(np.float64) - p64) / np.abs(p64.astype(np.float64))
+    diff = int(np.sum(res["reference_fp64"].pop("in_halo") != res["batched_f32_tuned"].pop("in_halo")))
+    res.update({
+        "workload": f"{args.bound_halos} isolated NFW halos (generator of the cube-unit set, seed 4), "
+                    f"{n} particles, box 300 Mpc/h, n_sim 1024^3, z 0.5, include_hubble_flow, "
+                    "masses log-normal 0.3 dex around 2e9 Msun/h, softening 0.01 Mpc/h, host inputs",
+        "speedup": round(res["reference_fp64"]["ms_per_call"] / res["batched_f32_tuned"]["ms_per_call"], 2),
+        "potential_error_f32_vs_fp64": {"members": int(len(p32)),
+                                        "same_members": bool(np.array_equal(i64, i32)),
+                                        "max_rel": float(f"{rel.max():.3e}"),
+                                        "rms_rel": float(f"{np.sqrt(np.mean(rel ** 2)):.3e}")},
+        "status_differs_vs_reference_form": {"particles": diff, "fraction": float(f"{diff / n:.3e}")},
+    })
+    return res
 
 
 def bench_bound(args, torch, _native, ctx):
@@ -152,7 +224,7 @@ def bench_bound(args, torch, _native, ctx):
     out = {
         "workload": f"{args.bound_halos} isolated NFW halos, n ~ n^-1.9 on [20, 1e6], seed 3, "
                     f"{n} particles, 10% interlopers, cube units, b {b}, softening {app}, "
-                    f"s_15 {s15:g}, bound_check True",
+                    f"s_15 {s15:.3f}, bound_check True",
         "bound_halos_per_s": round(st["nvalid"] / dt, 1),
         "candidates_per_s": round(ncand / dt, 1),
         "Mparticles_per_s": round(n / dt / 1e6, 3),
@@ -211,6 +283,7 @@ def bench_bound(args, torch, _native, ctx):
                 "particles": diff, "fraction": float(f"{diff / n:.3e}")},
             "tuning": table,
         }
+        out["physical_units"] = bench_physical(args, _native, dmax)
     out["deterministic"] = all(same)  # repeated calls of each form gave identical in_halo
     if not args.no_cpu and args.bound_cpu_halos > 0:
         from oracle import oracle_fof_bound

@@ -223,10 +223,17 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
                     if (j == k) continue;
                     const float4 qj = S.q[j];
                     const double t = pair_term<F32>(qk, qj, Gd, Gf, P.app, appf);
-                    if (F32) accf = __fsub_rn(accf, (float)t);
-                    else acc = __dsub_rn(acc, t);
+                    if (F32) {
+                        accf = __fsub_rn(accf, (float)t);
+                        if ((j & 255) == 255) {  // f32 within 256 partners, f64 across
+                            acc = __dadd_rn(acc, (double)accf);
+                            accf = 0.0f;
+                        }
+                    } else {
+                        acc = __dsub_rn(acc, t);
+                    }
                 }
-                S.phid[k] = F32 ? (double)accf : acc;
+                S.phid[k] = F32 ? __dadd_rn(acc, (double)accf) : acc;
             }
         } else {
             // add back the sources deleted since the last refresh, in deletion order
@@ -234,12 +241,11 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
                 if (!S.keep[i]) continue;
                 const float4 qi = S.q[i];
                 double acc = S.phid[i];
-                float accf = (float)acc;
                 for (int k = 0; k < n_just; ++k) {
                     const int j = S.just[k];
                     const float4 qj = S.q[j];
-                    if (F32) {
-                        accf = __fadd_rn(accf, (float)pair_term<true>(qi, qj, Gd, Gf, P.app, appf));
+                    if (F32) {  // f32 term, f64 sum
+                        acc = __dadd_rn(acc, (double)(float)pair_term<true>(qi, qj, Gd, Gf, P.app, appf));
                     } else {
                         const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y),
                                      dz = (double)(qj.z - qi.z);
@@ -247,7 +253,7 @@ __global__ void __launch_bounds__(kB) k_direct_solve(const int32_t* __restrict__
                         acc = __dadd_rn(acc, __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, P.app)));
                     }
                 }
-                S.phid[i] = F32 ? (double)accf : acc;
+                S.phid[i] = acc;
             }
         }
         n_just = 0;
@@ -368,21 +374,23 @@ __global__ void __launch_bounds__(kB) k_dl_full(const int2* __restrict__ tiles,
     const double Gd = P.a / (6.0 * kPi);
     const float Gf = (float)Gd, appf = (float)P.app;
     double acc = 0.0;
-    float accf = 0.0f;
+    float accf;
     for (int jb = 0; jb < n; jb += kB) {
         __syncthreads();
         if (jb + (int)threadIdx.x < n) sq[threadIdx.x] = qw[s0 + jb + threadIdx.x];
         __syncthreads();
         const int jn = min(kB, n - jb);
         if (!valid) continue;
+        accf = 0.0f;
         for (int jj = 0; jj < jn; ++jj) {
             if (jb + jj == i) continue;
             const double t = pair_term<F32>(qi, sq[jj], Gd, Gf, P.app, appf);
             if (F32) accf = __fsub_rn(accf, (float)t);
             else acc = __dsub_rn(acc, t);
         }
+        if (F32) acc = __dadd_rn(acc, (double)accf);  // f32 within a tile, f64 across tiles
     }
-    if (valid) phid[s0 + i] = F32 ? (double)accf : acc;
+    if (valid) phid[s0 + i] = acc;
 }
 
 // add back the deleted partners dj[dstart[h] .. dstart[h+1]) to kept members
@@ -404,7 +412,6 @@ __global__ void __launch_bounds__(kB) k_dl_update(const int2* __restrict__ tiles
     const double Gd = P.a / (6.0 * kPi);
     const float Gf = (float)Gd, appf = (float)P.app;
     double acc = valid ? phid[s0 + i] : 0.0;
-    float accf = (float)acc;
     const uint32_t d0 = dstart[tl.x], d1 = dstart[tl.x + 1];
     for (uint32_t jb = d0; jb < d1; jb += kB) {
         __syncthreads();
@@ -412,13 +419,15 @@ __global__ void __launch_bounds__(kB) k_dl_update(const int2* __restrict__ tiles
         __syncthreads();
         const int jn = (int)min((uint32_t)kB, d1 - jb);
         if (!valid) continue;
+        float accf = 0.0f;
         for (int jj = 0; jj < jn; ++jj) {
             const double t = addback_term<F32>(qi, sq[jj], Gd, Gf, P.app, appf);
             if (F32) accf = __fadd_rn(accf, (float)t);
             else acc = __dadd_rn(acc, t);
         }
+        if (F32) acc = __dadd_rn(acc, (double)accf);
     }
-    if (valid) phid[s0 + i] = F32 ? (double)accf : acc;
+    if (valid) phid[s0 + i] = acc;
 }
 
 // members of active large-direct halos deleted since the last potential

@@ -4,9 +4,10 @@
   in FP64 and in f32: every kernel treats a candidate on its own;
 * direct summation above the reference's 1000 members (global-memory
   kernels) in FP64 matches an independent numpy FP64 brute force;
-* the f32 direct-summation form stays within the stated potential error of
-  the FP64 form (relative error of every member's first-pass potential
-  <= 2e-5, RMS <= 2e-6), and bound status differs for at most 0.1 % of the
+* the f32 direct-summation form (f32 pair terms, partial sums over 256
+  partners in f32 added in f64) stays within the stated potential error of
+  the FP64 form: relative error of every member's first-pass potential
+  <= 1e-5, RMS <= 1e-6; bound status differs for at most 0.1 % of the
   members.
 """
 import numpy as np
@@ -98,15 +99,15 @@ def test_large_direct_fp64_matches_numpy():
 
 def test_f32_potential_error_and_status():
     rng = np.random.default_rng(12)
-    sizes = np.concatenate([rng.integers(20, 1000, 150), [1200, 2500, 5000, 9000, 15000]])
+    sizes = np.concatenate([rng.integers(20, 1000, 150), [1200, 2500, 5000, 9000, 15000, 40000]])
     xs, vs = _halos(sizes, 12)
-    r64 = _run(xs, vs, direct_max=16384, f32=False, record=True)
-    r32 = _run(xs, vs, direct_max=16384, f32=True, record=True)
+    r64 = _run(xs, vs, direct_max=65536, f32=False, record=True)
+    r32 = _run(xs, vs, direct_max=65536, f32=True, record=True)
     assert np.array_equal(r64["index"], r32["index"])
     p64 = r64["phi"].astype(np.float64)
     p32 = r32["phi"].astype(np.float64)
     rel = np.abs(p32 - p64) / np.abs(p64)
-    assert rel.max() <= 2e-5, rel.max()
-    assert np.sqrt(np.mean(rel ** 2)) <= 2e-6
+    assert rel.max() <= 1e-5, rel.max()
+    assert np.sqrt(np.mean(rel ** 2)) <= 1e-6
     diff = int(np.sum(r64["in_halo"] != r32["in_halo"]))
     assert diff <= 1e-3 * max(1, int(r64["in_halo"].sum())), diff
