@@ -18,4 +18,5 @@ from .oracle import (  # noqa: F401
     oracle_green,
     oracle_select,
     oracle_depth,
+    oracle_tl_info,
 )

@@ -20,9 +20,10 @@
  *   - The FFT is this file's own mixed-radix complex FFT in double precision
  *     (the reference links FFTW3f, single precision). PM potentials therefore
  *     differ from the reference at the float rounding level.
- *   - The two-level inner refinement of 256^3 halos (bound_fortran.f90:403-1517)
- *     is not restated: 256-grid halos use the C1 + C2 split with the outer
- *     grid only. Parity for halos that the reference refines is unpinned.
+ *   - The two-level refinement of 256^3 halos (bound_fortran.f90:403-1517) is
+ *     restated (tl_*): block ratios, inner band grids, per-target cutoffs.
+ *     Its short-range pair sums gather per target over 27 cells of width
+ *     3.5 sp instead of the reference's task order (f32 sum order only).
  * Everything else (direct summation order, float/double mix, removal order,
  * statuses) follows the reference statement by statement.
  */
@@ -276,9 +277,12 @@ static double shape_s(double y)
 
 /* G on the half spectrum [n][n][n/2+1] (float) and the 5^3 self kernel.
  * Outer grids: apm_soft = 3.5, apm_range = 0, truncated (cutoff 1 - sin(kR)/kR). */
-void oracle_green(int32_t n, float *G, float *self125)
+/* Green function and 5^3 self kernel (bound_fortran.f90:2300-2382) of a grid
+ * of n, split softening `soft`, band range `range` (0: the outer grid's
+ * kernel) and truncation at R = n/2 (`truncated`). */
+void oracle_green_ex(int32_t n, double soft, double range, int truncated, float *G, float *self125)
 {
-    const double apm_soft = 3.5, radius = 0.5 * (double)n;
+    const double apm_soft = soft, radius = 0.5 * (double)n;
     const int nh = n / 2 + 1;
     #pragma omp parallel for schedule(dynamic)
     for (int k = 1; k <= n; ++k)
@@ -296,12 +300,15 @@ void oracle_green(int32_t n, float *G, float *self125)
                             asum += uk2;
                             double target;
                             if (kmag == 0.0) {
-                                target = -4.0 * BPI * radius * radius / 6.0;
+                                if (range == 0.0) target = -4.0 * BPI * radius * radius / 6.0;
+                                else if (truncated) target = 0.0;
+                                else target = -2.0 * BPI * (range * range - apm_soft * apm_soft) / 15.0;
                             } else {
                                 double ss = shape_s(apm_soft * kmag / 2.0);
+                                double sr = range == 0.0 ? 0.0 : shape_s(range * kmag / 2.0);
                                 double kr = kmag * radius;
-                                double cut = 1.0 - sin(kr) / kr;
-                                target = -4.0 * BPI * (ss * ss) * cut / (kmag * kmag);
+                                double cut = truncated ? 1.0 - sin(kr) / kr : 1.0;
+                                target = -4.0 * BPI * (ss * ss - sr * sr) * cut / (kmag * kmag);
                             }
                             bsum += uk2 * target;
                         }
@@ -328,6 +335,11 @@ void oracle_green(int32_t n, float *G, float *self125)
                     (float)(g[(size_t)x + (size_t)n * (y + (size_t)n * z)].re * inv);
             }
     free(g);
+}
+
+void oracle_green(int32_t n, float *G, float *self125)
+{
+    oracle_green_ex(n, 3.5, 0.0, 1, G, self125);
 }
 
 /* ------------------------------------------------------------------ */
@@ -526,6 +538,332 @@ typedef struct {
 
 typedef struct { int n; float *G; float self[125]; } green_t;
 
+/* ------------------------------------------------------------------ */
+/* two-level refinement of 256^3 halos (bound_fortran.f90:239-242,       */
+/* 403-1517): blocks of 16^3 coarse cells with many members get an inner  */
+/* grid (128^3 at ratio 2, 216^3 at ratio 4) carrying the band between    */
+/* 3.5 sp / r and 3.5 sp; their members' short-range cutoff drops to      */
+/* 3.5 sp / r. phi = C1 + phi_in + C2 (bound_fortran.f90:2744).           */
+
+#define TL_NREG 512
+#define TL_N12 3926
+#define TL_N24 14688
+#define TL_NR2 128
+#define TL_NR4 216
+
+typedef struct {
+    int n_active;
+    int region_r[TL_NREG];
+    int active[TL_NREG];
+    int *particle_r, *target_reg;
+    int *src_start, *src_member, *src_first;
+    float *src_wt;
+    int *tgt_first;
+    float *tgt_wt, *self_in;
+} tl_t;
+
+static void tl_free(tl_t *tl)
+{
+    free(tl->particle_r); free(tl->target_reg); free(tl->src_start); free(tl->src_member);
+    free(tl->src_first); free(tl->src_wt); free(tl->tgt_first); free(tl->tgt_wt); free(tl->self_in);
+    memset(tl, 0, sizeof(*tl));
+}
+
+/* inner band kernels, computed once per process (bound_init_inner_gk) */
+static const green_t *tl_green(int r)
+{
+    static green_t g2, g4;
+    green_t *g = r == 2 ? &g2 : &g4;
+    if (!g->G) {
+        const int n = r == 2 ? TL_NR2 : TL_NR4;
+        g->n = n;
+        g->G = (float *)malloc(sizeof(float) * (size_t)(n / 2 + 1) * n * n);
+        oracle_green_ex(n, 3.5, r == 2 ? 7.0 : 14.0, 0, g->G, g->self);
+    }
+    return g;
+}
+
+/* bound_region_bounds (bound_fortran.f90:531-539); b is 0-based */
+static void tl_region_bounds(int b, float lo[3], float hi[3])
+{
+    const int bb[3] = {b % 8, (b / 8) % 8, b / 64};
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = (float)(bb[d] * 16);
+        hi[d] = fminf(lo[d] + 16.0f, 127.0f);
+    }
+}
+
+/* TSC on the inner grid: first node and weights; 0 if outside [1, n_in/2] */
+static int tl_tsc(const float t[3], const float u0[3], int r, int n_in, int first[3], float wt[9])
+{
+    for (int ax = 0; ax < 3; ++ax) {
+        float coord = (t[ax] - u0[ax]) * (float)r;
+        int nearest = nint_f(coord);
+        float delta = coord - (float)nearest;
+        first[ax] = nearest;
+        float a0 = 0.5f - delta, a2 = 0.5f + delta;
+        wt[3 * ax + 0] = 0.5f * (a0 * a0);
+        wt[3 * ax + 1] = 0.75f - delta * delta;
+        wt[3 * ax + 2] = 0.5f * (a2 * a2);
+        if (nearest < 1 || nearest + 2 > n_in / 2) return 0;
+    }
+    return 1;
+}
+
+/* bound_two_level_setup_ratios + build_halo (bound_fortran.f90:551-823).
+ * Returns 0, or -1 (ierr 1) when a node leaves the inner half domain. */
+static int tl_setup(tl_t *tl, int n, const float *q, float spacing, const float origin[3], float app)
+{
+    memset(tl, 0, sizeof(*tl));
+    float *t = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+    int cnt[TL_NREG] = {0};
+    tl->particle_r = (int *)malloc(sizeof(int) * (size_t)n);
+    tl->target_reg = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int bb[3];
+        for (int d = 0; d < 3; ++d) {
+            t[3 * i + d] = (float)(((double)q[3 * i + d] - (double)origin[d]) / (double)spacing);
+            int cell = (int)floor((double)t[3 * i + d]);
+            int v = cell / 16;
+            bb[d] = v < 0 ? 0 : (v > 7 ? 7 : v);
+        }
+        tl->target_reg[i] = bb[0] + 8 * bb[1] + 64 * bb[2];
+        cnt[tl->target_reg[i]]++;
+    }
+    for (int b = 0; b < TL_NREG; ++b) {
+        int r = cnt[b] < TL_N12 ? 1 : (cnt[b] < TL_N24 ? 2 : 4);
+        if (r == 4 && 3.5 * (double)spacing / 4.0 < (double)app) r = 2;
+        if (r == 2 && 3.5 * (double)spacing / 2.0 < (double)app) r = 1;
+        tl->region_r[b] = r;
+        if (cnt[b] > 0 && r > 1) tl->active[tl->n_active++] = b;
+    }
+    for (int i = 0; i < n; ++i) tl->particle_r[i] = tl->region_r[tl->target_reg[i]];
+    if (tl->n_active == 0) { free(t); return 0; }
+    /* sources of each active region: members within its band */
+    tl->src_start = (int *)calloc((size_t)tl->n_active + 1, sizeof(int));
+    int cap = 0;
+    for (int k = 0; k < tl->n_active; ++k) {
+        int b = tl->active[k], r = tl->region_r[b];
+        float lo[3], hi[3];
+        tl_region_bounds(b, lo, hi);
+        float bo = 3.5f + 3.0f / (float)r;
+        for (int j = 0; j < n; ++j) {
+            const float *tj = t + 3 * j;
+            if (tj[0] >= lo[0] - bo && tj[0] <= hi[0] + bo && tj[1] >= lo[1] - bo &&
+                tj[1] <= hi[1] + bo && tj[2] >= lo[2] - bo && tj[2] <= hi[2] + bo) cap++;
+        }
+    }
+    tl->src_member = (int *)malloc(sizeof(int) * (size_t)(cap > 0 ? cap : 1));
+    tl->src_first = (int *)malloc(sizeof(int) * 3 * (size_t)(cap > 0 ? cap : 1));
+    tl->src_wt = (float *)malloc(sizeof(float) * 9 * (size_t)(cap > 0 ? cap : 1));
+    tl->tgt_first = (int *)calloc(3 * (size_t)n, sizeof(int));
+    tl->tgt_wt = (float *)calloc(9 * (size_t)n, sizeof(float));
+    tl->self_in = (float *)calloc((size_t)n, sizeof(float));
+    int p = 0, bad = 0;
+    for (int k = 0; k < tl->n_active && !bad; ++k) {
+        int b = tl->active[k], r = tl->region_r[b];
+        int n_in = r == 2 ? TL_NR2 : TL_NR4;
+        float lo[3], hi[3], u0[3];
+        tl_region_bounds(b, lo, hi);
+        float bo = 3.5f + 3.0f / (float)r;
+        for (int d = 0; d < 3; ++d) u0[d] = lo[d] - bo - (1.0f / (float)r) + (0.0f / (float)r);
+        tl->src_start[k] = p;
+        for (int j = 0; j < n; ++j) {
+            const float *tj = t + 3 * j;
+            if (!(tj[0] >= lo[0] - bo && tj[0] <= hi[0] + bo && tj[1] >= lo[1] - bo &&
+                  tj[1] <= hi[1] + bo && tj[2] >= lo[2] - bo && tj[2] <= hi[2] + bo)) continue;
+            tl->src_member[p] = j;
+            if (!tl_tsc(tj, u0, r, n_in, tl->src_first + 3 * p, tl->src_wt + 9 * p)) { bad = 1; break; }
+            ++p;
+        }
+        /* targets: members of the region, with the inner self term */
+        const green_t *g = tl_green(r);
+        for (int i = 0; i < n && !bad; ++i) {
+            if (tl->target_reg[i] != b) continue;
+            float *w9 = tl->tgt_wt + 9 * i;
+            if (!tl_tsc(t + 3 * i, u0, r, n_in, tl->tgt_first + 3 * i, w9)) { bad = 1; break; }
+            double acc = 0.0;
+            for (int sz = 0; sz < 3; ++sz)
+                for (int sy = 0; sy < 3; ++sy)
+                    for (int sx = 0; sx < 3; ++sx) {
+                        float sw = w9[sx] * w9[3 + sy] * w9[6 + sz];
+                        for (int tz = 0; tz < 3; ++tz)
+                            for (int ty = 0; ty < 3; ++ty)
+                                for (int tx = 0; tx < 3; ++tx) {
+                                    float tw = w9[tx] * w9[3 + ty] * w9[6 + tz];
+                                    acc = acc + (double)(sw * tw) *
+                                                    (double)g->self[(tx - sx + 2) + 5 * ((ty - sy + 2) + 5 * (tz - sz + 2))];
+                                }
+                    }
+            tl->self_in[i] = (float)acc;
+        }
+    }
+    tl->src_start[tl->n_active] = p;
+    free(t);
+    return bad ? -1 : 0;
+}
+
+/* test hook: block ratios of a member set (q f32 xyz, n members) on a
+ * 256-grid placement; counts[512] and ratios[512] out; returns the number
+ * of active blocks, or -1 when a stencil leaves an inner grid */
+int32_t oracle_tl_info(int32_t n, const float *q, float spacing, const float *origin, float app,
+                       int32_t *counts, int32_t *ratios)
+{
+    tl_t tl;
+    int rc = tl_setup(&tl, n, q, spacing, origin, app);
+    for (int b = 0; b < TL_NREG; ++b) {
+        ratios[b] = tl.region_r[b];
+        counts[b] = 0;
+    }
+    if (tl.target_reg)
+        for (int i = 0; i < n; ++i) counts[tl.target_reg[i]]++;
+    int na = tl.n_active;
+    tl_free(&tl);
+    return rc != 0 ? -1 : na;
+}
+
+/* bound_two_level_run_pm (bound_fortran.f90:825-942): the band potential of
+ * the members of each active region from its inner grid */
+static void tl_phi_in(const tl_t *tl, int n, const char *keep, const float *w, float spacing,
+                      double a, float *phi_in)
+{
+    for (int i = 0; i < n; ++i) phi_in[i] = 0.0f;
+    for (int k = 0; k < tl->n_active; ++k) {
+        int b = tl->active[k], r = tl->region_r[b];
+        const green_t *g = tl_green(r);
+        const int ng = g->n, nh = ng / 2 + 1;
+        const size_t nn = (size_t)ng * ng * ng;
+        float *rho = (float *)calloc(nn, sizeof(float));
+        cpx *grid = (cpx *)calloc(nn, sizeof(cpx));
+        for (int p = tl->src_start[k]; p < tl->src_start[k + 1]; ++p) {
+            int j = tl->src_member[p];
+            if (!keep[j]) continue;
+            const int *f = tl->src_first + 3 * p;
+            const float *w9 = tl->src_wt + 9 * p;
+            for (int sz = 0; sz < 3; ++sz)
+                for (int sy = 0; sy < 3; ++sy)
+                    for (int sx = 0; sx < 3; ++sx) {
+                        size_t gi = (size_t)(f[0] + sx) + (size_t)ng * ((f[1] + sy) + (size_t)ng * (f[2] + sz));
+                        rho[gi] = rho[gi] + w[j] * w9[sx] * w9[3 + sy] * w9[6 + sz];
+                    }
+        }
+        for (size_t gi = 0; gi < nn; ++gi) grid[gi].re = rho[gi];
+        fft3(grid, ng, -1);
+        for (int z = 0; z < ng; ++z)
+            for (int y = 0; y < ng; ++y)
+                for (int x = 0; x < ng; ++x) {
+                    int hx = x, hy = y, hz = z;
+                    if (x >= nh) { hx = ng - x; hy = (ng - y) % ng; hz = (ng - z) % ng; }
+                    double gv = g->G[(size_t)hx + (size_t)nh * (hy + (size_t)ng * hz)];
+                    size_t gi = (size_t)x + (size_t)ng * (y + (size_t)ng * z);
+                    grid[gi].re *= gv;
+                    grid[gi].im *= gv;
+                }
+        fft3(grid, ng, +1);
+        const float norm = (float)((double)ng * ng * ng);
+        for (size_t gi = 0; gi < nn; ++gi) rho[gi] = (float)grid[gi].re / norm;
+        const float pref = (float)(a * (double)r / (6.0 * BPI * (double)spacing));
+        for (int i = 0; i < n; ++i) {
+            if (tl->target_reg[i] != b || !keep[i]) continue;
+            const int *f = tl->tgt_first + 3 * i;
+            const float *w9 = tl->tgt_wt + 9 * i;
+            float raw = 0.0f;
+            for (int sz = 0; sz < 3; ++sz)
+                for (int sy = 0; sy < 3; ++sy)
+                    for (int sx = 0; sx < 3; ++sx)
+                        raw = raw + rho[(size_t)(f[0] + sx) + (size_t)ng * ((f[1] + sy) + (size_t)ng * (f[2] + sz))] *
+                                        w9[sx] * w9[3 + sy] * w9[6 + sz];
+            phi_in[i] = pref * (raw - w[i] * tl->self_in[i]);
+        }
+        free(rho);
+        free(grid);
+    }
+}
+
+/* short-range pair term of a two-level halo for target i (cutoff
+ * a_i = 3.5 sp / r_i), bound_fortran.f90:1160-1175: w_j * dphi, f32 */
+static float tl_pair(const float *qi, const float *qj, float wj, double ai, double eps, double factor)
+{
+    double dx = (double)qj[0] - (double)qi[0], dy = (double)qj[1] - (double)qi[1],
+           dz = (double)qj[2] - (double)qi[2];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < ai * ai)) return 0.0f;
+    double d = sqrt(r2);
+    float dphi = -(float)(factor * (oracle_depth(d, eps) - oracle_depth(d, ai)));
+    return wj * dphi;
+}
+
+/* bound_two_level_pp_full / _update (bound_fortran.f90:1088-1498) as a
+ * gather over the 27 cells of width 3.5 sp around each target (every pair
+ * within any cutoff lies there). The reference's f32 sums run in its own
+ * task order; only that rounding differs. */
+static void tl_pp_full(const c2cells_t *cc, int n, const float *q, const char *keep, const float *w,
+                       float spacing, float app_val, double a, const int *pr, float *phi_c2)
+{
+    int nk = 0;
+    for (int i = 0; i < n; ++i) { phi_c2[i] = 0.0f; nk += keep[i] ? 1 : 0; }
+    if (nk <= 1) return;
+    const double eps = (double)app_val, factor = a / (6.0 * BPI);
+    const int *nc = cc->ncell;
+    for (int i = 0; i < n; ++i) {
+        if (!keep[i]) continue;
+        const double ai = 3.5 * (double)spacing / (double)pr[i];
+        int ci = cc->cell_id[i];
+        int cz = (ci - 1) / (nc[0] * nc[1]) + 1, cy = ((ci - 1) % (nc[0] * nc[1])) / nc[0] + 1,
+            cx = ((ci - 1) % (nc[0] * nc[1])) % nc[0] + 1;
+        float acc = 0.0f;
+        for (int nz = cz - 1; nz <= cz + 1; ++nz) {
+            if (nz < 1 || nz > nc[2]) continue;
+            for (int ny = cy - 1; ny <= cy + 1; ++ny) {
+                if (ny < 1 || ny > nc[1]) continue;
+                for (int nx = cx - 1; nx <= cx + 1; ++nx) {
+                    if (nx < 1 || nx > nc[0]) continue;
+                    int c2 = nx + (ny - 1) * nc[0] + (nz - 1) * nc[0] * nc[1];
+                    for (int p = cc->cell_start[c2 - 1]; p < cc->cell_start[c2]; ++p) {
+                        int j = cc->cell_part[p];
+                        if (j == i || !keep[j]) continue;
+                        acc = acc + tl_pair(q + 3 * i, q + 3 * j, w[j], ai, eps, factor);
+                    }
+                }
+            }
+        }
+        phi_c2[i] = acc;
+    }
+}
+
+static void tl_pp_update(const c2cells_t *cc, int n, const float *q, const char *keep,
+                         const float *w, const int *just, int njust, float spacing, float app_val,
+                         double a, const int *pr, float *phi_c2)
+{
+    int nk = 0;
+    for (int i = 0; i < n; ++i) nk += keep[i] ? 1 : 0;
+    if (njust <= 0 || nk <= 0) return;
+    const double eps = (double)app_val, factor = a / (6.0 * BPI);
+    const int *nc = cc->ncell;
+    for (int k = 0; k < njust; ++k) {
+        int j = just[k];
+        int cj = cc->cell_id[j];
+        int cz = (cj - 1) / (nc[0] * nc[1]) + 1, cy = ((cj - 1) % (nc[0] * nc[1])) / nc[0] + 1,
+            cx = ((cj - 1) % (nc[0] * nc[1])) % nc[0] + 1;
+        for (int nz = cz - 1; nz <= cz + 1; ++nz) {
+            if (nz < 1 || nz > nc[2]) continue;
+            for (int ny = cy - 1; ny <= cy + 1; ++ny) {
+                if (ny < 1 || ny > nc[1]) continue;
+                for (int nx = cx - 1; nx <= cx + 1; ++nx) {
+                    if (nx < 1 || nx > nc[0]) continue;
+                    int c2 = nx + (ny - 1) * nc[0] + (nz - 1) * nc[0] * nc[1];
+                    for (int p = cc->cell_start[c2 - 1]; p < cc->cell_start[c2]; ++p) {
+                        int i = cc->cell_part[p];
+                        if (!keep[i]) continue;
+                        const double ai = 3.5 * (double)spacing / (double)pr[i];
+                        phi_c2[i] = phi_c2[i] - tl_pair(q + 3 * i, q + 3 * j, w[j], ai, eps, factor);
+                    }
+                }
+            }
+        }
+    }
+}
+
 /* energy ordering of bound_check_grid's heap sort: "greater" = larger energy,
  * or equal energy and smaller local index */
 static const float *g_energy;
@@ -561,6 +899,9 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
     int *first = NULL;
     c2cells_t cc = {{0, 0, 0}, 0, {0, 0, 0}, NULL, NULL, NULL};
     cpx *grid = NULL;
+    tl_t tl;
+    memset(&tl, 0, sizeof(tl));
+    float *phi_in = NULL;
     const int gid = t->gid;
     const int ng = gid >= 1 ? c->bn[gid - 1] : 0;
     float spacing = (float)t->spacing;
@@ -600,6 +941,11 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
             self[i] = (float)acc;
         }
         build_cells(&cc, n, q, spacing);
+        /* two-level refinement of 256^3 halos (bound_fortran.f90:2603-2606) */
+        if (gid == 4) {
+            if (tl_setup(&tl, n, q, spacing, t->origin, (float)c->app) != 0) { status = -1; goto done; }
+            if (tl.n_active > 0) phi_in = (float *)calloc((size_t)n, sizeof(float));
+        }
     }
     const double G_dir = c->a / (6.0 * BPI);
     for (int pass = 1; pass <= max_passes; ++pass) {
@@ -693,13 +1039,21 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
                 if (full_c2) phi_c2[i] = 0.0f;
             }
             free(rho);
-            if (full_c2) c2_full(&cc, n, q, keep, w, spacing, (float)c->app, c->a, phi_c2);
-            else c2_incremental(&cc, n, q, keep, w, just, n_just, spacing, (float)c->app, c->a, phi_c2);
+            if (tl.n_active > 0) {
+                tl_phi_in(&tl, n, keep, w, spacing, c->a, phi_in);
+                if (full_c2) tl_pp_full(&cc, n, q, keep, w, spacing, (float)c->app, c->a, tl.particle_r, phi_c2);
+                else tl_pp_update(&cc, n, q, keep, w, just, n_just, spacing, (float)c->app, c->a,
+                                  tl.particle_r, phi_c2);
+            } else if (full_c2) {
+                c2_full(&cc, n, q, keep, w, spacing, (float)c->app, c->a, phi_c2);
+            } else {
+                c2_incremental(&cc, n, q, keep, w, just, n_just, spacing, (float)c->app, c->a, phi_c2);
+            }
             n_just = 0;
         }
         for (int i = 0; i < n; ++i) {
             if (!keep[i]) continue;
-            if (gid >= 1) phi[i] = phi_c1[i] + phi_c2[i];
+            if (gid >= 1) phi[i] = tl.n_active > 0 ? phi_c1[i] + phi_in[i] + phi_c2[i] : phi_c1[i] + phi_c2[i];
             if (np_bound == 1 || n_source == 1) phi[i] = 0.0f;
             if (!isfinite(phi[i])) { status = -4; goto done; }
         }
@@ -756,6 +1110,8 @@ done:
     *np_bound_out = np_bound;
     free(phi); free(energy); free(phid); free(order); free(just);
     free(self); free(wt); free(first); free(phi_c1); free(phi_c2); free(grid);
+    tl_free(&tl);
+    free(phi_in);
     if (gid >= 1) free_cells(&cc);
     return status;
 }

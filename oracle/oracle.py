@@ -214,5 +214,21 @@ def oracle_select(np_, span, s15=17.152, ng=256, app=0.1):
     return st, g.value, l.value, sp.value
 
 
+def oracle_tl_info(q, spacing, origin, app):
+    """Two-level block counts and ratios (512 each) of members q (3, n) f32
+    relative positions on a 256-grid placement; returns (n_active, counts,
+    ratios); n_active is -1 when a stencil leaves an inner grid."""
+    lib = load_oracle()
+    q = np.ascontiguousarray(np.asarray(q, np.float32).T)
+    org = np.ascontiguousarray(origin, dtype=np.float32)
+    counts = np.zeros(512, np.int32)
+    ratios = np.zeros(512, np.int32)
+    lib.oracle_tl_info.restype = ctypes.c_int32
+    na = lib.oracle_tl_info(int(q.shape[0]), _ptr(q, ctypes.c_float), ctypes.c_float(spacing),
+                            _ptr(org, ctypes.c_float), ctypes.c_float(app),
+                            _ptr(counts, ctypes.c_int32), _ptr(ratios, ctypes.c_int32))
+    return int(na), counts, ratios
+
+
 def oracle_depth(r, h):
     return load_oracle().oracle_depth(float(r), float(h))

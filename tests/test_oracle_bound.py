@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from helpers import chain_halo
-from oracle import oracle_depth, oracle_fof_bound, oracle_green, oracle_select
+from oracle import oracle_depth, oracle_fof_bound, oracle_green, oracle_select, oracle_tl_info
 
 B = 2.0
 L = 512.0
@@ -199,3 +199,68 @@ def test_selection_table():
     st, g, lev, sp = oracle_select(30000, 17.0)
     assert (st, g) == (0, 4) and sp == pytest.approx(17.152 / 125)
     assert oracle_select(30000, 17.2)[0] == 2
+
+
+def test_two_level_block_ratio_rule():
+    # bound_fortran.f90:551-591: blocks of 16^3 coarse cells; < 3926 members
+    # ratio 1, < 14688 ratio 2, else 4; demoted while 3.5 sp / r < softening
+    sp = 0.125
+    origin = np.zeros(3, np.float32)
+    counts = {0: 3925, 1: 3926, 2: 14687, 3: 14688}
+    pts = []
+    for b, c in counts.items():
+        # block b along x: cells [16 b, 16 b + 16); members at the block centre
+        centre = np.array([(16 * b + 8) * sp, 8 * sp, 8 * sp])
+        pts.append(np.repeat(centre[:, None], c, axis=1))
+    q = np.hstack(pts).astype(np.float32)
+    na, cnt, rat = oracle_tl_info(q, sp, origin, 0.1)
+    assert list(cnt[:4]) == [3925, 3926, 14687, 14688] and cnt[4:].sum() == 0
+    assert list(rat[:4]) == [1, 2, 2, 4] and na == 3
+    # 3.5 * 0.125 / 4 = 0.109: ratio 4 survives app 0.1, falls to 2 at app 0.11,
+    # and 3.5 * 0.125 / 2 = 0.219 < 0.22 demotes ratio 2 to 1
+    assert list(oracle_tl_info(q, sp, origin, 0.11)[2][:4]) == [1, 2, 2, 2]
+    na2, _, rat2 = oracle_tl_info(q, sp, origin, 0.22)
+    assert list(rat2[:4]) == [1, 1, 1, 1] and na2 == 0
+
+
+def _direct_phi_targets(q, tgt, w, a, app, chunk=500):
+    """FP64 softened potential at the members tgt from every member of q."""
+    out = np.empty(len(tgt))
+    for s0 in range(0, len(tgt), chunk):
+        ti = tgt[s0:s0 + chunk]
+        d = q[:, ti, None] - q[:, None, :]
+        r = np.sqrt((d * d).sum(0))
+        h = app
+        e = 2 * r / h
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inner = (208 - e ** 2 * (112 + e ** 2 * (-56 + e * (14 + e * (8 - 3 * e))))) / (70 * h)
+            outer = (128 - (e ** 2 * (448 + e * (-280 + e * (56 + e * (14 + e * (-8 + e)))))
+                            - 224 * e - 12 / e)) / (70 * h)
+            D = np.where(r >= h, 1 / r, np.where(e <= 1, inner, outer))
+        D[np.arange(len(ti)), ti] = 0.0
+        out[s0:s0 + chunk] = -(a / (6 * np.pi)) * (D * w[None, :]).sum(1)
+    return out
+
+
+@pytest.mark.slow
+def test_two_level_potential_vs_direct_within_method_error():
+    # a concentrated 60,000-member halo on the 256 grid whose central block is
+    # refined: ratio 2 at softening 0.1, ratio 4 at 0.05 (3.5 sp / 4 >= 0.05)
+    rng = np.random.default_rng(31)
+    n = 60000
+    x = _nfw_halo(rng, n, 2.5, c=30.0) + 256.0
+    tgt = np.sort(rng.choice(n, 3000, replace=False))
+    for app, ratio in ((0.1, 2), (0.05, 4)):
+        r = oracle_fof_bound(np.asfortranarray(x), [n], L, 0.6, 10, np.zeros((3, n)), app=app,
+                             want_phi=True)
+        assert r["ierr"] == 0 and list(r["gid"]) == [4]
+        q = (x - x[:, :1]).astype(np.float32)
+        lo = np.minimum(q.min(1), 0).astype(np.float32)
+        sp = np.float32(r["spacing"][0])
+        na, cnt, rat = oracle_tl_info(q, sp, (lo - sp).astype(np.float32), app)
+        assert na >= 1 and (rat[cnt > 0] == ratio).any()
+        # members are in index order (one region, one candidate of all members)
+        ref = _direct_phi_targets(q.astype(np.float64), tgt, np.ones(n), 1.0, app)
+        got = r["phi"][tgt].astype(np.float64)
+        rel = np.abs(got - ref) / np.abs(ref)
+        assert np.median(rel) < 5e-3 and rel.max() < 3e-2, (app, np.median(rel), rel.max())
