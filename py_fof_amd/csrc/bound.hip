@@ -481,7 +481,10 @@ __device__ __forceinline__ double wave_k(int idx, int n) {
     return 2.0 * kPi * (double)mode / (double)n;
 }
 
-__global__ void k_green(float* __restrict__ G, int32_t n) {
+// range 0: the outer grids' kernel (truncated at R = n/2); range > 0: the
+// band kernel of the inner two-level grids, not truncated
+// (bound_fortran.f90:2318-2347)
+__global__ void k_green(float* __restrict__ G, int32_t n, double range, int32_t truncated) {
     const uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x;
     const uint64_t tot = (uint64_t)n * n * n;
     if (id >= tot) return;
@@ -511,14 +514,22 @@ __global__ void k_green(float* __restrict__ G, int32_t n) {
                 asum += uk2;
                 double target;
                 if (kmag == 0.0) {
-                    target = -4.0 * kPi * radius * radius / 6.0;
+                    if (range == 0.0) target = -4.0 * kPi * radius * radius / 6.0;
+                    else if (truncated) target = 0.0;
+                    else target = -2.0 * kPi * (range * range - soft * soft) / 15.0;
                 } else {
                     const double yv = soft * kmag / 2.0;
                     const double y4 = yv * yv * yv * yv;
                     const double ss = 12.0 * (1.0 / y4) * (2.0 - 2.0 * cos(yv) - yv * sin(yv));
+                    double sr = 0.0;
+                    if (range != 0.0) {
+                        const double yr = range * kmag / 2.0;
+                        const double yr4 = yr * yr * yr * yr;
+                        sr = 12.0 * (1.0 / yr4) * (2.0 - 2.0 * cos(yr) - yr * sin(yr));
+                    }
                     const double kr = kmag * radius;
-                    const double cut = 1.0 - sin(kr) / kr;
-                    target = -4.0 * kPi * (ss * ss) * cut / (kmag * kmag);
+                    const double cut = truncated ? 1.0 - sin(kr) / kr : 1.0;
+                    target = -4.0 * kPi * (ss * ss - sr * sr) * cut / (kmag * kmag);
                 }
                 bsum += uk2 * target;
             }
@@ -549,7 +560,7 @@ struct PmHalo {       // per PM halo
     int32_t n;        // grid size
     int32_t c2off;    // first C2 cell of the halo (global)
     int32_t nc[3];    // C2 cells per axis
-    int32_t pad;
+    int32_t tl;       // 1: two-level (per-member short-range cutoff, inner grids)
     double umin[3];   // C2 cell grid origin in u = q / spacing
 };
 
@@ -684,6 +695,248 @@ __global__ void k_interp(const float4* __restrict__ qw, const uint32_t* __restri
     phi_c1[t] = __fmul_rn(pref, __fsub_rn(__fsub_rn(raw, __fmul_rn(w, self[t])), dz0));
 }
 
+// ---------------------------------------------------------------------------
+// two-level refinement of 256^3 halos (bound_fortran.f90:403-1517)
+
+constexpr int kTlRegions = 512;   // 8^3 blocks of 16^3 coarse cells
+constexpr int kTlN12 = 3926, kTlN24 = 14688;
+constexpr int kTlN2 = 128, kTlN4 = 216;
+
+struct TlSlot {        // one refined block (inner grid)
+    float lo[3], hi[3];  // block bounds in coarse-cell units
+    float bo;            // band half-width 3.5 + 3 / r
+    float u0[3];         // inner grid origin in coarse-cell units
+    int32_t r, n_in, halo, cls_slot;  // ratio, inner grid, halo, slot within its ratio class
+    float sp;            // the halo's spacing
+};
+
+// coarse-cell coordinate of a member, as bound_fortran.f90:567 computes it
+__device__ __forceinline__ void tl_coord(const float4& q, const PmHalo& H, float t[3]) {
+    const double sp = (double)H.origin.w;
+    t[0] = (float)(((double)q.x - (double)H.origin.x) / sp);
+    t[1] = (float)(((double)q.y - (double)H.origin.y) / sp);
+    t[2] = (float)(((double)q.z - (double)H.origin.z) / sp);
+}
+
+__device__ __forceinline__ int tl_block(float t) {
+    const int cell = (int)floor((double)t);
+    const int v = cell / 16;  // truncation toward zero, as Fortran
+    return v < 0 ? 0 : (v > 7 ? 7 : v);
+}
+
+// block of every member of a two-level candidate, and member counts per block
+__global__ void k_tl_count(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                           const int32_t* __restrict__ tl_of_halo, const int32_t* __restrict__ pm_of_halo,
+                           const PmHalo* __restrict__ pm, uint32_t nmem, int16_t* __restrict__ treg,
+                           uint32_t* __restrict__ counts) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = tl_of_halo[h];
+    if (k < 0) { treg[t] = -1; return; }
+    float c[3];
+    tl_coord(qw[t], pm[pm_of_halo[h]], c);
+    const int b = tl_block(c[0]) + 8 * tl_block(c[1]) + 64 * tl_block(c[2]);
+    treg[t] = (int16_t)b;
+    atomicAdd(&counts[(size_t)k * kTlRegions + b], 1u);
+}
+
+// per-member ratio (1 outside two-level candidates and unrefined blocks)
+__global__ void k_tl_ratio(const uint32_t* __restrict__ mhalo, const int32_t* __restrict__ tl_of_halo,
+                           const int16_t* __restrict__ treg, const int8_t* __restrict__ region_r,
+                           uint32_t nmem, uint8_t* __restrict__ pr) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const int32_t k = tl_of_halo[mhalo[t]];
+    pr[t] = (uint8_t)(k < 0 ? 1 : region_r[(size_t)k * kTlRegions + treg[t]]);
+}
+
+__device__ __forceinline__ bool tl_in_band(const float c[3], const TlSlot& S) {
+    return c[0] >= S.lo[0] - S.bo && c[0] <= S.hi[0] + S.bo && c[1] >= S.lo[1] - S.bo &&
+           c[1] <= S.hi[1] + S.bo && c[2] >= S.lo[2] - S.bo && c[2] <= S.hi[2] + S.bo;
+}
+
+// inner-grid TSC of a coarse coordinate; false outside nodes [1, n_in/2]
+__device__ __forceinline__ bool tl_tsc(const float c[3], const TlSlot& S, int f[3], float w9[9]) {
+    bool ok = true;
+    for (int ax = 0; ax < 3; ++ax) {
+        const float coord = __fmul_rn(__fsub_rn(c[ax], S.u0[ax]), (float)S.r);
+        const int nearest = (int)roundf(coord);
+        const float d = __fsub_rn(coord, (float)nearest);
+        f[ax] = nearest;
+        const float a0 = __fsub_rn(0.5f, d), a2 = __fadd_rn(0.5f, d);
+        w9[3 * ax + 0] = __fmul_rn(0.5f, __fmul_rn(a0, a0));
+        w9[3 * ax + 1] = __fsub_rn(0.75f, __fmul_rn(d, d));
+        w9[3 * ax + 2] = __fmul_rn(0.5f, __fmul_rn(a2, a2));
+        if (nearest < 1 || nearest + 2 > S.n_in / 2) ok = false;
+    }
+    return ok;
+}
+
+// sources: every member of a candidate within the band of a refined block
+// (its own or a neighbouring one). FILL = false counts, true writes entries.
+template <bool FILL>
+__global__ void k_tl_sources(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                             const int32_t* __restrict__ tl_of_halo, const int32_t* __restrict__ pm_of_halo,
+                             const PmHalo* __restrict__ pm, const int16_t* __restrict__ treg,
+                             const int32_t* __restrict__ reg_slot, const TlSlot* __restrict__ slots,
+                             uint32_t nmem, uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                             int32_t* __restrict__ src_slot, uint32_t* __restrict__ src_member,
+                             int4* __restrict__ src_first, float* __restrict__ src_wt,
+                             int32_t* __restrict__ err) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = tl_of_halo[h];
+    if (k < 0) {
+        if (!FILL) cnt[t] = 0;
+        return;
+    }
+    float c[3];
+    tl_coord(qw[t], pm[pm_of_halo[h]], c);
+    const int b = treg[t];
+    const int bx = b % 8, by = (b / 8) % 8, bz = b / 64;
+    uint32_t m = 0, o = FILL ? off[t] : 0u;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = bx + dx, ny = by + dy, nz = bz + dz;
+                if (nx < 0 || nx > 7 || ny < 0 || ny > 7 || nz < 0 || nz > 7) continue;
+                const int32_t sl = reg_slot[(size_t)k * kTlRegions + nx + 8 * ny + 64 * nz];
+                if (sl < 0) continue;
+                const TlSlot S = slots[sl];
+                if (!tl_in_band(c, S)) continue;
+                if (FILL) {
+                    int f[3];
+                    float w9[9];
+                    if (!tl_tsc(c, S, f, w9)) atomicCAS(err, 0, 1);  // bound_fortran.f90:736-739
+                    src_slot[o + m] = sl;
+                    src_member[o + m] = t;
+                    src_first[o + m] = make_int4(f[0], f[1], f[2], 0);
+                    for (int i = 0; i < 9; ++i) src_wt[9 * (size_t)(o + m) + i] = w9[i];
+                }
+                ++m;
+            }
+    if (!FILL) cnt[t] = m;
+}
+
+// targets: members of refined blocks, their inner stencil and self term
+__global__ void k_tl_targets(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                             const int32_t* __restrict__ tl_of_halo, const int32_t* __restrict__ pm_of_halo,
+                             const PmHalo* __restrict__ pm, const int16_t* __restrict__ treg,
+                             const int32_t* __restrict__ reg_slot, const TlSlot* __restrict__ slots,
+                             const float* __restrict__ self_r2, const float* __restrict__ self_r4,
+                             uint32_t nmem, int32_t* __restrict__ tgt_slot, int4* __restrict__ tgt_first,
+                             float* __restrict__ tgt_wt, float* __restrict__ self_in,
+                             int32_t* __restrict__ err) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem) return;
+    const uint32_t h = mhalo[t];
+    const int32_t k = tl_of_halo[h];
+    const int32_t sl = k < 0 ? -1 : reg_slot[(size_t)k * kTlRegions + treg[t]];
+    tgt_slot[t] = sl;
+    if (sl < 0) return;
+    const TlSlot S = slots[sl];
+    float c[3];
+    tl_coord(qw[t], pm[pm_of_halo[h]], c);
+    int f[3];
+    float w9[9];
+    if (!tl_tsc(c, S, f, w9)) {  // bound_fortran.f90:766-769
+        atomicCAS(err, 0, 1);
+        return;
+    }
+    tgt_first[t] = make_int4(f[0], f[1], f[2], 0);
+    for (int i = 0; i < 9; ++i) tgt_wt[9 * (size_t)t + i] = w9[i];
+    const float* sk = S.r == 2 ? self_r2 : self_r4;
+    double acc = 0.0;
+    for (int sz = 0; sz < 3; ++sz)
+        for (int sy = 0; sy < 3; ++sy)
+            for (int sx = 0; sx < 3; ++sx) {
+                const float sw = __fmul_rn(__fmul_rn(w9[sx], w9[3 + sy]), w9[6 + sz]);
+                for (int tz = 0; tz < 3; ++tz)
+                    for (int ty = 0; ty < 3; ++ty)
+                        for (int tx = 0; tx < 3; ++tx) {
+                            const float tw = __fmul_rn(__fmul_rn(w9[tx], w9[3 + ty]), w9[6 + tz]);
+                            acc = __dadd_rn(acc, __dmul_rn((double)__fmul_rn(sw, tw),
+                                                           (double)sk[(tx - sx + 2) + 5 * ((ty - sy + 2) + 5 * (tz - sz + 2))]));
+                        }
+            }
+    self_in[t] = (float)acc;
+}
+
+// fixed-point deposit of kept sources of the slots [s_lo, s_hi) of one ratio
+// class into their inner grids (local slot = slot's class index - s_lo)
+__global__ void k_tl_deposit(const int32_t* __restrict__ src_slot, const uint32_t* __restrict__ src_member,
+                             const int4* __restrict__ src_first, const float* __restrict__ src_wt,
+                             uint32_t nsrc, const TlSlot* __restrict__ slots, int32_t ratio, int32_t s_lo,
+                             int32_t s_hi, const uint8_t* __restrict__ keep,
+                             const uint8_t* __restrict__ active, const float4* __restrict__ qw,
+                             const double* __restrict__ fscale, unsigned long long* __restrict__ acc) {
+    const uint32_t e = blockIdx.x * kB + threadIdx.x;
+    if (e >= nsrc) return;
+    const TlSlot S = slots[src_slot[e]];
+    if (S.r != ratio || S.cls_slot < s_lo || S.cls_slot >= s_hi || !active[S.halo]) return;
+    const uint32_t t = src_member[e];
+    if (!keep[t]) return;
+    const int n = S.n_in;
+    const int4 f = src_first[e];
+    const float* w9 = src_wt + 9 * (size_t)e;
+    const float w = qw[t].w;
+    const double fs = fscale[S.halo];
+    unsigned long long* g = acc + (size_t)(S.cls_slot - s_lo) * n * n * n;
+    for (int sz = 0; sz < 3; ++sz)
+        for (int sy = 0; sy < 3; ++sy)
+            for (int sx = 0; sx < 3; ++sx) {
+                const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                const long long fx = llrint((double)v * fs);
+                atomicAdd(&g[(size_t)(f.x + sx) + (size_t)n * ((size_t)(f.y + sy) + (size_t)n * (f.z + sz))],
+                          (unsigned long long)fx);
+            }
+}
+
+// band potential of kept targets (bound_fortran.f90:906-938)
+__global__ void k_tl_interp(const int32_t* __restrict__ tgt_slot, const int4* __restrict__ tgt_first,
+                            const float* __restrict__ tgt_wt, const float* __restrict__ self_in,
+                            const TlSlot* __restrict__ slots, int32_t ratio, int32_t s_lo, int32_t s_hi,
+                            const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
+                            const float4* __restrict__ qw, const float2* __restrict__ grid, uint32_t nmem,
+                            double a, float* __restrict__ phi_in) {
+    const uint32_t t = blockIdx.x * kB + threadIdx.x;
+    if (t >= nmem || !keep[t]) return;
+    const int32_t sl = tgt_slot[t];
+    if (sl < 0) return;
+    const TlSlot S = slots[sl];
+    if (S.r != ratio || S.cls_slot < s_lo || S.cls_slot >= s_hi || !active[S.halo]) return;
+    const int n = S.n_in;
+    const float norm = (float)((double)n * n * n);
+    const float2* gg = grid + (size_t)(S.cls_slot - s_lo) * n * n * n;
+    const int4 f = tgt_first[t];
+    const float* w9 = tgt_wt + 9 * (size_t)t;
+    float raw = 0.0f;
+    for (int sz = 0; sz < 3; ++sz)
+        for (int sy = 0; sy < 3; ++sy)
+            for (int sx = 0; sx < 3; ++sx) {
+                const float rho = __fdiv_rn(gg[(size_t)(f.x + sx) + (size_t)n * ((size_t)(f.y + sy) + (size_t)n * (f.z + sz))].x, norm);
+                raw = __fadd_rn(raw, __fmul_rn(__fmul_rn(__fmul_rn(rho, w9[sx]), w9[3 + sy]), w9[6 + sz]));
+            }
+    const float pref = (float)(a * (double)S.r / (6.0 * kPi * (double)S.sp));
+    phi_in[t] = __fmul_rn(pref, __fsub_rn(raw, __fmul_rn(qw[t].w, self_in[t])));
+}
+
+// short-range term of a two-level candidate for target i with cutoff a_i
+// (bound_fortran.f90:1160-1175): f64 differences of the f32 positions, depth
+// kernels in cube units, w_j * dphi in f32
+__device__ __forceinline__ float tl_pair(const float4& qi, const float4& qj, double ai, double eps,
+                                         double factor) {
+    const double dx = (double)qj.x - (double)qi.x, dy = (double)qj.y - (double)qi.y,
+                 dz = (double)qj.z - (double)qi.z;
+    const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    if (!(r2 < ai * ai)) return 0.0f;
+    const double d = sqrt(r2);
+    const float dphi = -(float)(factor * (depth_continuous(d, eps) - depth_continuous(d, ai)));
+    return __fmul_rn(qj.w, dphi);
+}
+
 // C2 cells (bound_fortran.f90:3125-3223): width 3.5 in u = q / spacing
 __global__ void __launch_bounds__(kB) k_c2_extent(const float4* __restrict__ qw,
                                                   const uint32_t* __restrict__ hoff,
@@ -797,7 +1050,7 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
                           const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
                           const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cstart,
                           const uint32_t* __restrict__ clist, uint32_t nmem, double a, double app,
-                          float* __restrict__ phi_c2) {
+                          const uint8_t* __restrict__ pr, float* __restrict__ phi_c2) {
     const uint32_t ps = blockIdx.x * kB + threadIdx.x;
     if (ps >= nmem) return;
     const uint32_t t = clist[ps];
@@ -810,6 +1063,8 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
     const double ht = (double)(float)app / (double)sp, hs = 3.5;
     const double r2cut = 12.25 * (double)sp * (double)sp;
     const float pref = (float)(a / (6.0 * kPi * (double)sp));
+    const double ai = 3.5 * (double)sp / (double)(pr ? pr[t] : 1), eps = (double)(float)app,
+                 factor = a / (6.0 * kPi);
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -827,6 +1082,10 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
                     const uint32_t j = clist[p];
                     if (j == t || !keep[j]) continue;
                     const float4 qj = qw[j];
+                    if (H.tl) {  // two-level: the target's own cutoff 3.5 sp / r
+                        acc = __fadd_rn(acc, tl_pair(qi, qj, ai, eps, factor));
+                        continue;
+                    }
                     const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
                     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
                     if (r2 < r2cut) {
@@ -876,7 +1135,8 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
                             const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
                             const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ dstart,
                             const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ clist,
-                            uint32_t nmem, double a, double app, float* __restrict__ phi_c2) {
+                            uint32_t nmem, double a, double app, const uint8_t* __restrict__ pr,
+                            float* __restrict__ phi_c2) {
     const uint32_t ps = blockIdx.x * kB + threadIdx.x;
     if (ps >= nmem) return;
     const uint32_t t = clist[ps];
@@ -889,6 +1149,8 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
     const double ht = (double)(float)app / (double)sp, hs = 3.5;
     const double r2cut = 12.25 * (double)sp * (double)sp;
     const float pref = (float)(a / (6.0 * kPi * (double)sp));
+    const double ai = 3.5 * (double)sp / (double)(pr ? pr[t] : 1), eps = (double)(float)app,
+                 factor = a / (6.0 * kPi);
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -904,6 +1166,10 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
                 const uint32_t c = (uint32_t)H.c2off + (uint32_t)x + (uint32_t)H.nc[0] * ((uint32_t)y + (uint32_t)H.nc[1] * (uint32_t)z);
                 for (uint32_t p = dstart[c]; p < dstart[c + 1]; ++p) {
                     const float4 qj = qw[dlist[p]];
+                    if (H.tl) {
+                        acc = __fsub_rn(acc, tl_pair(qi, qj, ai, eps, factor));
+                        continue;
+                    }
                     const double dx = (double)(qi.x - qj.x), dy = (double)(qi.y - qj.y), dz = (double)(qi.z - qj.z);
                     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
                     if (r2 < r2cut) {
@@ -925,15 +1191,18 @@ __global__ void k_copy_u8(const uint8_t* __restrict__ src, uint8_t* __restrict__
 
 // phi = C1 + C2 (bound_fortran.f90:2739-2754); zero for a single member
 __global__ void k_phi_combine(const uint32_t* __restrict__ mhalo, const int32_t* __restrict__ pm_of_halo,
+                              const PmHalo* __restrict__ pm,
                               const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
                               const int32_t* __restrict__ npb, const float* __restrict__ c1,
-                              const float* __restrict__ c2, uint32_t nmem, float* __restrict__ phi,
-                              int32_t* __restrict__ err) {
+                              const float* __restrict__ c2, const float* __restrict__ pin, uint32_t nmem,
+                              float* __restrict__ phi, int32_t* __restrict__ err) {
     const uint32_t t = blockIdx.x * kB + threadIdx.x;
     if (t >= nmem || !keep[t]) return;
     const uint32_t h = mhalo[t];
-    if (pm_of_halo[h] < 0 || !active[h]) return;
-    float p = __fadd_rn(c1[t], c2[t]);
+    const int32_t k = pm_of_halo[h];
+    if (k < 0 || !active[h]) return;
+    // two-level: C1 + band + C2 (bound_fortran.f90:2742-2744)
+    float p = pm[k].tl ? __fadd_rn(__fadd_rn(c1[t], pin[t]), c2[t]) : __fadd_rn(c1[t], c2[t]);
     if (npb[h] == 1) p = 0.0f;
     if (!isfinite(p)) atomicCAS(err, 0, 4);
     phi[t] = p;
@@ -1250,8 +1519,9 @@ int sel_grid(const SelCtx& c, int64_t np, double span, int& gid, double& sp_out)
     return 2;
 }
 
-GreenCache& green_for(Ctx& C, int n, hipStream_t s) {
-    GreenCache*& gc = C.greens[n];
+// range 0: outer kernel of grid n; range > 0: inner band kernel
+GreenCache& green_for(Ctx& C, int n, hipStream_t s, double range = 0.0) {
+    GreenCache*& gc = C.greens[n + (range > 0.0 ? 100000 * (int)range : 0)];
     if (!gc) gc = new GreenCache();
     if (gc->ready) return *gc;
     gc->n = n;
@@ -1259,7 +1529,7 @@ GreenCache& green_for(Ctx& C, int n, hipStream_t s) {
     float* G = gc->G.as<float>(tot);
     float2* tw = gc->tw.as<float2>(n);
     fft_make_plan(gc->plan, n, tw, s);
-    k_green<<<blocks(tot), kB, 0, s>>>(G, n);
+    k_green<<<blocks(tot), kB, 0, s>>>(G, n, range, range > 0.0 ? 0 : 1);
     PYFOF_LAUNCH_CHECK();
     // self kernel: inverse transform of G, / n^3, at the 5^3 corner
     DevBuf tmp;
@@ -1409,6 +1679,22 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     float *wt = nullptr, *selfv = nullptr, *phi_c1 = nullptr, *phi_c2 = nullptr;
     double* fscale = nullptr;
     void* tmp = C.buf("c2tmp").get(std::max({radix_temp_bytes(M), scan_temp_bytes(M), scan_temp_bytes(nh + 1)}) + 256);
+    // two-level state (256^3 candidates with refined blocks)
+    bool tl_on = false;
+    std::vector<TlSlot> tl_slots;
+    int tl_nslot[2] = {0, 0};  // slots of ratio 2 and 4
+    TlSlot* d_tlslots = nullptr;
+    int32_t *src_slot = nullptr, *tgt_slot = nullptr, *d_tl_ls = nullptr, *d_tl_sh = nullptr;
+    uint32_t* src_member = nullptr;
+    int4 *src_first = nullptr, *tgt_first = nullptr;
+    float *src_wt = nullptr, *tgt_wt = nullptr, *self_in = nullptr, *phi_in = nullptr;
+    uint8_t* pr = nullptr;
+    uint32_t nsrc = 0;
+    const float* Gin[2] = {nullptr, nullptr};
+    const FftPlan* plan_in[2] = {nullptr, nullptr};
+    unsigned long long* acc_in[2] = {nullptr, nullptr};
+    float2* grid_in[2] = {nullptr, nullptr};
+    const int tl_chunk[2] = {96, 24};  // inner grids per launch: 3.2 GB (128^3), 3.6 GB (216^3)
     if (npm > 0) {
         for (auto& H : pm) H.slot = nclass[H.cls]++;
         for (int g = 0; g < 4; ++g) res.n_pm[g] = (uint32_t)nclass[g];
@@ -1497,6 +1783,120 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
             d_slot_halo[g] = C.buf("slothalo" + std::to_string(g)).as<int32_t>(nclass[g]);
             PYFOF_HIP(hipMemcpyAsync(d_slot_halo[g], slot_halo[g].data(), 4 * (size_t)nclass[g], hipMemcpyHostToDevice, s));
         }
+        // ---- two-level refinement (bound_fortran.f90:551-823, 2603-2606)
+        std::vector<int32_t> tl_of_halo(nh, -1), tl_halos;
+        for (uint32_t k = 0; k < npm; ++k) {
+            // a 256^3 candidate; no block survives the demotion when 3.5 sp / 2 < app
+            if (pm[k].cls != 3 || 3.5 * (double)pm[k].origin.w / 2.0 < (double)(float)A.app) continue;
+            tl_of_halo[pm_halo[k]] = (int32_t)tl_halos.size();
+            tl_halos.push_back(pm_halo[k]);
+        }
+        if (!tl_halos.empty()) {
+            const size_t ntl = tl_halos.size();
+            int16_t* treg = C.buf("tlreg").as<int16_t>(M);
+            uint32_t* tcnt = C.buf("tlcnt").as<uint32_t>(ntl * kTlRegions);
+            int32_t* d_tlof = C.buf("tlof").as<int32_t>(nh);
+            PYFOF_HIP(hipMemcpyAsync(d_tlof, tl_of_halo.data(), 4 * (size_t)nh, hipMemcpyHostToDevice, s));
+            k_fill32<<<blocks(ntl * kTlRegions), kB, 0, s>>>(tcnt, ntl * kTlRegions, 0u);
+            PYFOF_LAUNCH_CHECK();
+            k_tl_count<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_tlof, d_pmof, d_pm, M, treg, tcnt);
+            PYFOF_LAUNCH_CHECK();
+            std::vector<uint32_t> cnt(ntl * kTlRegions);
+            PYFOF_HIP(hipMemcpyAsync(cnt.data(), tcnt, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
+            PYFOF_HIP(hipStreamSynchronize(s));
+            // ratios with demotion (bound_fortran.f90:569-586), refined blocks -> slots
+            std::vector<int8_t> region_r(ntl * kTlRegions, 1);
+            std::vector<int32_t> reg_slot(ntl * kTlRegions, -1);
+            for (size_t k = 0; k < ntl; ++k) {
+                PmHalo& H = pm[pm_of_halo[tl_halos[k]]];
+                const double sp = (double)H.origin.w, app32 = (double)(float)A.app;
+                for (int b = 0; b < kTlRegions; ++b) {
+                    const uint32_t c = cnt[k * kTlRegions + b];
+                    int r = c < (uint32_t)kTlN12 ? 1 : (c < (uint32_t)kTlN24 ? 2 : 4);
+                    if (r == 4 && 3.5 * sp / 4.0 < app32) r = 2;
+                    if (r == 2 && 3.5 * sp / 2.0 < app32) r = 1;
+                    region_r[k * kTlRegions + b] = (int8_t)r;
+                    if (c == 0 || r == 1) continue;
+                    TlSlot S;
+                    const int bb[3] = {b % 8, (b / 8) % 8, b / 64};
+                    S.bo = 3.5f + 3.0f / (float)r;
+                    for (int d = 0; d < 3; ++d) {
+                        S.lo[d] = (float)(bb[d] * 16);
+                        S.hi[d] = fminf(S.lo[d] + 16.0f, 127.0f);
+                        S.u0[d] = S.lo[d] - S.bo - (1.0f / (float)r) + (0.0f / (float)r);
+                    }
+                    S.r = r;
+                    S.n_in = r == 2 ? kTlN2 : kTlN4;
+                    S.halo = tl_halos[k];
+                    S.cls_slot = tl_nslot[r == 2 ? 0 : 1]++;
+                    S.sp = H.origin.w;
+                    reg_slot[k * kTlRegions + b] = (int32_t)tl_slots.size();
+                    tl_slots.push_back(S);
+                    H.tl = 1;
+                }
+            }
+            tl_on = !tl_slots.empty();
+            if (tl_on) {
+                PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
+                int8_t* d_rr = C.buf("tlrr").as<int8_t>(region_r.size());
+                int32_t* d_rs = C.buf("tlrs").as<int32_t>(reg_slot.size());
+                d_tlslots = C.buf("tlslots").as<TlSlot>(tl_slots.size());
+                PYFOF_HIP(hipMemcpyAsync(d_rr, region_r.data(), region_r.size(), hipMemcpyHostToDevice, s));
+                PYFOF_HIP(hipMemcpyAsync(d_rs, reg_slot.data(), 4 * reg_slot.size(), hipMemcpyHostToDevice, s));
+                PYFOF_HIP(hipMemcpyAsync(d_tlslots, tl_slots.data(), sizeof(TlSlot) * tl_slots.size(),
+                                         hipMemcpyHostToDevice, s));
+                pr = C.buf("tlpr").as<uint8_t>(M);
+                k_tl_ratio<<<blocks(M), kB, 0, s>>>(A.mhalo, d_tlof, treg, d_rr, M, pr);
+                PYFOF_LAUNCH_CHECK();
+                // sources: count, scan, fill
+                uint32_t* scnt = C.buf("tlscnt").as<uint32_t>(M);
+                uint32_t* soff = C.buf("tlsoff").as<uint32_t>(M);
+                uint32_t* stot = C.buf("tlstot").as<uint32_t>(1);
+                k_tl_sources<false><<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_tlof, d_pmof, d_pm, treg, d_rs,
+                                                             d_tlslots, M, scnt, nullptr, nullptr, nullptr,
+                                                             nullptr, nullptr, err);
+                PYFOF_LAUNCH_CHECK();
+                exclusive_scan_u32(scnt, soff, M, stot, tmp, s);
+                PYFOF_HIP(hipMemcpyAsync(&nsrc, stot, 4, hipMemcpyDeviceToHost, s));
+                PYFOF_HIP(hipStreamSynchronize(s));
+                src_slot = C.buf("tlsslot").as<int32_t>(std::max<uint32_t>(nsrc, 1));
+                src_member = C.buf("tlsmem").as<uint32_t>(std::max<uint32_t>(nsrc, 1));
+                src_first = C.buf("tlsfirst").as<int4>(std::max<uint32_t>(nsrc, 1));
+                src_wt = C.buf("tlswt").as<float>(9 * (size_t)std::max<uint32_t>(nsrc, 1));
+                k_tl_sources<true><<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_tlof, d_pmof, d_pm, treg, d_rs,
+                                                            d_tlslots, M, nullptr, soff, src_slot, src_member,
+                                                            src_first, src_wt, err);
+                PYFOF_LAUNCH_CHECK();
+                // inner kernels (bound_init_inner_gk) and targets
+                for (int c = 0; c < 2; ++c) {
+                    if (!tl_nslot[c]) continue;
+                    const int n_in = c == 0 ? kTlN2 : kTlN4;
+                    GreenCache& gc = green_for(C, n_in, s, c == 0 ? 7.0 : 14.0);
+                    Gin[c] = gc.G.as<float>(1);
+                    plan_in[c] = &gc.plan;
+                    const uint64_t cap = (uint64_t)std::min(tl_nslot[c], tl_chunk[c]) * n_in * n_in * n_in;
+                    acc_in[c] = C.buf(c == 0 ? "tlacc2" : "tlacc4").as<unsigned long long>(cap);
+                    grid_in[c] = C.buf(c == 0 ? "tlgrid2" : "tlgrid4").as<float2>(cap);
+                }
+                const float* self2 = tl_nslot[0] ? green_for(C, kTlN2, s, 7.0).self.as<float>(125) : nullptr;
+                const float* self4 = tl_nslot[1] ? green_for(C, kTlN4, s, 14.0).self.as<float>(125) : nullptr;
+                tgt_slot = C.buf("tltslot").as<int32_t>(M);
+                tgt_first = C.buf("tltfirst").as<int4>(M);
+                tgt_wt = C.buf("tltwt").as<float>(9 * (size_t)M);
+                self_in = C.buf("tlself").as<float>(M);
+                k_tl_targets<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_tlof, d_pmof, d_pm, treg, d_rs, d_tlslots,
+                                                      self2, self4, M, tgt_slot, tgt_first, tgt_wt, self_in, err);
+                PYFOF_LAUNCH_CHECK();
+                phi_in = C.buf("tlphiin").as<float>(M);
+                PYFOF_HIP(hipMemsetAsync(phi_in, 0, 4 * (size_t)M, s));
+                d_tl_ls = C.buf("tlls").as<int32_t>(std::max(tl_chunk[0], tl_chunk[1]));
+                d_tl_sh = C.buf("tlsh").as<int32_t>(std::max(tl_chunk[0], tl_chunk[1]));
+                int32_t herr = 0;
+                PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+                PYFOF_HIP(hipStreamSynchronize(s));
+                if (herr) return herr;  // a stencil left its inner grid (ierr 1)
+            }
+        }
         fscale = C.buf("fscale").as<double>(nh);
         phi_c1 = C.buf("phic1").as<float>(M);
         phi_c2 = C.buf("phic2").as<float>(M);
@@ -1546,9 +1946,40 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                                                       mkeep, grid[g], g, M, A.a, phi_c1);
                     PYFOF_LAUNCH_CHECK();
                 }
+                // two-level band potentials on the inner grids (bound_fortran.f90:825-942)
+                for (int c = 0; tl_on && c < 2; ++c) {
+                    const int ratio = c == 0 ? 2 : 4;
+                    const int n_in = c == 0 ? kTlN2 : kTlN4;
+                    for (int cs = 0; cs < tl_nslot[c]; cs += tl_chunk[c]) {
+                        std::vector<int32_t> ls, sh(tl_chunk[c], 0);
+                        for (const TlSlot& S : tl_slots) {
+                            if (S.r != ratio || S.cls_slot < cs || S.cls_slot >= cs + tl_chunk[c] || !on[S.halo])
+                                continue;
+                            ls.push_back(S.cls_slot - cs);
+                            sh[S.cls_slot - cs] = S.halo;
+                        }
+                        if (ls.empty()) continue;
+                        PYFOF_HIP(hipMemcpyAsync(d_tl_ls, ls.data(), 4 * ls.size(), hipMemcpyHostToDevice, s));
+                        PYFOF_HIP(hipMemcpyAsync(d_tl_sh, sh.data(), 4 * sh.size(), hipMemcpyHostToDevice, s));
+                        const uint32_t mc = (uint32_t)(n_in / 2 + 2);
+                        const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)ls.size());
+                        k_zero_region<<<gz, kB, 0, s>>>(acc_in[c], d_tl_ls, n_in, mc);
+                        PYFOF_LAUNCH_CHECK();
+                        k_tl_deposit<<<blocks(nsrc), kB, 0, s>>>(src_slot, src_member, src_first, src_wt, nsrc,
+                                                                 d_tlslots, ratio, cs, cs + tl_chunk[c], keep,
+                                                                 active, qw, fscale, acc_in[c]);
+                        PYFOF_LAUNCH_CHECK();
+                        fft_conv3d(*plan_in[c], grid_in[c], acc_in[c], d_tl_ls, (int32_t)ls.size(), d_tl_sh,
+                                   fscale, Gin[c], s);
+                        k_tl_interp<<<blocks(M), kB, 0, s>>>(tgt_slot, tgt_first, tgt_wt, self_in, d_tlslots, ratio,
+                                                             cs, cs + tl_chunk[c], keep, active, qw, grid_in[c], M,
+                                                             A.a, phi_in);
+                        PYFOF_LAUNCH_CHECK();
+                    }
+                }
                 if (pass == 1) {
                     k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
-                                                       clist, M, A.a, A.app, phi_c2);
+                                                       clist, M, A.a, A.app, pr, phi_c2);
                     PYFOF_LAUNCH_CHECK();
                 } else {
                     k_c2_del_flags<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, active, keep, keep_prev, M, dflag);
@@ -1569,12 +2000,12 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                         PYFOF_LAUNCH_CHECK();
                         inclusive_max_scan_u32(dstart + 1, dstart + 1, c2cells, tmp, s);
                         k_c2_update<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of,
-                                                             dstart, dvs, clist, M, A.a, A.app, phi_c2);
+                                                             dstart, dvs, clist, M, A.a, A.app, pr, phi_c2);
                         PYFOF_LAUNCH_CHECK();
                     }
                 }
-                k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, keep, active, npb, phi_c1, phi_c2, M,
-                                                       phi, err);
+                k_phi_combine<<<blocks(M), kB, 0, s>>>(A.mhalo, d_pmof, d_pm, keep, active, npb, phi_c1, phi_c2,
+                                                       phi_in, M, phi, err);
                 PYFOF_LAUNCH_CHECK();
             }
             // ---- large direct halos

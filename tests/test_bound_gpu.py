@@ -261,3 +261,67 @@ def test_pm_halos_agree_with_oracle():
 def test_pm_256_grid_halo_agrees_with_oracle():
     st, r = _pm_compare([26000], seed=32)
     assert st["n_pm256"] == 1
+
+
+def _tl_halo(seed=31, n=60000, c=30.0, rvir=2.5):
+    """A concentrated halo whose central 16^3-cell block is refined."""
+    from py_fof_amd.synth import isotropic, nfw_radii
+    rng = np.random.default_rng(seed)
+    x = np.asfortranarray(nfw_radii(rng, n, c, rvir) * isotropic(rng, n) + 256.0)
+    sigma = np.sqrt((1.0 / (6.0 * np.pi)) * n / (2.0 * rvir))
+    v = np.asfortranarray(sigma * rng.standard_normal((3, n)))
+    return x, v
+
+
+def _direct_phi_targets(q, tgt, a, app, chunk=500):
+    out = np.empty(len(tgt))
+    for s0 in range(0, len(tgt), chunk):
+        ti = tgt[s0:s0 + chunk]
+        d = q[:, ti, None] - q[:, None, :]
+        r = np.sqrt((d * d).sum(0))
+        h = app
+        e = 2 * r / h
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inner = (208 - e ** 2 * (112 + e ** 2 * (-56 + e * (14 + e * (8 - 3 * e))))) / (70 * h)
+            outer = (128 - (e ** 2 * (448 + e * (-280 + e * (56 + e * (14 + e * (-8 + e)))))
+                            - 224 * e - 12 / e)) / (70 * h)
+            D = np.where(r >= h, 1 / r, np.where(e <= 1, inner, outer))
+        D[np.arange(len(ti)), ti] = 0.0
+        out[s0:s0 + chunk] = -(a / (6 * np.pi)) * D.sum(1)
+    return out
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("app", [0.1, 0.05])
+def test_two_level_halo_agrees_with_oracle_and_direct_sum(app):
+    # refined at ratio 2 (app 0.1) and 4 (app 0.05), bound_fortran.f90:403-1517
+    from py_fof_amd import _native
+    import py_fof_amd as pf
+    x, v = _tl_halo()
+    n = x.shape[1]
+    ctx = _native.Context(0)
+    ctx.set_option(_native.OPT_RECORD_PHI, 1)
+    inh, hid, am = pf._fof_core(x, v, L_Box=L, linking_length=0.6, np_halo_min=NP_MIN,
+                                bound_check=True, return_ang_mom=True, softening=app, _ctx=ctx)
+    st = ctx.stats()
+    phi, idx = ctx.potential()
+    ctx.close()
+    assert st["n_pm256"] == 1
+    r = oracle_fof_bound(x, [n], L, 0.6, NP_MIN, v, app=app, s15=15.0 * 0.67 * L / 300.0,
+                         want_phi=True)
+    assert r["ierr"] == 0 and list(r["gid"]) == [4]
+    # first-pass potential: GPU (f32 FFTs) against the oracle (f64 FFTs)
+    assert np.array_equal(idx, np.arange(n))
+    rel = np.abs(phi.astype(np.float64) - r["phi"][:n]) / np.abs(r["phi"][:n].astype(np.float64))
+    assert np.median(rel) < 1e-5 and rel.max() < 1e-3, (np.median(rel), rel.max())
+    # and against the FP64 direct sum within the method error
+    rng = np.random.default_rng(5)
+    tgt = np.sort(rng.choice(n, 2000, replace=False))
+    q = (x - x[:, :1]).astype(np.float32).astype(np.float64)
+    ref = _direct_phi_targets(q, tgt, 1.0, app)
+    rel_d = np.abs(phi[tgt].astype(np.float64) - ref) / np.abs(ref)
+    assert np.median(rel_d) < 5e-3 and rel_d.max() < 3e-2
+    # bound status
+    ref_in = r["in_halo"].astype(bool)
+    assert 0 < ref_in.sum() < n
+    assert np.sum(inh != ref_in) <= 2e-3 * ref_in.sum()
