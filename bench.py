@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate, jobs/s (0: 0.9 x the sequential rate)")
-    ap.add_argument("--only", choices=["all", "fof", "bound", "serve"], default="all",
+    ap.add_argument("--only", choices=["all", "fof", "bound", "serve", "copy"], default="all",
                     help="run one leg only (profiling); the headline needs 'all' or 'fof'")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
@@ -287,20 +287,26 @@ def bench_bound(args, torch, _native, ctx):
     out["deterministic"] = all(same)  # repeated calls of each form gave identical in_halo
     if not args.no_cpu and args.bound_cpu_halos > 0:
         from oracle import oracle_fof_bound
-        k = min(args.bound_cpu_halos, len(np_halo))
-        nk = int(np_halo[:k].sum())
-        log(args, f"unbinding cpu baseline on the first {k} halos ({nk} particles)")
+        # bounded sample: the first halos of the set below 20,000 members (the
+        # 256^3 two-level halos would take minutes each in the f64 oracle)
+        starts = np.concatenate([[0], np.cumsum(np_halo)])
+        pick = [i for i in range(len(np_halo)) if np_halo[i] < 20000][:args.bound_cpu_halos]
+        cols = np.concatenate([np.arange(starts[i], starts[i + 1]) for i in pick])
+        k, nk = len(pick), int(len(cols))
+        log(args, f"unbinding cpu baseline on {k} halos ({nk} particles)")
         t0 = time.perf_counter()
-        r = oracle_fof_bound(xp[:, :nk], np_halo[:k], L, b, 10, vp[:, :nk], app=app, s15=s15)
+        r = oracle_fof_bound(np.asfortranarray(xp[:, cols]), np_halo[pick], L, b, 10,
+                             np.asfortranarray(vp[:, cols]), app=app, s15=s15)
         dtc = time.perf_counter() - t0
         nvalid = int(len(np.unique(r["halo_id"][r["halo_id"] > 0])))
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = {
             "value": round(nvalid / dtc, 2), "unit": "bound halos/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle FoF + unbinding on the first {k} halos of the same set "
-                      f"({nk} particles, {nvalid} bound halos); FoF linking and direct sums "
-                      f"on 1 thread, FFTs and Green functions on {threads}; {dtc:.2f} s",
+            "sample": f"oracle FoF + unbinding on the first {k} halos of the same set with "
+                      f"fewer than 20,000 members ({nk} particles, {nvalid} bound halos); FoF "
+                      f"linking and direct sums on 1 thread, FFTs and Green functions on "
+                      f"{threads}; {dtc:.2f} s",
         }
     return out
 
@@ -429,6 +435,21 @@ def main():
         ctx = _native.Context(0)
         print(json.dumps({"metric": METRIC, "n_gpus": 1,
                           "unbinding": bench_bound(args, torch, _native, ctx)}), flush=True)
+        return
+    if args.only == "copy":
+        # calibration for HBM counters: a device copy of 1 GiB (1 GiB read,
+        # 1 GiB written per launch), 3 launches
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 3
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30,
+                          "ms_per_copy": round(dt * 1e3, 3),
+                          "GB_per_s": round(2 * (1 << 30) / dt / 1e9, 1)}), flush=True)
         return
     if args.only == "serve":
         print(json.dumps({"metric": METRIC, "n_gpus": 1, "serving": bench_serve(args)}),

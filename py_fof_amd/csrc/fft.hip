@@ -65,12 +65,12 @@ __device__ __forceinline__ void dft_small(float2* v, int r) {
 
 // Stockham autosort stages on the kLines lines held in buf[start]; returns the
 // buffer holding the result. Every thread of the workgroup must call it.
-template <bool INV>
-__device__ __forceinline__ int stockham(float2 (*buf)[kLines * kMaxN], int start, int n,
+template <bool INV, int NL = kLines>
+__device__ __forceinline__ int stockham(float2 (*buf)[NL * kMaxN], int start, int n,
                                         const Radices& rad, const float2* __restrict__ tw) {
     int cur = start;
     int ns = 1;
-    const int tpl = kFftBlock / kLines;  // threads per line
+    const int tpl = kFftBlock / NL;  // threads per line
     const int c = threadIdx.x / tpl;
     const int lt = threadIdx.x - c * tpl;
     for (int st = 0; st < rad.nrad; ++st) {
@@ -151,63 +151,110 @@ struct ConvDesc {
     int32_t out_lim;   // elements e > out_lim are not stored
 };
 
-template <int MODE>
+// NL lines per workgroup (8 or 16: a row segment of 16 float2 is one 128 B
+// line). The z pass runs every listed slot in one workgroup: the spectrum G
+// of its lines is read once into registers and reused for all slots.
+template <int MODE, int NL>
 __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ data,
                                                         const unsigned long long* __restrict__ acc,
                                                         int32_t n, const int32_t* __restrict__ slots,
+                                                        int32_t nslots,
                                                         const int32_t* __restrict__ slot_halo,
                                                         const double* __restrict__ fscale,
                                                         const float* __restrict__ G, Radices rad,
                                                         const float2* __restrict__ tw, ConvDesc d) {
-    __shared__ float2 buf[2][kLines * kMaxN];
+    __shared__ float2 buf[2][NL * kMaxN];
+    constexpr int kPer = NL * kMaxN / kFftBlock;  // elements per thread at most
     const AxisDesc& ax = d.ax;
     const int64_t grid = (int64_t)n * n * n;
     const int32_t g = blockIdx.x;
     const int32_t i0 = g % ax.c0;
     const int32_t i1 = (g / ax.c0) % ax.c1;
-    const int32_t ib = g / (ax.c0 * ax.c1);
+    const int32_t ib0 = MODE == kConvZ ? 0 : g / (ax.c0 * ax.c1);
+    const int32_t ib1 = MODE == kConvZ ? nslots : ib0 + 1;
     const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
-    const int64_t sbase = (int64_t)slots[ib] * grid;
-    const int line0 = ax.x_contig ? i0 * kLines : 0;  // y of line 0 in the x passes
-    double inv = 0.0;
-    if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
+    const int line0 = ax.x_contig ? i0 * NL : 0;  // y of line 0 in the x passes
     const int tid = threadIdx.x;
-    const int total = kLines * n;
-    for (int idx = tid; idx < total; idx += kFftBlock) {
-        int c, e;
-        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
-        else { e = idx / kLines; c = idx - e * kLines; }
-        float2 v = make_float2(0.0f, 0.0f);
-        if (e <= d.in_lim && (!ax.x_contig || line0 + c <= d.line_lim)) {
-            const int64_t o = sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride;
-            if (MODE == kConvXFwd) v = make_float2((float)((double)(long long)acc[o] * inv), 0.0f);
-            else v = data[o];
+    const int total = NL * n;
+    float gk[kPer];
+    if (MODE == kConvZ) {
+        for (int k = 0; k < kPer; ++k) {
+            const int idx = tid + k * kFftBlock;
+            gk[k] = 0.0f;
+            if (idx < total) {
+                const int e = idx / NL, c = idx - e * NL;
+                gk[k] = G[goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
+            }
         }
-        buf[0][c * n + e] = v;
     }
-    __syncthreads();
-    int cur;
-    if (MODE == kConvXFwd || MODE == kConvYFwd) {
-        cur = stockham<false>(buf, 0, n, rad, tw);
-    } else if (MODE == kConvZ) {
-        cur = stockham<false>(buf, 0, n, rad, tw);
+    for (int32_t ib = ib0; ib < ib1; ++ib) {
+        const int64_t sbase = (int64_t)slots[ib] * grid;
+        double inv = 0.0;
+        if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
         for (int idx = tid; idx < total; idx += kFftBlock) {
-            const int e = idx / kLines, c = idx - e * kLines;
-            const float gk = G[goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
-            float2& v = buf[cur][c * n + e];
-            v = make_float2(v.x * gk, v.y * gk);
+            int c, e;
+            if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+            else { e = idx / NL; c = idx - e * NL; }
+            float2 v = make_float2(0.0f, 0.0f);
+            if (e <= d.in_lim && (!ax.x_contig || line0 + c <= d.line_lim)) {
+                const int64_t o = sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride;
+                if (MODE == kConvXFwd) v = make_float2((float)((double)(long long)acc[o] * inv), 0.0f);
+                else v = data[o];
+            }
+            buf[0][c * n + e] = v;
         }
         __syncthreads();
-        cur = stockham<true>(buf, cur, n, rad, tw);
-    } else {
-        cur = stockham<true>(buf, 0, n, rad, tw);
+        int cur;
+        if (MODE == kConvXFwd || MODE == kConvYFwd) {
+            cur = stockham<false, NL>(buf, 0, n, rad, tw);
+        } else if (MODE == kConvZ) {
+            cur = stockham<false, NL>(buf, 0, n, rad, tw);
+            for (int k = 0; k < kPer; ++k) {
+                const int idx = tid + k * kFftBlock;
+                if (idx >= total) break;
+                const int e = idx / NL, c = idx - e * NL;
+                float2& v = buf[cur][c * n + e];
+                v = make_float2(v.x * gk[k], v.y * gk[k]);
+            }
+            __syncthreads();
+            cur = stockham<true, NL>(buf, cur, n, rad, tw);
+        } else {
+            cur = stockham<true, NL>(buf, 0, n, rad, tw);
+        }
+        for (int idx = tid; idx < total; idx += kFftBlock) {
+            int c, e;
+            if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+            else { e = idx / NL; c = idx - e * NL; }
+            if (e > d.out_lim || (ax.x_contig && line0 + c > d.line_lim)) continue;
+            data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * n + e];
+        }
+        __syncthreads();  // buf is reused by the next slot
     }
-    for (int idx = tid; idx < total; idx += kFftBlock) {
-        int c, e;
-        if (ax.x_contig) { c = idx / n; e = idx - c * n; }
-        else { e = idx / kLines; c = idx - e * kLines; }
-        if (e > d.out_lim || (ax.x_contig && line0 + c > d.line_lim)) continue;
-        data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * n + e];
+}
+
+template <int NL>
+void conv_yz(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
+             int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
+             const Radices& rad, int m, int mc, int64_t n2, hipStream_t s, bool forward_y) {
+    const int n = p.n;
+    const AxisDesc ax_y{NL, n2, n / NL, mc, 1, n, 0};
+    const uint32_t gy = (uint32_t)((n / NL) * mc) * (uint32_t)nslots;
+    if (forward_y) {
+        const ConvDesc dyf{ax_y, n, m, n - 1};
+        k_fft_conv<kConvYFwd, NL><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
+                                                            rad, p.tw, dyf);
+        PYFOF_LAUNCH_CHECK();
+        const AxisDesc ax_z{NL, n, n / NL, n, 1, n2, 0};
+        const ConvDesc dz{ax_z, n, m, m};
+        k_fft_conv<kConvZ, NL><<<(uint32_t)((n / NL) * n), kFftBlock, 0, s>>>(grid, acc, n, slots, nslots,
+                                                                              slot_halo, fscale, G, rad,
+                                                                              p.tw, dz);
+        PYFOF_LAUNCH_CHECK();
+    } else {
+        const ConvDesc dyi{ax_y, n, n - 1, m};
+        k_fft_conv<kConvYInv, NL><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
+                                                            rad, p.tw, dyi);
+        PYFOF_LAUNCH_CHECK();
     }
 }
 
@@ -270,22 +317,18 @@ void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, c
     const int64_t n2 = (int64_t)n * n;
     const int ygroups = (mc + kLines - 1) / kLines;
     const AxisDesc ax_x{(int64_t)kLines * n, n2, ygroups, mc, n, 1, 1};
-    const AxisDesc ax_y{kLines, n2, n / kLines, mc, 1, n, 0};
-    const AxisDesc ax_z{kLines, n, n / kLines, n, 1, n2, 0};
-    const ConvDesc dxf{ax_x, m, m, n - 1}, dyf{ax_y, n, m, n - 1}, dz{ax_z, n, m, m},
-        dyi{ax_y, n, n - 1, m}, dxi{ax_x, m, n - 1, m};
+    const ConvDesc dxf{ax_x, m, m, n - 1}, dxi{ax_x, m, n - 1, m};
     const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
-    const uint32_t gy = (uint32_t)((n / kLines) * mc) * (uint32_t)nslots;
-    const uint32_t gz = (uint32_t)((n / kLines) * n) * (uint32_t)nslots;
-    k_fft_conv<kConvXFwd><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dxf);
+    k_fft_conv<kConvXFwd, kLines><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
+                                                             rad, p.tw, dxf);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvYFwd><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dyf);
-    PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvZ><<<gz, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dz);
-    PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvYInv><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dyi);
-    PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvXInv><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, slot_halo, fscale, G, rad, p.tw, dxi);
+    // y and z passes: 16 lines (128 B row segments) when n allows
+    if (n % 16 == 0) conv_yz<16>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, true);
+    else conv_yz<kLines>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, true);
+    if (n % 16 == 0) conv_yz<16>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, false);
+    else conv_yz<kLines>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, false);
+    k_fft_conv<kConvXInv, kLines><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
+                                                             rad, p.tw, dxi);
     PYFOF_LAUNCH_CHECK();
 }
 
