@@ -44,6 +44,16 @@ constexpr int kWave = 64;
 
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// MI355X deals the workgroups of a launch round-robin to its 8 XCDs, each
+// with its own L2: workgroup b runs on XCD b % 8. xcd_block(b, n) maps b to a
+// logical block so that every XCD works through one contiguous range of
+// logical blocks (a bijection of [0, n)); kernels whose neighbouring blocks
+// share data (cell rows, z-slabs) then find it in one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    const uint32_t q = n / 8, r = n % 8, x = b % 8, k = b / 8;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Order-preserving map of doubles onto uint64 so that atomicMin/Max work.
 __host__ __device__ inline unsigned long long dbl_to_ordered(double v) {
     unsigned long long u;

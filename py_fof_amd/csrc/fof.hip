@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(kBlk) k_link(const float4* __restrict__ xc,
                                                const uint32_t* __restrict__ sub_list,
                                                const RegionDev* __restrict__ reg, int32_t* parent,
                                                LinkParams P) {
-    const uint32_t p = blockIdx.x * kBlk + threadIdx.x;
+    const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
     if (p >= P.N) return;
     const float4 me = xc[p];
     const RegionDev& g = reg[__float_as_int(me.w)];
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_intra(const uint32_t* __restrict
                                                       int32_t* __restrict__ croot,
                                                       int32_t* __restrict__ needs_sub, LinkParams P) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const uint32_t d = blockIdx.x;
+    const uint32_t d = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t c = dense_cells[d];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (!g.clique) {
@@ -676,7 +676,7 @@ __global__ void __launch_bounds__(kWave) k_dense_cell_cross(const uint32_t* __re
                                                             const uint32_t* __restrict__ sub_list,
                                                             const float4* __restrict__ xc, int32_t* parent,
                                                             LinkParams P) {
-    const uint32_t d = blockIdx.x;
+    const uint32_t d = xcd_block(blockIdx.x, gridDim.x);
     if (d >= ndense || ncomp[d] != 1) return;
     const uint32_t c = dense_cells[d];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
@@ -784,7 +784,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
                                                       const RegionDev* __restrict__ reg,
                                                       const float4* __restrict__ xc, int32_t* parent,
                                                       LinkParams P) {
-    const uint32_t t = blockIdx.x * kBlk + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
     if (t >= nsubs) return;
     const uint4 ent = subs[t];
     const RegionDev& g = reg[ent.w];
