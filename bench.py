@@ -33,6 +33,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FoF Mparticles/s; bound halos/s batched vs plain at stated err; p99 latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per call of the bbox + binning + linking kernels on the
+# 134,217,728-particle workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# scaled by a 1 GiB copy (profiles/fof_nfw134M_hbm_traffic.md)
+TRAFFIC_BYTES_134M = 92.06e9
 PHASES = ["bbox", "binning", "linking", "relabel", "unbinding", "catalog"]
 
 
@@ -525,8 +529,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "binning + linking (k_bbox..k_link), algorithmic bytes N*64 + cells*12",
+            "traffic": TRAFFIC_BYTES_134M if n == 134_217_728 else None,
+            "kernel": "binning + linking (k_bbox..k_dense_cross), algorithmic bytes N*64 + cells*12; "
+                      "traffic: measured HBM bytes of the same kernels per call",
         },
         "phases_ms": {k: round(float(v), 3) for k, v in zip(
             PHASES, phases)},

@@ -2070,18 +2070,30 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         return IERR_OK;
     };
 
+    // largest candidates first: workgroups of one launch are scheduled in
+    // order, so the long ones start early and the launch ends sooner (the
+    // order changes no result: every candidate is solved on its own)
+    auto by_size = [&](std::vector<int32_t> v) {
+        std::stable_sort(v.begin(), v.end(), [&](int32_t x, int32_t y) { return hsize(x) > hsize(y); });
+        return v;
+    };
     int32_t* d_dl = C.buf("dlist").as<int32_t>(std::max<size_t>(dsmall.size(), 1));
-    if (!dsmall.empty())
-        PYFOF_HIP(hipMemcpyAsync(d_dl, dsmall.data(), 4 * dsmall.size(), hipMemcpyHostToDevice, s));
     if (A.batched) {
         // every halo of a kind in one launch per step
-        if (!dsmall.empty()) direct_small(d_dl, (uint32_t)dsmall.size());
+        if (!dsmall.empty()) {
+            const std::vector<int32_t> order = by_size(dsmall);
+            PYFOF_HIP(hipMemcpyAsync(d_dl, order.data(), 4 * order.size(), hipMemcpyHostToDevice, s));
+            direct_small(d_dl, (uint32_t)order.size());
+        }
         std::vector<int32_t> it;
         for (uint32_t h = 0; h < nh; ++h) if (kind[h] != 0) it.push_back((int32_t)h);
+        it = by_size(it);
         const int32_t e = iterate(it);
         if (e != IERR_OK) return e;
     } else {
         // the plain form: one halo at a time, in candidate order
+        if (!dsmall.empty())
+            PYFOF_HIP(hipMemcpyAsync(d_dl, dsmall.data(), 4 * dsmall.size(), hipMemcpyHostToDevice, s));
         size_t k = 0;
         for (uint32_t h = 0; h < nh; ++h) {
             if (kind[h] == 0) {
