@@ -1051,7 +1051,7 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
                           const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cstart,
                           const uint32_t* __restrict__ clist, uint32_t nmem, double a, double app,
                           const uint8_t* __restrict__ pr, float* __restrict__ phi_c2) {
-    const uint32_t ps = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
     if (ps >= nmem) return;
     const uint32_t t = clist[ps];
     if (!keep[t]) return;
@@ -1137,7 +1137,7 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
                             const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ clist,
                             uint32_t nmem, double a, double app, const uint8_t* __restrict__ pr,
                             float* __restrict__ phi_c2) {
-    const uint32_t ps = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
     if (ps >= nmem) return;
     const uint32_t t = clist[ps];
     if (!keep[t]) return;

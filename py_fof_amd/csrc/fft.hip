@@ -132,6 +132,52 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_axis(float2* __restrict__ dat
 
 
 // ---------------------------------------------------------------------------
+// Compile-time radix plans of the grid sizes in use (the same factor order as
+// fft_make_plan: 4s, then 2s, then 3s). With N, the radix and the stage span
+// known at compile time, every index division folds to shifts or multiplies.
+
+template <int N> struct RadixPlan;
+template <> struct RadixPlan<48> { static constexpr int nrad = 3; static constexpr int r[5] = {4, 4, 3, 1, 1}; };
+template <> struct RadixPlan<64> { static constexpr int nrad = 3; static constexpr int r[5] = {4, 4, 4, 1, 1}; };
+template <> struct RadixPlan<128> { static constexpr int nrad = 4; static constexpr int r[5] = {4, 4, 4, 2, 1}; };
+template <> struct RadixPlan<216> { static constexpr int nrad = 5; static constexpr int r[5] = {4, 2, 3, 3, 3}; };
+template <> struct RadixPlan<256> { static constexpr int nrad = 4; static constexpr int r[5] = {4, 4, 4, 4, 1}; };
+
+template <bool INV, int NL, int N, int R, int NS>
+__device__ __forceinline__ void stage_ct(const float2* __restrict__ in, float2* __restrict__ out, int lt,
+                                         const float2* __restrict__ tw) {
+    constexpr int nb = N / R, tpl = kFftBlock / NL, twstep = N / (NS * R);
+    for (int j = lt; j < nb; j += tpl) {
+        const int k = j % NS;
+        float2 v[4];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            float2 a = in[j + q * nb];
+            if (q > 0 && k > 0) a = cmul_tw<INV>(a, tw[(k * q * twstep) % N]);
+            v[q] = a;
+        }
+        dft_small<INV>(v, R);
+        const int dst = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) out[dst + q * NS] = v[q];
+    }
+}
+
+template <bool INV, int NL, int N, int ST, int NS>
+__device__ __forceinline__ int stages_ct(float2 (*buf)[NL * N], int cur, const float2* __restrict__ tw) {
+    if constexpr (ST == RadixPlan<N>::nrad) {
+        return cur;
+    } else {
+        constexpr int R = RadixPlan<N>::r[ST];
+        constexpr int tpl = kFftBlock / NL;
+        const int c = threadIdx.x / tpl, lt = threadIdx.x - c * tpl;
+        stage_ct<INV, NL, N, R, NS>(&buf[cur][c * N], &buf[cur ^ 1][c * N], lt, tw);
+        __syncthreads();
+        return stages_ct<INV, NL, N, ST + 1, NS * R>(buf, cur ^ 1, tw);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pruned convolution of zero-padded grids. Sources occupy nodes [0, m]^3 with
 // m = n/2 + 1 and only those nodes are read back, so each pass touches only
 // the lines that can be non-zero or are needed later, and zero-fills the rest
@@ -154,19 +200,19 @@ struct ConvDesc {
 // NL lines per workgroup (8 or 16: a row segment of 16 float2 is one 128 B
 // line). The z pass runs every listed slot in one workgroup: the spectrum G
 // of its lines is read once into registers and reused for all slots.
-template <int MODE, int NL>
+template <int MODE, int NL, int N>
 __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ data,
                                                         const unsigned long long* __restrict__ acc,
-                                                        int32_t n, const int32_t* __restrict__ slots,
-                                                        int32_t nslots,
+                                                        const int32_t* __restrict__ slots, int32_t nslots,
                                                         const int32_t* __restrict__ slot_halo,
                                                         const double* __restrict__ fscale,
-                                                        const float* __restrict__ G, Radices rad,
+                                                        const float* __restrict__ G,
                                                         const float2* __restrict__ tw, ConvDesc d) {
-    __shared__ float2 buf[2][NL * kMaxN];
-    constexpr int kPer = NL * kMaxN / kFftBlock;  // elements per thread at most
+    __shared__ float2 buf[2][NL * N];  // LDS sized for this grid: more workgroups per CU for small N
+    constexpr int total = NL * N;
+    constexpr int kPer = (total + kFftBlock - 1) / kFftBlock;  // elements per thread at most
     const AxisDesc& ax = d.ax;
-    const int64_t grid = (int64_t)n * n * n;
+    constexpr int64_t grid = (int64_t)N * N * N;
     const int32_t g = blockIdx.x;
     const int32_t i0 = g % ax.c0;
     const int32_t i1 = (g / ax.c0) % ax.c1;
@@ -175,9 +221,9 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
     const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
     const int line0 = ax.x_contig ? i0 * NL : 0;  // y of line 0 in the x passes
     const int tid = threadIdx.x;
-    const int total = NL * n;
     float gk[kPer];
     if (MODE == kConvZ) {
+#pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int idx = tid + k * kFftBlock;
             gk[k] = 0.0f;
@@ -193,7 +239,7 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
         if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
         for (int idx = tid; idx < total; idx += kFftBlock) {
             int c, e;
-            if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+            if (ax.x_contig) { c = idx / N; e = idx - c * N; }
             else { e = idx / NL; c = idx - e * NL; }
             float2 v = make_float2(0.0f, 0.0f);
             if (e <= d.in_lim && (!ax.x_contig || line0 + c <= d.line_lim)) {
@@ -201,61 +247,64 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
                 if (MODE == kConvXFwd) v = make_float2((float)((double)(long long)acc[o] * inv), 0.0f);
                 else v = data[o];
             }
-            buf[0][c * n + e] = v;
+            buf[0][c * N + e] = v;
         }
         __syncthreads();
         int cur;
         if (MODE == kConvXFwd || MODE == kConvYFwd) {
-            cur = stockham<false, NL>(buf, 0, n, rad, tw);
+            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tw);
         } else if (MODE == kConvZ) {
-            cur = stockham<false, NL>(buf, 0, n, rad, tw);
+            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tw);
+#pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int idx = tid + k * kFftBlock;
-                if (idx >= total) break;
-                const int e = idx / NL, c = idx - e * NL;
-                float2& v = buf[cur][c * n + e];
-                v = make_float2(v.x * gk[k], v.y * gk[k]);
+                if (idx < total) {
+                    const int e = idx / NL, c = idx - e * NL;
+                    float2& v = buf[cur][c * N + e];
+                    v = make_float2(v.x * gk[k], v.y * gk[k]);
+                }
             }
             __syncthreads();
-            cur = stockham<true, NL>(buf, cur, n, rad, tw);
+            cur = stages_ct<true, NL, N, 0, 1>(buf, cur, tw);
         } else {
-            cur = stockham<true, NL>(buf, 0, n, rad, tw);
+            cur = stages_ct<true, NL, N, 0, 1>(buf, 0, tw);
         }
         for (int idx = tid; idx < total; idx += kFftBlock) {
             int c, e;
-            if (ax.x_contig) { c = idx / n; e = idx - c * n; }
+            if (ax.x_contig) { c = idx / N; e = idx - c * N; }
             else { e = idx / NL; c = idx - e * NL; }
             if (e > d.out_lim || (ax.x_contig && line0 + c > d.line_lim)) continue;
-            data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * n + e];
+            data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * N + e];
         }
         __syncthreads();  // buf is reused by the next slot
     }
 }
 
-template <int NL>
-void conv_yz(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
-             int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
-             const Radices& rad, int m, int mc, int64_t n2, hipStream_t s, bool forward_y) {
-    const int n = p.n;
-    const AxisDesc ax_y{NL, n2, n / NL, mc, 1, n, 0};
-    const uint32_t gy = (uint32_t)((n / NL) * mc) * (uint32_t)nslots;
-    if (forward_y) {
-        const ConvDesc dyf{ax_y, n, m, n - 1};
-        k_fft_conv<kConvYFwd, NL><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
-                                                            rad, p.tw, dyf);
-        PYFOF_LAUNCH_CHECK();
-        const AxisDesc ax_z{NL, n, n / NL, n, 1, n2, 0};
-        const ConvDesc dz{ax_z, n, m, m};
-        k_fft_conv<kConvZ, NL><<<(uint32_t)((n / NL) * n), kFftBlock, 0, s>>>(grid, acc, n, slots, nslots,
-                                                                              slot_halo, fscale, G, rad,
-                                                                              p.tw, dz);
-        PYFOF_LAUNCH_CHECK();
-    } else {
-        const ConvDesc dyi{ax_y, n, n - 1, m};
-        k_fft_conv<kConvYInv, NL><<<gy, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
-                                                            rad, p.tw, dyi);
-        PYFOF_LAUNCH_CHECK();
-    }
+template <int N>
+void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
+                 int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G, hipStream_t s) {
+    constexpr int NLYZ = N % 16 == 0 ? 16 : kLines;  // 128 B row segments when N allows
+    constexpr int m = N / 2 + 1, mc = m + 1;
+    constexpr int64_t n2 = (int64_t)N * N;
+    constexpr int ygroups = (mc + kLines - 1) / kLines;
+    const AxisDesc ax_x{(int64_t)kLines * N, n2, ygroups, mc, N, 1, 1};
+    const AxisDesc ax_y{NLYZ, n2, N / NLYZ, mc, 1, N, 0};
+    const AxisDesc ax_z{NLYZ, N, N / NLYZ, N, 1, n2, 0};
+    const ConvDesc dxf{ax_x, m, m, N - 1}, dxi{ax_x, m, N - 1, m};
+    const ConvDesc dyf{ax_y, N, m, N - 1}, dyi{ax_y, N, N - 1, m}, dz{ax_z, N, m, m};
+    const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
+    const uint32_t gy = (uint32_t)((N / NLYZ) * mc) * (uint32_t)nslots;
+    const uint32_t gzz = (uint32_t)((N / NLYZ) * N);
+    k_fft_conv<kConvXFwd, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvYFwd, NLYZ, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvZ, NLYZ, N><<<gzz, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dz);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvYInv, NLYZ, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyi);
+    PYFOF_LAUNCH_CHECK();
+    k_fft_conv<kConvXInv, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxi);
+    PYFOF_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -308,28 +357,14 @@ void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, c
                 int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
                 hipStream_t s) {
     if (nslots <= 0) return;
-    const int n = p.n;
-    const int m = n / 2 + 1;
-    const int mc = m + 1;
-    Radices rad;
-    rad.nrad = p.nrad;
-    for (int i = 0; i < 16; ++i) rad.r[i] = p.radix[i];
-    const int64_t n2 = (int64_t)n * n;
-    const int ygroups = (mc + kLines - 1) / kLines;
-    const AxisDesc ax_x{(int64_t)kLines * n, n2, ygroups, mc, n, 1, 1};
-    const ConvDesc dxf{ax_x, m, m, n - 1}, dxi{ax_x, m, n - 1, m};
-    const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
-    k_fft_conv<kConvXFwd, kLines><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
-                                                             rad, p.tw, dxf);
-    PYFOF_LAUNCH_CHECK();
-    // y and z passes: 16 lines (128 B row segments) when n allows
-    if (n % 16 == 0) conv_yz<16>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, true);
-    else conv_yz<kLines>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, true);
-    if (n % 16 == 0) conv_yz<16>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, false);
-    else conv_yz<kLines>(p, grid, acc, slots, nslots, slot_halo, fscale, G, rad, m, mc, n2, s, false);
-    k_fft_conv<kConvXInv, kLines><<<gx, kFftBlock, 0, s>>>(grid, acc, n, slots, nslots, slot_halo, fscale, G,
-                                                             rad, p.tw, dxi);
-    PYFOF_LAUNCH_CHECK();
+    switch (p.n) {
+        case 48: conv_launch<48>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 64: conv_launch<64>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 128: conv_launch<128>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 216: conv_launch<216>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 256: conv_launch<256>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        default: throw HipError(hipErrorInvalidValue, "no convolution kernel for this grid size");
+    }
 }
 
 }  // namespace pyfof
