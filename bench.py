@@ -16,7 +16,7 @@ input. Reported as bound halos/s and particles/s.
 
 Prints one JSON line (see the keys below). The CPU baselines are the oracle
 (the C restatement of the reference) timed on bounded samples: FoF on the
-same generator at 2,097,152 particles in a 128^3 box (same density) with the
+same generator at 4,194,304 particles in a 161^3 box (same density) with the
 reference's thread rule (2 linking threads for one region); unbinding on the
 first halos of the same halo set.
 """
@@ -47,12 +47,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=134_217_728, help="particles")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-n", type=int, default=2_097_152, help="CPU baseline sample size")
+    ap.add_argument("--cpu-n", type=int, default=4_194_304, help="CPU baseline sample size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bound", action="store_true", help="skip the unbinding leg")
     ap.add_argument("--bound-halos", type=int, default=20_000)
     ap.add_argument("--bound-steps", type=int, default=3)
-    ap.add_argument("--bound-cpu-halos", type=int, default=2_000)
+    ap.add_argument("--bound-cpu-halos", type=int, default=4_000)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the plain / f32 / tuned unbinding forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
