@@ -325,3 +325,17 @@ def test_two_level_halo_agrees_with_oracle_and_direct_sum(app):
     ref_in = r["in_halo"].astype(bool)
     assert 0 < ref_in.sum() < n
     assert np.sum(inh != ref_in) <= 2e-3 * ref_in.sum()
+
+
+@pytest.mark.parametrize("np_min", [0, 1, 2, -3])
+def test_small_and_nonpositive_np_halo_min(np_min):
+    # np_halo_min <= 0 makes every CRS position a candidate, empty ones
+    # included (fof_fortran.f90:848-920); they end OK and stay valid
+    rng = np.random.default_rng(5)
+    xs, vs = [], []
+    for k in range(3):
+        xp, _ = chain_halo(30 + 10 * k, B, 20, rng)
+        xs.append(np.asfortranarray(xp + np.array([80.0 * k, 0.0, 0.0])[:, None]))
+        vs.append(np.asfortranarray(rng.standard_normal(xp.shape) * 0.1))
+    inh, hid, am, _ = _gpu(xs, vs, np_halo_min=np_min)
+    _same_as_oracle(inh, hid, am, _oracle(xs, vs, np_min=np_min))
