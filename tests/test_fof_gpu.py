@@ -204,3 +204,33 @@ def test_repeat_calls_identical():
     b = py_fof_amd._fof_core(xp, L_Box=36.0, linking_length=0.2, _want_roots=True)
     assert np.array_equal(a[3]["roots"], b[3]["roots"])
     assert np.array_equal(a[1], b[1])
+
+
+def _near_threshold_pairs(rng, b, npairs, eps, origin, spacing):
+    """Pairs of particles at separations b (1 + u), u uniform in [-eps, eps],
+    in random directions, the pairs spaced far apart on a lattice."""
+    k = int(np.ceil(npairs ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), 0).reshape(3, -1)[:, :npairs]
+    p = origin + spacing * g.astype(np.float64)
+    d = rng.normal(size=(3, npairs))
+    d /= np.linalg.norm(d, axis=0)
+    q = p + d * (b * (1 + rng.uniform(-eps, eps, npairs)))
+    return np.asfortranarray(np.concatenate([p, q], axis=1))
+
+
+def test_near_threshold_pairs_bit_exact():
+    # pairs placed at b (1 +/- eps): both outcomes of the inclusive f64
+    # comparison occur, down to separations a few ulps of f32 from b
+    rng = np.random.default_rng(2024)
+    for b, eps in ((0.2, 3e-6), (0.2, 3e-7), (1.7, 3e-6), (0.013, 2e-6)):
+        xp = _near_threshold_pairs(rng, b, 20000, eps, 3 * b, 3 * b)
+        L = float(np.ceil(xp.max() + 3 * b))
+        g, r = _run_both([xp], L, b, 2)
+        _check(g, r)
+        n = xp.shape[1] // 2
+        linked = int(np.sum(r[3][:n] == r[3][n:]))
+        assert 0 < linked < n, linked  # both outcomes occur
+    # a small periodic box where most pairs are near the minimum-image wrap
+    xp = np.asfortranarray(rng.uniform(0.0, 2.0, (3, 4000)))
+    g, r = _run_both([xp], 2.0, 0.3, 2)
+    _check(g, r)
