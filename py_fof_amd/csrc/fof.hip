@@ -256,6 +256,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
 struct LinkCtx {
     int32_t* parent;
     int32_t rp;  // last known root of the thread's particle
+    uint32_t dmask;  // neighbour offsets (bit o) whose dense cells are left to k_link_dense
     __device__ __forceinline__ void link(int32_t q) {
         const int32_t rq = dsu_find(parent, q);
         if (rq != rp) rp = dsu_unite(parent, rp, rq);
@@ -268,63 +269,90 @@ __device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, co
     return pred_flat(a, q, P.rp2);
 }
 
-// Tests the particle `me` (position p) against cell jc. For the own cell only
-// partners q > p are tested. Pruning (pr) is exact: a pair passing the
-// predicate is within b (+ float rounding < m) per axis, so its partner lies
-// inside the query box [x - R, x + R] with R = b + m; the cell and sub-cell
-// coordinates used to prune are monotone in x.
+// Tests the particle `me` (position p) against cell jc, reached by neighbour
+// offset o (13: its own cell). For the own cell only partners q > p are
+// tested. Pruning is exact: a pair passing the predicate is within b (+ float
+// rounding < m) per axis, so its partner lies inside the query box
+// [x - R, x + R] with R = b + m; the cell and sub-cell coordinates used to
+// prune are monotone in x. A cell with sub-cells (dense) is not scanned here:
+// bit o of the thread's mask is set and k_link_dense scans it, which keeps the
+// register footprint of this kernel, and so its occupancy, low.
 template <bool PERIODIC>
-__device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, const int32_t jijk[3],
-                                           const double q[3], bool own, const RegionDev& g,
-                                           const LinkParams& P, const uint32_t* __restrict__ cell_ptr,
+__device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
+                                           const int32_t jijk[3], const double q[3], bool own,
+                                           const RegionDev& g, const LinkParams& P,
+                                           const uint32_t* __restrict__ cell_ptr,
                                            const int32_t* __restrict__ aux,
-                                           const uint32_t* __restrict__ sub_ptr,
-                                           const uint32_t* __restrict__ sub_list,
                                            const float4* __restrict__ xc, LinkCtx& L) {
-    double lo[3];
-    for (int d = 0; d < 3; ++d) lo[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
     if (g.prune && !own) {
         for (int d = 0; d < 3; ++d) {
-            if (q[d] + P.R < lo[d] - P.m || q[d] - P.R > lo[d] + g.cell[d] + P.m) return;
+            const double lo = g.origin[d] + (double)jijk[d] * g.cell[d];
+            if (q[d] + P.R < lo - P.m || q[d] - P.R > lo + g.cell[d] + P.m) return;
         }
     }
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
     if (s >= e) return;
+    if (aux[jc] >= 0) {
+        L.dmask |= 1u << o;
+        return;
+    }
+    for (uint32_t qq = own ? p + 1 : s; qq < e; ++qq) {
+        if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
+    }
+}
+
+// Scan of a dense cell jc (one with sub-cells) for the particle `me`: only
+// the sub-cells inside the query box; in a clique region one link per
+// sub-cell suffices, and none if the cell is one component that already
+// shares this particle's root (one link then joins all of it).
+template <bool PERIODIC>
+__device__ __forceinline__ void visit_dense(const float4& me, uint32_t p, uint32_t jc, const int32_t jijk[3],
+                                            const double q[3], bool own, const RegionDev& g,
+                                            const LinkParams& P, const uint32_t* __restrict__ cell_ptr,
+                                            const int32_t* __restrict__ aux,
+                                            const uint32_t* __restrict__ sub_ptr,
+                                            const uint32_t* __restrict__ sub_list,
+                                            const int32_t* __restrict__ ncomp,
+                                            const int32_t* __restrict__ croot,
+                                            const float4* __restrict__ xc, LinkCtx& L) {
+    double lo[3];
+    for (int d = 0; d < 3; ++d) lo[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
+    const uint32_t s = cell_ptr[jc];
     const int32_t dcell = aux[jc];
-    if (dcell >= 0) {
-        const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
-        int32_t a[3], bnd[3], mine[3] = {-1, -1, -1};
-        const int32_t kk[3] = {k0, k1, k2};
-        for (int d = 0; d < 3; ++d) {
-            a[d] = sub_coord(q[d] - P.R, lo[d], g.sub[d], kk[d]);
-            bnd[d] = sub_coord(q[d] + P.R, lo[d], g.sub[d], kk[d]);
-        }
-        if (own && g.clique) {
-            mine[0] = sub_coord((double)me.x, lo[0], g.sub[0], k0);
-            mine[1] = sub_coord((double)me.y, lo[1], g.sub[1], k1);
-            mine[2] = sub_coord((double)me.z, lo[2], g.sub[2], k2);
-        }
-        const uint32_t* row0 = sub_ptr + (size_t)dcell * P.sub_stride;
-        for (int32_t sz = a[2]; sz <= bnd[2]; ++sz) {
-            for (int32_t sy = a[1]; sy <= bnd[1]; ++sy) {
-                const uint32_t* row = row0 + (size_t)k0 * (sy + k1 * sz);
-                for (int32_t sx = a[0]; sx <= bnd[0]; ++sx) {
-                    if (own && g.clique && sx == mine[0] && sy == mine[1] && sz == mine[2]) continue;
-                    const uint32_t t0 = row[sx], t1 = row[sx + 1];
-                    for (uint32_t t = t0; t < t1; ++t) {
-                        const uint32_t qq = sub_list[s + t];
-                        if (own && qq <= p) continue;
-                        if (pair_linked<PERIODIC>(me, xc[qq], P)) {
-                            L.link((int32_t)qq);
-                            if (g.clique) break;
-                        }
+    const bool single = g.clique && !own && ncomp[dcell] == 1;
+    if (single) {
+        L.rp = dsu_find(L.parent, L.rp);
+        if (dsu_find(L.parent, croot[dcell]) == L.rp) return;
+    }
+    const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
+    int32_t a[3], bnd[3], mine[3] = {-1, -1, -1};
+    const int32_t kk[3] = {k0, k1, k2};
+    for (int d = 0; d < 3; ++d) {
+        a[d] = sub_coord(q[d] - P.R, lo[d], g.sub[d], kk[d]);
+        bnd[d] = sub_coord(q[d] + P.R, lo[d], g.sub[d], kk[d]);
+    }
+    if (own && g.clique) {
+        mine[0] = sub_coord((double)me.x, lo[0], g.sub[0], k0);
+        mine[1] = sub_coord((double)me.y, lo[1], g.sub[1], k1);
+        mine[2] = sub_coord((double)me.z, lo[2], g.sub[2], k2);
+    }
+    const uint32_t* row0 = sub_ptr + (size_t)dcell * P.sub_stride;
+    for (int32_t sz = a[2]; sz <= bnd[2]; ++sz) {
+        for (int32_t sy = a[1]; sy <= bnd[1]; ++sy) {
+            const uint32_t* row = row0 + (size_t)k0 * (sy + k1 * sz);
+            for (int32_t sx = a[0]; sx <= bnd[0]; ++sx) {
+                if (own && g.clique && sx == mine[0] && sy == mine[1] && sz == mine[2]) continue;
+                const uint32_t t0 = row[sx], t1 = row[sx + 1];
+                for (uint32_t t = t0; t < t1; ++t) {
+                    const uint32_t qq = sub_list[s + t];
+                    if (own && qq <= p) continue;
+                    if (pair_linked<PERIODIC>(me, xc[qq], P)) {
+                        L.link((int32_t)qq);
+                        if (single) return;
+                        if (g.clique) break;
                     }
                 }
             }
-        }
-    } else {
-        for (uint32_t qq = own ? p + 1 : s; qq < e; ++qq) {
-            if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
         }
     }
 }
@@ -353,67 +381,156 @@ __device__ __forceinline__ uint32_t local_cell(const RegionDev& g, const int32_t
 }
 
 template <bool PERIODIC>
-__device__ __forceinline__ void link_particle(uint32_t p, const float4& me, const RegionDev& g,
-                                              uint32_t c, const LinkParams& P,
-                                              const uint32_t* __restrict__ cell_ptr,
-                                              const int32_t* __restrict__ aux,
-                                              const uint32_t* __restrict__ sub_ptr,
-                                              const uint32_t* __restrict__ sub_list,
-                                              const float4* __restrict__ xc, int32_t* parent) {
-    LinkCtx L{parent, dsu_find(parent, (int32_t)p)};
+__device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, const RegionDev& g,
+                                                  uint32_t c, const LinkParams& P,
+                                                  const uint32_t* __restrict__ cell_ptr,
+                                                  const int32_t* __restrict__ aux,
+                                                  const float4* __restrict__ xc, int32_t* parent) {
+    LinkCtx L{parent, (int32_t)p, 0u};  // rp becomes a root at the first unite
     const uint32_t lc = c - g.cell_off;
     int32_t ijk[3];
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    visit_cell<PERIODIC>(me, p, c, ijk, x0, true, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, L);
-    for (int o = 0; o < 27; ++o) {
-        if (o == 13) continue;
+    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, g, P, cell_ptr, aux, xc, L);
+    auto neighbour = [&](int o) {
         int32_t jijk[3];
         double sh[3];
-        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
+        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) return;
         const uint32_t jl = local_cell(g, jijk);
         // sparse-sparse pairs once (forward cells); every pair with a dense
         // clique cell is taken here, from the sparse side, in both directions
         const bool fwd = jl > lc;
-        if (!fwd && !g.clique) continue;
-        if (!fwd && aux[g.cell_off + jl] < 0) continue;
+        if (!fwd && !g.clique) return;
+        if (!fwd && aux[g.cell_off + jl] < 0) return;
         if (!g.prune) {
             // tiny periodic grids: the same neighbour can be reached twice
-            bool dup = false;
-            for (int o2 = 0; o2 < o && !dup; ++o2) {
+            for (int o2 = 0; o2 < o; ++o2) {
                 if (o2 == 13) continue;
                 int32_t j2[3];
                 double s2[3];
-                if (neighbour_cell(g, ijk, o2, j2, s2, P.L) && local_cell(g, j2) == jl) dup = true;
+                if (neighbour_cell(g, ijk, o2, j2, s2, P.L) && local_cell(g, j2) == jl) return;
             }
-            if (dup) continue;
         }
         const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_cell<PERIODIC>(me, p, g.cell_off + jl, jijk, q, false, g, P, cell_ptr, aux, sub_ptr,
-                             sub_list, xc, L);
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, g, P, cell_ptr, aux, xc, L);
+    };
+    if (!g.prune) {
+        for (int o = 0; o < 27; ++o)
+            if (o != 13) neighbour(o);
+        return L.dmask;
     }
+    // Offsets along each axis whose neighbour the query box [x - R, x + R] can
+    // reach: a superset of what the exact test in visit_cell keeps (its corners
+    // differ from these by f64 rounding, far below the extra margin m), so only
+    // about (1 + 2R / cell)^3 - 1 neighbours are visited instead of 26. A
+    // neighbour across a periodic edge is always taken and tested there.
+    int32_t olo[3], ohi[3];
+    for (int d = 0; d < 3; ++d) {
+        const double c0 = g.origin[d] + (double)ijk[d] * g.cell[d];
+        const bool wrap_lo = g.periodic[d] && ijk[d] == 0;
+        const bool wrap_hi = g.periodic[d] && ijk[d] == g.n[d] - 1;
+        olo[d] = (wrap_lo || x0[d] - P.R <= c0 + 2.0 * P.m) ? -1 : 0;
+        ohi[d] = (wrap_hi || x0[d] + P.R >= c0 + g.cell[d] - 2.0 * P.m) ? 1 : 0;
+    }
+    for (int oz = olo[2]; oz <= ohi[2]; ++oz)
+        for (int oy = olo[1]; oy <= ohi[1]; ++oy)
+            for (int ox = olo[0]; ox <= ohi[0]; ++ox)
+                if (ox != 0 || oy != 0 || oz != 0) neighbour((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1));
+    return L.dmask;
 }
 
-__global__ void __launch_bounds__(kBlk) k_link(const float4* __restrict__ xc,
+// One thread per particle outside dense clique cells: pairs with sparse cells
+// (own cell and forward neighbours), and the mask of dense cells to scan,
+// written for every particle (0 when none).
+// Occupancy targets (6 and 5 waves per SIMD) chosen by measurement on the
+// 134M-particle clump set: both kernels wait on chains of dependent loads and
+// DSU atomics, so more resident waves beat the few registers spilled.
+__global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
                                                const uint32_t* __restrict__ skeys,
                                                const uint32_t* __restrict__ cell_ptr,
                                                const int32_t* __restrict__ aux,
-                                               const uint32_t* __restrict__ sub_ptr,
-                                               const uint32_t* __restrict__ sub_list,
                                                const RegionDev* __restrict__ reg, int32_t* parent,
-                                               LinkParams P) {
+                                               uint32_t* __restrict__ dmask, LinkParams P) {
     const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
     if (p >= P.N) return;
     const float4 me = xc[p];
+    const int32_t rid = __float_as_int(me.w);
+    uint32_t mask = 0;
+    // One pass per distinct region in the wave (almost always one): the
+    // region record is then wave-uniform and read with scalar loads.
+    bool pending = true;
+    while (pending) {
+        const int32_t r0 = __builtin_amdgcn_readfirstlane(rid);
+        if (rid != r0) continue;
+        pending = false;
+        const RegionDev& g = reg[r0];
+        if (g.skip) break;
+        const uint32_t c = skeys[p];
+        if (g.clique && aux[c] >= 0) break;  // dense clique cells: k_dense_* kernels
+        mask = g.is_periodic ? link_particle<true>(p, me, g, c, P, cell_ptr, aux, xc, parent)
+                             : link_particle<false>(p, me, g, c, P, cell_ptr, aux, xc, parent);
+    }
+    dmask[p] = mask;
+}
+
+template <bool PERIODIC>
+__device__ __forceinline__ void link_dense_cells(uint32_t p, const float4& me, uint32_t mask,
+                                                 const RegionDev& g, uint32_t c, const LinkParams& P,
+                                                 const uint32_t* __restrict__ cell_ptr,
+                                                 const int32_t* __restrict__ aux,
+                                                 const uint32_t* __restrict__ sub_ptr,
+                                                 const uint32_t* __restrict__ sub_list,
+                                                 const int32_t* __restrict__ ncomp,
+                                                 const int32_t* __restrict__ croot,
+                                                 const float4* __restrict__ xc, int32_t* parent) {
+    LinkCtx L{parent, (int32_t)p, 0u};
+    int32_t ijk[3];
+    double lo_unused[3];
+    cell_lower_corner(g, c - g.cell_off, lo_unused, ijk);
+    const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
+    while (mask) {
+        const int o = __builtin_ctz(mask);
+        mask &= mask - 1;
+        if (o == 13) {
+            visit_dense<PERIODIC>(me, p, c, ijk, x0, true, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp,
+                                  croot, xc, L);
+            continue;
+        }
+        int32_t jijk[3];
+        double sh[3];
+        neighbour_cell(g, ijk, o, jijk, sh, P.L);  // exists: k_link reached it
+        const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
+        visit_dense<PERIODIC>(me, p, g.cell_off + local_cell(g, jijk), jijk, q, false, g, P, cell_ptr, aux,
+                              sub_ptr, sub_list, ncomp, croot, xc, L);
+    }
+}
+
+// The dense-cell scans left by k_link, one thread per particle with a
+// non-empty mask.
+__global__ void __launch_bounds__(kBlk, 5) k_link_dense(const float4* __restrict__ xc,
+                                                     const uint32_t* __restrict__ skeys,
+                                                     const uint32_t* __restrict__ cell_ptr,
+                                                     const int32_t* __restrict__ aux,
+                                                     const uint32_t* __restrict__ sub_ptr,
+                                                     const uint32_t* __restrict__ sub_list,
+                                                     const int32_t* __restrict__ ncomp,
+                                                     const int32_t* __restrict__ croot,
+                                                     const RegionDev* __restrict__ reg, int32_t* parent,
+                                                     const uint32_t* __restrict__ dmask, LinkParams P) {
+    const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
+    if (p >= P.N) return;
+    const uint32_t mask = dmask[p];
+    if (mask == 0) return;
+    const float4 me = xc[p];
     const RegionDev& g = reg[__float_as_int(me.w)];
-    if (g.skip) return;
     const uint32_t c = skeys[p];
-    if (g.clique && aux[c] >= 0) return;  // handled per sub-cell by k_link_dense
     if (g.is_periodic)
-        link_particle<true>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
+        link_dense_cells<true>(p, me, mask, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc,
+                               parent);
     else
-        link_particle<false>(p, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent);
+        link_dense_cells<false>(p, me, mask, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc,
+                                parent);
 }
 
 // Linking of dense cells whose sub-cells are cliques.
@@ -1224,8 +1341,16 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                     sub_list, xc, parent, ncomp, croot, needs_sub, P);
             PYFOF_LAUNCH_CHECK();
         }
-        k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, d_reg, parent, P);
+        // per-particle masks of dense cells to scan, in the buffer that holds
+        // the labels after linking
+        uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
+        k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
+        if (ndense > 0) {
+            k_link_dense<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
+                                                  d_reg, parent, dmask, P);
+            PYFOF_LAUNCH_CHECK();
+        }
         if (nsubs > 0) {
             k_dense_cell_cross<<<ndense, kWave, 0, s>>>(dense_cells, ndense, cell_ptr, d_coff, d_reg, H,
                                                         aux, ncomp, croot, needs_sub, sub_ptr, sub_list,

@@ -42,6 +42,17 @@ def _run_both(xps, L, b, np_min, vps=None, mps=None, am=False):
     return (in_h, hid, ang, ex), (o_in, o_hid, o_am, o_lab)
 
 
+def _dense_cells(xps, L, b):
+    """Dense cells (with sub-cells) the library used for this input."""
+    from py_fof_amd import _native
+    ctx = _native.Context(0)
+    try:
+        py_fof_amd._fof_core(xps, L_Box=L, linking_length=b, np_halo_min=10, _ctx=ctx)
+        return ctx.stats()["ndense"]
+    finally:
+        ctx.close()
+
+
 def _check(gpu, ref, am=False):
     (in_h, hid, ang, ex), (o_in, o_hid, o_am, o_lab) = gpu, ref
     assert np.array_equal(ex["roots"], o_lab), "partition roots differ"
@@ -98,6 +109,16 @@ def test_clumpy_plummer_262k_bit_exact():
     _check(g, r)
 
 
+def test_clumpy_nfw_2m_bit_exact():
+    # the benchmark's generator at 2,097,152 particles and the same density
+    # (L = 128): sparse cells, dense cells of one and of several components,
+    # and the sparse-dense interface between them
+    xp = synth.clumpy_nfw(n=2_097_152, L=128.0, seed=2)
+    g, r = _run_both([xp], 128.0, 0.2, 10)
+    _check(g, r)
+    assert _dense_cells([xp], 128.0, 0.2) > 100
+
+
 def test_dense_clump_subcells_and_angmom():
     # one very dense clump: dense cells, clique sub-cells; velocities + masses
     rng = np.random.default_rng(21)
@@ -109,18 +130,21 @@ def test_dense_clump_subcells_and_angmom():
     g, r = _run_both([np.asfortranarray(x)], 512.0, 0.05, 10, vps=[np.asfortranarray(v)],
                      mps=[m], am=True)
     _check(g, r, am=True)
+    assert _dense_cells([np.asfortranarray(x)], 512.0, 0.05) > 0
 
 
 def test_dense_clique_subcells_nonperiodic_and_coarse_k():
     rng = np.random.default_rng(31)
     # isolated dense blob: L_mesh clamps to b, two clique sub-cells per axis
-    x = 200.0 + 0.5 * rng.standard_normal((3, 20000))
+    x = 200.0 + 0.35 * rng.standard_normal((3, 20000))
     g, r = _run_both([np.asfortranarray(x)], 512.0, 0.2, 10)
     _check(g, r)
+    assert _dense_cells([np.asfortranarray(x)], 512.0, 0.2) > 0
     # periodic clumps with b = 0.35 in unit cells (k = 5 sub-cells per axis)
     xp = synth.clumpy_plummer(n=65536, L=40.0, n_clumps=50, a=0.5, seed=12)
     g, r = _run_both([xp], 40.0, 0.35, 10)
     _check(g, r)
+    assert _dense_cells([xp], 40.0, 0.35) > 0
 
 
 def test_nonperiodic_cells_equal_to_b():
