@@ -296,7 +296,22 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
         L.dmask |= 1u << o;
         return;
     }
-    for (uint32_t qq = own ? p + 1 : s; qq < e; ++qq) {
+    uint32_t qq = own ? p + 1 : s;
+    // two partners at a time: their positions, and then the parents of the
+    // linked ones, are loaded together; a partner whose parent is already the
+    // thread's known root needs no union (pairs beat fours here: the wider
+    // batch spills registers at this kernel's occupancy target)
+    for (; qq + 2 <= e; qq += 2) {
+        const float4 y0 = xc[qq], y1 = xc[qq + 1];
+        const bool h0 = pair_linked<PERIODIC>(me, y0, P), h1 = pair_linked<PERIODIC>(me, y1, P);
+        if (!(h0 || h1)) continue;
+        const int32_t q0 = (int32_t)qq;
+        const int32_t a0 = h0 ? atomic_load_relaxed(&L.parent[q0]) : 0;
+        const int32_t a1 = h1 ? atomic_load_relaxed(&L.parent[q0 + 1]) : 0;
+        if (h0 && a0 != L.rp) L.link(q0);
+        if (h1 && a1 != L.rp) L.link(q0 + 1);
+    }
+    for (; qq < e; ++qq) {
         if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
     }
 }
