@@ -316,62 +316,6 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
     }
 }
 
-// Scan of a dense cell jc (one with sub-cells) for the particle `me`: only
-// the sub-cells inside the query box; in a clique region one link per
-// sub-cell suffices, and none if the cell is one component that already
-// shares this particle's root (one link then joins all of it).
-template <bool PERIODIC>
-__device__ __forceinline__ void visit_dense(const float4& me, uint32_t p, uint32_t jc, const int32_t jijk[3],
-                                            const double q[3], bool own, const RegionDev& g,
-                                            const LinkParams& P, const uint32_t* __restrict__ cell_ptr,
-                                            const int32_t* __restrict__ aux,
-                                            const uint32_t* __restrict__ sub_ptr,
-                                            const uint32_t* __restrict__ sub_list,
-                                            const int32_t* __restrict__ ncomp,
-                                            const int32_t* __restrict__ croot,
-                                            const float4* __restrict__ xc, LinkCtx& L) {
-    double lo[3];
-    for (int d = 0; d < 3; ++d) lo[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
-    const uint32_t s = cell_ptr[jc];
-    const int32_t dcell = aux[jc];
-    const bool single = g.clique && !own && ncomp[dcell] == 1;
-    if (single) {
-        L.rp = dsu_find(L.parent, L.rp);
-        if (dsu_find(L.parent, croot[dcell]) == L.rp) return;
-    }
-    const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
-    int32_t a[3], bnd[3], mine[3] = {-1, -1, -1};
-    const int32_t kk[3] = {k0, k1, k2};
-    for (int d = 0; d < 3; ++d) {
-        a[d] = sub_coord(q[d] - P.R, lo[d], g.sub[d], kk[d]);
-        bnd[d] = sub_coord(q[d] + P.R, lo[d], g.sub[d], kk[d]);
-    }
-    if (own && g.clique) {
-        mine[0] = sub_coord((double)me.x, lo[0], g.sub[0], k0);
-        mine[1] = sub_coord((double)me.y, lo[1], g.sub[1], k1);
-        mine[2] = sub_coord((double)me.z, lo[2], g.sub[2], k2);
-    }
-    const uint32_t* row0 = sub_ptr + (size_t)dcell * P.sub_stride;
-    for (int32_t sz = a[2]; sz <= bnd[2]; ++sz) {
-        for (int32_t sy = a[1]; sy <= bnd[1]; ++sy) {
-            const uint32_t* row = row0 + (size_t)k0 * (sy + k1 * sz);
-            for (int32_t sx = a[0]; sx <= bnd[0]; ++sx) {
-                if (own && g.clique && sx == mine[0] && sy == mine[1] && sz == mine[2]) continue;
-                const uint32_t t0 = row[sx], t1 = row[sx + 1];
-                for (uint32_t t = t0; t < t1; ++t) {
-                    const uint32_t qq = sub_list[s + t];
-                    if (own && qq <= p) continue;
-                    if (pair_linked<PERIODIC>(me, xc[qq], P)) {
-                        L.link((int32_t)qq);
-                        if (single) return;
-                        if (g.clique) break;
-                    }
-                }
-            }
-        }
-    }
-}
-
 // Neighbour cell of (ijk + off) under the reference's rule (fof_fortran.f90:
 // 729-755 periodic, 684-704 non-periodic): wrap only along periodic axes,
 // keep jcell > icell. Returns false if the neighbour does not exist. shift
@@ -489,41 +433,92 @@ __global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
     dmask[p] = mask;
 }
 
+// One entry of a mask: particle p (wave-uniform) against the dense cell at
+// neighbour offset o (13: its own cell). The sub-cells inside the query box
+// are spread over the wave's lanes, one sub-cell per lane. In a clique region
+// a sub-cell needs one link at most; a cell that is one component needs none
+// if it already shares p's root, else one link joins all of it. In other
+// regions (sub-cells only prune) every linked pair is joined.
 template <bool PERIODIC>
-__device__ __forceinline__ void link_dense_cells(uint32_t p, const float4& me, uint32_t mask,
-                                                 const RegionDev& g, uint32_t c, const LinkParams& P,
-                                                 const uint32_t* __restrict__ cell_ptr,
-                                                 const int32_t* __restrict__ aux,
-                                                 const uint32_t* __restrict__ sub_ptr,
-                                                 const uint32_t* __restrict__ sub_list,
-                                                 const int32_t* __restrict__ ncomp,
-                                                 const int32_t* __restrict__ croot,
-                                                 const float4* __restrict__ xc, int32_t* parent) {
-    LinkCtx L{parent, (int32_t)p, 0u};
-    int32_t ijk[3];
-    double lo_unused[3];
-    cell_lower_corner(g, c - g.cell_off, lo_unused, ijk);
+__device__ void dense_entry(uint32_t p, int o, const float4& me, const RegionDev& g, uint32_t c,
+                            const LinkParams& P, const uint32_t* __restrict__ cell_ptr,
+                            const int32_t* __restrict__ aux, const uint32_t* __restrict__ sub_ptr,
+                            const uint32_t* __restrict__ sub_list, const int32_t* __restrict__ ncomp,
+                            const int32_t* __restrict__ croot, const float4* __restrict__ xc,
+                            int32_t* parent) {
+    const int lane = __lane_id();
+    int32_t ijk[3], jijk[3];
+    double lo[3], q[3];
+    cell_lower_corner(g, c - g.cell_off, lo, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    while (mask) {
-        const int o = __builtin_ctz(mask);
-        mask &= mask - 1;
-        if (o == 13) {
-            visit_dense<PERIODIC>(me, p, c, ijk, x0, true, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp,
-                                  croot, xc, L);
-            continue;
-        }
-        int32_t jijk[3];
+    const bool own = o == 13;
+    uint32_t jc = c;
+    if (own) {
+        for (int d = 0; d < 3; ++d) { jijk[d] = ijk[d]; q[d] = x0[d]; }
+    } else {
         double sh[3];
         neighbour_cell(g, ijk, o, jijk, sh, P.L);  // exists: k_link reached it
-        const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_dense<PERIODIC>(me, p, g.cell_off + local_cell(g, jijk), jijk, q, false, g, P, cell_ptr, aux,
-                              sub_ptr, sub_list, ncomp, croot, xc, L);
+        for (int d = 0; d < 3; ++d) {
+            q[d] = x0[d] + sh[d];
+            lo[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
+        }
+        jc = g.cell_off + local_cell(g, jijk);
+    }
+    const uint32_t s = cell_ptr[jc];
+    const int32_t dcell = aux[jc];
+    const bool single = g.clique && !own && ncomp[dcell] == 1;
+    if (single && dsu_find(parent, croot[dcell]) == dsu_find(parent, (int32_t)p)) return;
+    const int32_t kk[3] = {g.k[0], g.k[1], g.k[2]};
+    int32_t a[3], nb[3], mine[3] = {-1, -1, -1};
+    for (int d = 0; d < 3; ++d) {
+        a[d] = sub_coord(q[d] - P.R, lo[d], g.sub[d], kk[d]);
+        nb[d] = sub_coord(q[d] + P.R, lo[d], g.sub[d], kk[d]) - a[d] + 1;
+    }
+    if (own && g.clique) {
+        mine[0] = sub_coord((double)me.x, lo[0], g.sub[0], kk[0]);
+        mine[1] = sub_coord((double)me.y, lo[1], g.sub[1], kk[1]);
+        mine[2] = sub_coord((double)me.z, lo[2], g.sub[2], kk[2]);
+    }
+    const uint32_t total = (uint32_t)(nb[0] * nb[1] * nb[2]);
+    const uint32_t* row0 = sub_ptr + (size_t)dcell * P.sub_stride;
+    LinkCtx L{parent, (int32_t)p, 0u};
+    for (uint32_t base = 0; base < total; base += kWave) {
+        const uint32_t i = base + (uint32_t)lane;
+        int32_t hit = -1;
+        if (i < total) {
+            const int32_t sx = a[0] + (int32_t)(i % (uint32_t)nb[0]);
+            const int32_t sy = a[1] + (int32_t)((i / (uint32_t)nb[0]) % (uint32_t)nb[1]);
+            const int32_t sz = a[2] + (int32_t)(i / (uint32_t)(nb[0] * nb[1]));
+            if (!(sx == mine[0] && sy == mine[1] && sz == mine[2])) {
+                const uint32_t* row = row0 + (size_t)kk[0] * (uint32_t)(sy + kk[1] * sz);
+                const uint32_t t0 = row[sx], t1 = row[sx + 1];
+                for (uint32_t t = t0; t < t1; ++t) {
+                    const uint32_t qq = sub_list[s + t];
+                    if (own && qq <= p) continue;
+                    if (pair_linked<PERIODIC>(me, xc[qq], P)) {
+                        if (g.clique) { hit = (int32_t)qq; break; }
+                        L.link((int32_t)qq);
+                    }
+                }
+            }
+        }
+        if (!g.clique) continue;
+        const uint64_t hits = __ballot(hit >= 0);
+        if (single) {
+            if (hits) {
+                if (lane == __ffsll((long long)hits) - 1) L.link(hit);
+                return;
+            }
+        } else if (hit >= 0) {
+            L.link(hit);
+        }
     }
 }
 
-// The dense-cell scans left by k_link, one thread per particle with a
-// non-empty mask.
-__global__ void __launch_bounds__(kBlk, 5) k_link_dense(const float4* __restrict__ xc,
+// The dense-cell scans left by k_link. Each wave takes the entries (particle,
+// offset) of its 64 particles one at a time, lowest lane and offset first,
+// and scans each with all its lanes.
+__global__ void __launch_bounds__(kBlk) k_link_dense(const float4* __restrict__ xc,
                                                      const uint32_t* __restrict__ skeys,
                                                      const uint32_t* __restrict__ cell_ptr,
                                                      const int32_t* __restrict__ aux,
@@ -533,19 +528,25 @@ __global__ void __launch_bounds__(kBlk, 5) k_link_dense(const float4* __restrict
                                                      const int32_t* __restrict__ croot,
                                                      const RegionDev* __restrict__ reg, int32_t* parent,
                                                      const uint32_t* __restrict__ dmask, LinkParams P) {
-    const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
-    if (p >= P.N) return;
-    const uint32_t mask = dmask[p];
-    if (mask == 0) return;
-    const float4 me = xc[p];
-    const RegionDev& g = reg[__float_as_int(me.w)];
-    const uint32_t c = skeys[p];
-    if (g.is_periodic)
-        link_dense_cells<true>(p, me, mask, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc,
-                               parent);
-    else
-        link_dense_cells<false>(p, me, mask, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc,
-                                parent);
+    const uint32_t pl = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
+    uint32_t mask = pl < P.N ? dmask[pl] : 0u;
+    const int lane = __lane_id();
+    for (;;) {
+        const uint64_t todo = __ballot(mask != 0u);
+        if (todo == 0) break;
+        const int src = __ffsll((long long)todo) - 1;
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pl, src);
+        const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)mask, src);
+        const int o = __builtin_ctz(m);
+        if (lane == src) mask &= mask - 1u;
+        const float4 me = xc[p];
+        const RegionDev& g = reg[__float_as_int(me.w)];
+        const uint32_t c = skeys[p];
+        if (g.is_periodic)
+            dense_entry<true>(p, o, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent);
+        else
+            dense_entry<false>(p, o, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent);
+    }
 }
 
 // Linking of dense cells whose sub-cells are cliques.
