@@ -967,7 +967,7 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
                         const uint32_t* __restrict__ flag, const uint32_t* __restrict__ cand_idx,
                         const uint32_t* __restrict__ perm, const float4* __restrict__ xc,
                         const RegionDev* __restrict__ reg, uint32_t n, int32_t np_min,
-                        int32_t* __restrict__ in_halo, int32_t* __restrict__ halo_id,
+                        int32_t* __restrict__ halo_id,
                         int32_t* __restrict__ root_out, uint32_t* __restrict__ anchor,
                         int32_t* __restrict__ hregion) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -975,14 +975,19 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     const int32_t r = label[p];
     const bool cand = (int64_t)size[r] >= (int64_t)np_min;
     const uint32_t i = perm[p];
-    in_halo[i] = cand ? 1 : 0;
+    // the one scattered write per particle; in_halo follows from halo_id in
+    // a coalesced pass (k_in_halo)
     halo_id[i] = cand ? (int32_t)cand_idx[r] + 1 : 0;
-    const int32_t regid = __float_as_int(xc[p].w);
-    if (root_out) root_out[i] = reg[regid].skip ? -1 : (int32_t)perm[r];
+    if (root_out) root_out[i] = reg[__float_as_int(xc[p].w)].skip ? -1 : (int32_t)perm[r];
     if (flag[p]) {
         anchor[cand_idx[p]] = i;
-        hregion[cand_idx[p]] = regid;
+        hregion[cand_idx[p]] = __float_as_int(xc[p].w);
     }
+}
+
+__global__ void k_in_halo(const int32_t* __restrict__ halo_id, uint32_t n, int32_t* __restrict__ in_halo) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) in_halo[i] = halo_id[i] > 0 ? 1 : 0;
 }
 
 // anchor and region of each valid halo under its new (bound) number
@@ -1399,8 +1404,10 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, flag, cidx, perm, xc, d_reg, N, A.np_min, o_in,
-                                         o_id, o_root, anchor, hreg);
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, flag, cidx, perm, xc, d_reg, N, A.np_min, o_id,
+                                         o_root, anchor, hreg);
+        PYFOF_LAUNCH_CHECK();
+        k_in_halo<<<nblk(N), kBlk, 0, s>>>(o_id, N, o_in);
         PYFOF_LAUNCH_CHECK();
         uint32_t nh = 0;
         PYFOF_HIP(hipMemcpyAsync(&nh, scal + 1, 4, hipMemcpyDeviceToHost, s));
