@@ -93,10 +93,41 @@ __device__ __forceinline__ double addback_term(const float4& qi, const float4& q
     return __dmul_rn(__dmul_rn((double)qj.w, Gd), depth_continuous(r, app));
 }
 
-__device__ double c2_delta_depth(double r, double ht, double hs) {
-    if (r <= 0.0) return (208.0 / 70.0) * (1.0 / ht - 1.0 / hs);
-    if (r >= ht && r >= hs) return 0.0;
-    return depth_continuous(r, ht) - depth_continuous(r, hs);
+// depth_continuous(r, h) for a fixed h with the divisions by h taken out of
+// the pair loop: 2r/h, x/(70h) and 12/e = 6h/r become products, and the
+// caller passes 1/r once for both kernels of a C2 difference. The results
+// differ from depth_continuous by f64 rounding only (relative ~1e-16); the
+// pair terms are rounded to f32 afterwards.
+struct DepthK {
+    double h, two_over_h, inv_70h, six_h;
+    __device__ __forceinline__ void init(double hh) {
+        h = hh;
+        two_over_h = 2.0 / hh;
+        inv_70h = 1.0 / (70.0 * hh);
+        six_h = 6.0 * hh;
+    }
+    // r >= 0; inv_r = 1/r (unused when e <= 1, so r = 0 is fine)
+    __device__ __forceinline__ double eval(double r, double inv_r) const {
+        if (r >= h) return inv_r;
+        const double e = r * two_over_h;
+        if (e <= 1.0) {
+            const double prim = e * e * (112.0 + e * e * (-56.0 + e * (14.0 + e * (8.0 - 3.0 * e))));
+            return (208.0 - prim) * inv_70h;
+        }
+        const double prim = e * e * (448.0 + e * (-280.0 + e * (56.0 + e * (14.0 + e * (-8.0 + e))))) -
+                            224.0 * e - six_h * inv_r;
+        return (128.0 - prim) * inv_70h;
+    }
+};
+
+// C2 residual depth of a pair at distance d (in grid units: rs = d / sp),
+// cutoff kernel ht = app / sp and outer kernel hs = 3.5
+__device__ __forceinline__ double c2_delta_fast(double d, double inv_sp, double sp, const DepthK& Kt,
+                                                const DepthK& Ks) {
+    const double rs = d * inv_sp;
+    if (rs >= Kt.h && rs >= Ks.h) return 0.0;
+    const double inv_rs = sp / d;
+    return Kt.eval(rs, inv_rs) - Ks.eval(rs, inv_rs);
 }
 
 __device__ __forceinline__ double gm_rel(double x, double xa, double L, double Lh) {
@@ -926,14 +957,15 @@ __global__ void k_tl_interp(const int32_t* __restrict__ tgt_slot, const int4* __
 // short-range term of a two-level candidate for target i with cutoff a_i
 // (bound_fortran.f90:1160-1175): f64 differences of the f32 positions, depth
 // kernels in cube units, w_j * dphi in f32
-__device__ __forceinline__ float tl_pair(const float4& qi, const float4& qj, double ai, double eps,
-                                         double factor) {
+__device__ __forceinline__ float tl_pair(const float4& qi, const float4& qj, const DepthK& Ke,
+                                         const DepthK& Ka, double factor) {
     const double dx = (double)qj.x - (double)qi.x, dy = (double)qj.y - (double)qi.y,
                  dz = (double)qj.z - (double)qi.z;
     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-    if (!(r2 < ai * ai)) return 0.0f;
+    if (!(r2 < Ka.h * Ka.h)) return 0.0f;  // Ka.h = a_i, the target's cutoff
     const double d = sqrt(r2);
-    const float dphi = -(float)(factor * (depth_continuous(d, eps) - depth_continuous(d, ai)));
+    const double inv_d = 1.0 / d;
+    const float dphi = -(float)(factor * (Ke.eval(d, inv_d) - Ka.eval(d, inv_d)));
     return __fmul_rn(qj.w, dphi);
 }
 
@@ -1065,6 +1097,12 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
     const float pref = (float)(a / (6.0 * kPi * (double)sp));
     const double ai = 3.5 * (double)sp / (double)(pr ? pr[t] : 1), eps = (double)(float)app,
                  factor = a / (6.0 * kPi);
+    DepthK Kt, Ks, Ke, Ka;  // C2: ht and hs in grid units; two-level: eps and a_i
+    Kt.init(ht);
+    Ks.init(hs);
+    Ke.init(eps);
+    Ka.init(ai);
+    const double spd = (double)sp, inv_sp = 1.0 / spd;
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -1083,14 +1121,13 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
                     if (j == t || !keep[j]) continue;
                     const float4 qj = qw[j];
                     if (H.tl) {  // two-level: the target's own cutoff 3.5 sp / r
-                        acc = __fadd_rn(acc, tl_pair(qi, qj, ai, eps, factor));
+                        acc = __fadd_rn(acc, tl_pair(qi, qj, Ke, Ka, factor));
                         continue;
                     }
                     const double dx = (double)(qj.x - qi.x), dy = (double)(qj.y - qi.y), dz = (double)(qj.z - qi.z);
                     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
                     if (r2 < r2cut) {
-                        const double rs = sqrt(r2) / (double)sp;
-                        const float dphi = __fmul_rn(-pref, (float)c2_delta_depth(rs, ht, hs));
+                        const float dphi = __fmul_rn(-pref, (float)c2_delta_fast(sqrt(r2), inv_sp, spd, Kt, Ks));
                         acc = __fadd_rn(acc, __fmul_rn(qj.w, dphi));
                     }
                 }
@@ -1151,6 +1188,12 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
     const float pref = (float)(a / (6.0 * kPi * (double)sp));
     const double ai = 3.5 * (double)sp / (double)(pr ? pr[t] : 1), eps = (double)(float)app,
                  factor = a / (6.0 * kPi);
+    DepthK Kt, Ks, Ke, Ka;  // C2: ht and hs in grid units; two-level: eps and a_i
+    Kt.init(ht);
+    Ks.init(hs);
+    Ke.init(eps);
+    Ka.init(ai);
+    const double spd = (double)sp, inv_sp = 1.0 / spd;
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -1167,14 +1210,13 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
                 for (uint32_t p = dstart[c]; p < dstart[c + 1]; ++p) {
                     const float4 qj = qw[dlist[p]];
                     if (H.tl) {
-                        acc = __fsub_rn(acc, tl_pair(qi, qj, ai, eps, factor));
+                        acc = __fsub_rn(acc, tl_pair(qi, qj, Ke, Ka, factor));
                         continue;
                     }
                     const double dx = (double)(qi.x - qj.x), dy = (double)(qi.y - qj.y), dz = (double)(qi.z - qj.z);
                     const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
                     if (r2 < r2cut) {
-                        const double rs = sqrt(r2) / (double)sp;
-                        const float dphi = __fmul_rn(-pref, (float)c2_delta_depth(rs, ht, hs));
+                        const float dphi = __fmul_rn(-pref, (float)c2_delta_fast(sqrt(r2), inv_sp, spd, Kt, Ks));
                         acc = __fsub_rn(acc, __fmul_rn(qj.w, dphi));
                     }
                 }
