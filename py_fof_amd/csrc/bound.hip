@@ -645,36 +645,99 @@ __global__ void k_tsc_setup(const float4* __restrict__ qw, const uint32_t* __res
 }
 
 // deposit of kept members into int64 fixed-point grids (order independent)
-__global__ void k_deposit(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
-                          const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
-                          const uint8_t* __restrict__ keep, const int4* __restrict__ first,
-                          const float* __restrict__ wt, const uint8_t* __restrict__ active,
-                          const double* __restrict__ fscale, int32_t cls, uint32_t nmem,
-                          unsigned long long* __restrict__ acc) {
-    const uint32_t t = blockIdx.x * kB + threadIdx.x;
-    if (t >= nmem || !keep[t]) return;
-    const uint32_t h = mhalo[t];
-    const int32_t k = pm_of_halo[h];
-    if (k < 0 || pm[k].cls != cls || !active[h]) return;
-    const PmHalo H = pm[k];
-    const int n = H.n;
-    const float w = qw[t].w;
-    const int4 f = first[t];
-    const float* w9 = wt + 9 * (size_t)t;
-    unsigned long long* g = acc + (size_t)H.slot * n * n * n;
-    // stencil nodes nearest..nearest+2 per axis (bound_fortran.f90:2660-2668);
-    // k_tsc_setup's guard t in [0, n/2 - 1) keeps them inside [0, n/2 + 1]
-    for (int sz = 0; sz < 3; ++sz) {
-        const int iz = f.z + sz;
-        for (int sy = 0; sy < 3; ++sy) {
-            const int iy = f.y + sy;
-            for (int sx = 0; sx < 3; ++sx) {
-                const int ix = f.x + sx;
-                const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
-                const long long fx = llrint((double)v * fscale[h]);
-                atomicAdd(&g[(size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz)], (unsigned long long)fx);
+// TSC deposit of the kept members of active halos of one grid class
+// (bound_fortran.f90:2660-2668) as int64 fixed point, so the sums do not
+// depend on the order of the additions. Members come in C2-cell order (order
+// = the class's range of the C2 member list), so the 256 members of a
+// workgroup are neighbours: when they belong to one halo and their stencils
+// fit in a box of kDepBox nodes, they are summed into that box in LDS first
+// and the box is added to the grid once per node; otherwise each member adds
+// its 27 nodes to the grid directly.
+constexpr int kDepBox = 4096;
+
+__global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+                                                const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
+                                                const uint8_t* __restrict__ keep, const int4* __restrict__ first,
+                                                const float* __restrict__ wt, const uint8_t* __restrict__ active,
+                                                const double* __restrict__ fscale, int32_t cls,
+                                                const uint32_t* __restrict__ order, uint32_t count,
+                                                unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long box[kDepBox];
+    __shared__ int32_t blo[3], bhi[3], bslot, bn, mixed;
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) { blo[d] = 0x7fffffff; bhi[d] = -0x7fffffff; }
+        bslot = -1;
+        bn = 0;
+        mixed = 0;
+    }
+    __syncthreads();
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    bool dep = false;
+    uint32_t t = 0, h = 0;
+    int32_t slot = -1, n = 0;
+    int4 f = make_int4(0, 0, 0, 0);
+    if (ps < count) {
+        t = order[ps];
+        if (keep[t]) {
+            h = mhalo[t];
+            const int32_t k = pm_of_halo[h];
+            if (k >= 0 && pm[k].cls == cls && active[h]) {
+                dep = true;
+                slot = pm[k].slot;
+                n = pm[k].n;
+                f = first[t];
             }
         }
+    }
+    if (dep) {
+        atomicMin(&blo[0], f.x); atomicMin(&blo[1], f.y); atomicMin(&blo[2], f.z);
+        atomicMax(&bhi[0], f.x + 2); atomicMax(&bhi[1], f.y + 2); atomicMax(&bhi[2], f.z + 2);
+        if (atomicCAS(&bslot, -1, slot) == -1) bn = n;
+    }
+    __syncthreads();
+    if (dep && slot != bslot) mixed = 1;
+    __syncthreads();
+    // stencil nodes nearest..nearest+2 per axis; k_tsc_setup's guard t in
+    // [0, n/2 - 1) keeps them inside [0, n/2 + 1]
+    const float w = dep ? qw[t].w : 0.0f;
+    const float* w9 = wt + 9 * (size_t)t;
+    const double fs = dep ? fscale[h] : 0.0;
+    const int ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1, ez = bhi[2] - blo[2] + 1;
+    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= kDepBox;
+    if (!in_lds) {
+        if (!dep) return;
+        unsigned long long* g = acc + (size_t)slot * n * n * n;
+        for (int sz = 0; sz < 3; ++sz)
+            for (int sy = 0; sy < 3; ++sy)
+                for (int sx = 0; sx < 3; ++sx) {
+                    const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                    const long long fx = llrint((double)v * fs);
+                    const size_t node = (size_t)(f.x + sx) + (size_t)n * ((size_t)(f.y + sy) + (size_t)n * (f.z + sz));
+                    atomicAdd(&g[node], (unsigned long long)fx);
+                }
+        return;
+    }
+    const int vol = ex * ey * ez;
+    for (int i = threadIdx.x; i < vol; i += kB) box[i] = 0ull;
+    __syncthreads();
+    if (dep) {
+        for (int sz = 0; sz < 3; ++sz)
+            for (int sy = 0; sy < 3; ++sy)
+                for (int sx = 0; sx < 3; ++sx) {
+                    const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                    const long long fx = llrint((double)v * fs);
+                    const int i = (f.x + sx - blo[0]) + ex * ((f.y + sy - blo[1]) + ey * (f.z + sz - blo[2]));
+                    atomicAdd(&box[i], (unsigned long long)fx);
+                }
+    }
+    __syncthreads();
+    const int nn = bn;
+    unsigned long long* g = acc + (size_t)bslot * nn * nn * nn;
+    for (int i = threadIdx.x; i < vol; i += kB) {
+        const unsigned long long v = box[i];
+        if (v == 0ull) continue;  // no member reached this node (all terms are >= 0)
+        const int x = blo[0] + i % ex, y = blo[1] + (i / ex) % ey, z = blo[2] + i / (ex * ey);
+        atomicAdd(&g[(size_t)x + (size_t)nn * ((size_t)y + (size_t)nn * z)], v);
     }
 }
 
@@ -1716,6 +1779,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     PmHalo* d_pm = nullptr;
     uint64_t c2cells = 0;
     const uint32_t *clist = nullptr;
+    uint32_t cls_first[4] = {0, 0, 0, 0}, cls_count[4] = {0, 0, 0, 0}, cls_end_prev = 0;
     uint32_t *cell_of = nullptr, *cstart = nullptr, *dstart = nullptr;
     int4* first = nullptr;
     float *wt = nullptr, *selfv = nullptr, *phi_c1 = nullptr, *phi_c2 = nullptr;
@@ -1772,8 +1836,21 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                 H.nc[d] = nc < 1 ? 1 : nc;
                 H.umin[d] = ue[6 * k + d];
             }
-            H.c2off = (int32_t)c2cells;
-            c2cells += (uint64_t)H.nc[0] * H.nc[1] * H.nc[2];
+        }
+        // C2 cells numbered class by class, so that after the sort by cell the
+        // members of each grid class form one range of the C2 member list
+        // (the deposit walks that range); the order inside a cell is the
+        // member order either way
+        for (int g = 0; g < 4; ++g) {
+            cls_first[g] = cls_end_prev;
+            for (uint32_t k = 0; k < npm; ++k) {
+                PmHalo& H = pm[k];
+                if (H.cls != g) continue;
+                H.c2off = (int32_t)c2cells;
+                c2cells += (uint64_t)H.nc[0] * H.nc[1] * H.nc[2];
+                cls_end_prev += (uint32_t)hsize((uint32_t)pm_halo[k]);
+            }
+            cls_count[g] = cls_end_prev - cls_first[g];
         }
         if (c2cells >= 0x7fffffffull) return IERR_TOO_LARGE;
         PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
@@ -1979,8 +2056,9 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                     const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)sl.size());
                     k_zero_region<<<gz, kB, 0, s>>>(acc[g], d_slots, n, mc);
                     PYFOF_LAUNCH_CHECK();
-                    k_deposit<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, active,
-                                                       fscale, g, M, acc[g]);
+                    k_deposit<<<blocks(cls_count[g]), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt,
+                                                                  active, fscale, g, clist + cls_first[g],
+                                                                  cls_count[g], acc[g]);
                     PYFOF_LAUNCH_CHECK();
                     fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
                                Gfull[g], s);
