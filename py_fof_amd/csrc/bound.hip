@@ -960,32 +960,117 @@ __global__ void k_tl_targets(const float4* __restrict__ qw, const uint32_t* __re
 
 // fixed-point deposit of kept sources of the slots [s_lo, s_hi) of one ratio
 // class into their inner grids (local slot = slot's class index - s_lo)
-__global__ void k_tl_deposit(const int32_t* __restrict__ src_slot, const uint32_t* __restrict__ src_member,
-                             const int4* __restrict__ src_first, const float* __restrict__ src_wt,
-                             uint32_t nsrc, const TlSlot* __restrict__ slots, int32_t ratio, int32_t s_lo,
-                             int32_t s_hi, const uint8_t* __restrict__ keep,
-                             const uint8_t* __restrict__ active, const float4* __restrict__ qw,
-                             const double* __restrict__ fscale, unsigned long long* __restrict__ acc) {
+// sort keys of the source entries: slot rank (ratio-2 slots first, then
+// ratio 4, each by slot), then the 4^3-node block of the stencil's first
+// node; counts per slot rank for the chunk ranges
+__global__ void k_tl_src_keys(const int32_t* __restrict__ src_slot, const int4* __restrict__ src_first,
+                              const TlSlot* __restrict__ slots, uint32_t nsrc, int32_t nslot2,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                              uint32_t* __restrict__ rank_cnt) {
     const uint32_t e = blockIdx.x * kB + threadIdx.x;
     if (e >= nsrc) return;
     const TlSlot S = slots[src_slot[e]];
-    if (S.r != ratio || S.cls_slot < s_lo || S.cls_slot >= s_hi || !active[S.halo]) return;
-    const uint32_t t = src_member[e];
-    if (!keep[t]) return;
-    const int n = S.n_in;
+    const uint32_t rank = (uint32_t)(S.r == 2 ? S.cls_slot : nslot2 + S.cls_slot);
     const int4 f = src_first[e];
-    const float* w9 = src_wt + 9 * (size_t)e;
-    const float w = qw[t].w;
-    const double fs = fscale[S.halo];
-    unsigned long long* g = acc + (size_t)(S.cls_slot - s_lo) * n * n * n;
-    for (int sz = 0; sz < 3; ++sz)
-        for (int sy = 0; sy < 3; ++sy)
-            for (int sx = 0; sx < 3; ++sx) {
-                const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
-                const long long fx = llrint((double)v * fs);
-                atomicAdd(&g[(size_t)(f.x + sx) + (size_t)n * ((size_t)(f.y + sy) + (size_t)n * (f.z + sz))],
-                          (unsigned long long)fx);
+    keys[e] = (rank << 18) | ((uint32_t)(f.z >> 2) & 63u) << 12 | ((uint32_t)(f.y >> 2) & 63u) << 6 |
+              ((uint32_t)(f.x >> 2) & 63u);
+    vals[e] = e;
+    atomicAdd(&rank_cnt[rank], 1u);
+}
+
+// deposit of the band sources of one chunk of inner grids (bound_fortran.f90:
+// 895-904), as int64 fixed point. With order (the entries sorted by slot and
+// position) the launch covers the chunk's range only and a workgroup's
+// entries are neighbours in one inner grid: their stencils are summed in an
+// LDS box when it fits (kDepBox nodes), as in k_deposit.
+__global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ src_slot,
+                                                   const uint32_t* __restrict__ src_member,
+                                                   const int4* __restrict__ src_first,
+                                                   const float* __restrict__ src_wt,
+                                                   const uint32_t* __restrict__ order, uint32_t begin,
+                                                   uint32_t count, const TlSlot* __restrict__ slots,
+                                                   int32_t ratio, int32_t s_lo, int32_t s_hi,
+                                                   const uint8_t* __restrict__ keep,
+                                                   const uint8_t* __restrict__ active,
+                                                   const float4* __restrict__ qw,
+                                                   const double* __restrict__ fscale,
+                                                   unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long box[kDepBox];
+    __shared__ int32_t blo[3], bhi[3], bslot, bn, mixed;
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) { blo[d] = 0x7fffffff; bhi[d] = -0x7fffffff; }
+        bslot = -1;
+        bn = 0;
+        mixed = 0;
+    }
+    __syncthreads();
+    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    bool dep = false;
+    uint32_t e = 0, t = 0;
+    int32_t slot = -1, n = 0, halo = 0;
+    int4 f = make_int4(0, 0, 0, 0);
+    if (ps < count) {
+        e = order ? order[begin + ps] : begin + ps;
+        const TlSlot S = slots[src_slot[e]];
+        if (S.r == ratio && S.cls_slot >= s_lo && S.cls_slot < s_hi && active[S.halo]) {
+            t = src_member[e];
+            if (keep[t]) {
+                dep = true;
+                slot = S.cls_slot - s_lo;
+                n = S.n_in;
+                halo = S.halo;
+                f = src_first[e];
             }
+        }
+    }
+    if (dep) {
+        atomicMin(&blo[0], f.x); atomicMin(&blo[1], f.y); atomicMin(&blo[2], f.z);
+        atomicMax(&bhi[0], f.x + 2); atomicMax(&bhi[1], f.y + 2); atomicMax(&bhi[2], f.z + 2);
+        if (atomicCAS(&bslot, -1, slot) == -1) bn = n;
+    }
+    __syncthreads();
+    if (dep && slot != bslot) mixed = 1;
+    __syncthreads();
+    const float w = dep ? qw[t].w : 0.0f;
+    const float* w9 = src_wt + 9 * (size_t)e;
+    const double fs = dep ? fscale[halo] : 0.0;
+    const int ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1, ez = bhi[2] - blo[2] + 1;
+    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= kDepBox;
+    if (!in_lds) {
+        if (!dep) return;
+        unsigned long long* g = acc + (size_t)slot * n * n * n;
+        for (int sz = 0; sz < 3; ++sz)
+            for (int sy = 0; sy < 3; ++sy)
+                for (int sx = 0; sx < 3; ++sx) {
+                    const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                    const long long fx = llrint((double)v * fs);
+                    atomicAdd(&g[(size_t)(f.x + sx) + (size_t)n * ((size_t)(f.y + sy) + (size_t)n * (f.z + sz))],
+                              (unsigned long long)fx);
+                }
+        return;
+    }
+    const int vol = ex * ey * ez;
+    for (int i = threadIdx.x; i < vol; i += kB) box[i] = 0ull;
+    __syncthreads();
+    if (dep) {
+        for (int sz = 0; sz < 3; ++sz)
+            for (int sy = 0; sy < 3; ++sy)
+                for (int sx = 0; sx < 3; ++sx) {
+                    const float v = __fmul_rn(__fmul_rn(__fmul_rn(w, w9[sx]), w9[3 + sy]), w9[6 + sz]);
+                    const long long fx = llrint((double)v * fs);
+                    atomicAdd(&box[(f.x + sx - blo[0]) + ex * ((f.y + sy - blo[1]) + ey * (f.z + sz - blo[2]))],
+                              (unsigned long long)fx);
+                }
+    }
+    __syncthreads();
+    const int nn = bn;
+    unsigned long long* g = acc + (size_t)bslot * nn * nn * nn;
+    for (int i = threadIdx.x; i < vol; i += kB) {
+        const unsigned long long v = box[i];
+        if (v == 0ull) continue;
+        const int x = blo[0] + i % ex, y = blo[1] + (i / ex) % ey, z = blo[2] + i / (ex * ey);
+        atomicAdd(&g[(size_t)x + (size_t)nn * ((size_t)y + (size_t)nn * z)], v);
+    }
 }
 
 // band potential of kept targets (bound_fortran.f90:906-938)
@@ -1796,6 +1881,8 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
     float *src_wt = nullptr, *tgt_wt = nullptr, *self_in = nullptr, *phi_in = nullptr;
     uint8_t* pr = nullptr;
     uint32_t nsrc = 0;
+    const uint32_t* src_order = nullptr;  // entries sorted by slot rank and position (or null)
+    std::vector<uint32_t> rank_start;     // first sorted entry of each slot rank
     const float* Gin[2] = {nullptr, nullptr};
     const FftPlan* plan_in[2] = {nullptr, nullptr};
     unsigned long long* acc_in[2] = {nullptr, nullptr};
@@ -1986,6 +2073,30 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                                                             d_tlslots, M, nullptr, soff, src_slot, src_member,
                                                             src_first, src_wt, err);
                 PYFOF_LAUNCH_CHECK();
+                // entries sorted by slot and position: each chunk of inner
+                // grids is one range, and neighbouring entries share LDS boxes
+                const uint32_t nrank = (uint32_t)(tl_nslot[0] + tl_nslot[1]);
+                if (nsrc > 0 && nrank < (1u << 14)) {
+                    uint32_t* sk = C.buf("tlsk").as<uint32_t>(nsrc);
+                    uint32_t* sv = C.buf("tlsv").as<uint32_t>(nsrc);
+                    uint32_t* sk2 = C.buf("tlsk2").as<uint32_t>(nsrc);
+                    uint32_t* sv2 = C.buf("tlsv2").as<uint32_t>(nsrc);
+                    uint32_t* rcnt = C.buf("tlrcnt").as<uint32_t>(nrank);
+                    void* stmp = C.buf("tlstmp").get(radix_temp_bytes(nsrc) + 256);
+                    k_fill32<<<blocks(nrank), kB, 0, s>>>(rcnt, nrank, 0u);
+                    PYFOF_LAUNCH_CHECK();
+                    k_tl_src_keys<<<blocks(nsrc), kB, 0, s>>>(src_slot, src_first, d_tlslots, nsrc, tl_nslot[0], sk,
+                                                              sv, rcnt);
+                    PYFOF_LAUNCH_CHECK();
+                    const bool salt = radix_sort_pairs(sk, sv, sk2, sv2, nsrc, bits_for(((uint64_t)nrank << 18) - 1),
+                                                       stmp, s);
+                    src_order = salt ? sv2 : sv;
+                    std::vector<uint32_t> rc(nrank);
+                    PYFOF_HIP(hipMemcpyAsync(rc.data(), rcnt, 4 * (size_t)nrank, hipMemcpyDeviceToHost, s));
+                    PYFOF_HIP(hipStreamSynchronize(s));
+                    rank_start.assign(nrank + 1, 0u);
+                    for (uint32_t r = 0; r < nrank; ++r) rank_start[r + 1] = rank_start[r] + rc[r];
+                }
                 // inner kernels (bound_init_inner_gk) and targets
                 for (int c = 0; c < 2; ++c) {
                     if (!tl_nslot[c]) continue;
@@ -2085,10 +2196,21 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                         const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)ls.size());
                         k_zero_region<<<gz, kB, 0, s>>>(acc_in[c], d_tl_ls, n_in, mc);
                         PYFOF_LAUNCH_CHECK();
-                        k_tl_deposit<<<blocks(nsrc), kB, 0, s>>>(src_slot, src_member, src_first, src_wt, nsrc,
-                                                                 d_tlslots, ratio, cs, cs + tl_chunk[c], keep,
-                                                                 active, qw, fscale, acc_in[c]);
-                        PYFOF_LAUNCH_CHECK();
+                        uint32_t sbeg = 0, scount = nsrc;
+                        if (src_order) {
+                            const uint32_t base = c == 0 ? 0u : (uint32_t)tl_nslot[0];
+                            const uint32_t r_lo = base + (uint32_t)cs;
+                            const uint32_t r_hi = base + (uint32_t)std::min(cs + tl_chunk[c], tl_nslot[c]);
+                            sbeg = rank_start[r_lo];
+                            scount = rank_start[r_hi] - sbeg;
+                        }
+                        if (scount > 0) {
+                            k_tl_deposit<<<blocks(scount), kB, 0, s>>>(src_slot, src_member, src_first, src_wt,
+                                                                       src_order, sbeg, scount, d_tlslots, ratio, cs,
+                                                                       cs + tl_chunk[c], keep, active, qw, fscale,
+                                                                       acc_in[c]);
+                            PYFOF_LAUNCH_CHECK();
+                        }
                         fft_conv3d(*plan_in[c], grid_in[c], acc_in[c], d_tl_ls, (int32_t)ls.size(), d_tl_sh,
                                    fscale, Gin[c], s);
                         k_tl_interp<<<blocks(M), kB, 0, s>>>(tgt_slot, tgt_first, tgt_wt, self_in, d_tlslots, ratio,
