@@ -233,10 +233,38 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             }
         }
     }
+    // z pass: the next slot's columns are loaded into registers while this
+    // slot is transformed (the slot loop is the only loop over memory here)
+    float2 pre[MODE == kConvZ ? kPer : 1];
+    auto load_z = [&](int32_t ib) {
+        const int64_t sb = (int64_t)slots[ib] * grid;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int idx = tid + k * kFftBlock;
+            if (idx < total) {
+                const int e = idx / NL, c = idx - e * NL;
+                pre[k] = data[sb + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
+            }
+        }
+    };
+    if (MODE == kConvZ && ib0 < ib1) load_z(ib0);
     for (int32_t ib = ib0; ib < ib1; ++ib) {
         const int64_t sbase = (int64_t)slots[ib] * grid;
         double inv = 0.0;
         if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
+        if (MODE == kConvZ) {
+            // every element of a z column is an input (in_lim = N)
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int idx = tid + k * kFftBlock;
+                if (idx < total) {
+                    const int e = idx / NL, c = idx - e * NL;
+                    buf[0][c * N + e] = pre[k];
+                }
+            }
+            __syncthreads();
+            if (ib + 1 < ib1) load_z(ib + 1);
+        } else {
         for (int idx = tid; idx < total; idx += kFftBlock) {
             int c, e;
             if (ax.x_contig) { c = idx / N; e = idx - c * N; }
@@ -250,6 +278,7 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             buf[0][c * N + e] = v;
         }
         __syncthreads();
+        }
         int cur;
         if (MODE == kConvXFwd || MODE == kConvYFwd) {
             cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tw);
