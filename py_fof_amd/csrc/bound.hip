@@ -1430,6 +1430,70 @@ __device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* wcnt, uint32
     return before + in_wave;
 }
 
+// Large candidates (kBigMin members or more) run on several workgroups when
+// the context may use cooperative launches (k_pm_big) and on one otherwise
+// (k_pm_inner). Both form their f64 sums the same way, so both give the same
+// result: the members split into big_blocks() contiguous chunks; each chunk
+// is summed as a 256-thread workgroup sums it (thread-strided, then a wave
+// shuffle tree, then the 4 waves in order); the chunk sums are added in
+// chunk order.
+constexpr int kBigMin = 32768;    // members: at least this many use k_pm_big
+constexpr int kBigChunk = 16384;  // members per workgroup
+constexpr int kBigMaxB = 64;      // workgroups per candidate at most
+constexpr int kBW = kB / kWave;
+
+__host__ __device__ __forceinline__ uint32_t big_blocks(uint32_t n) {
+    const uint32_t b = (n + kBigChunk - 1) / kBigChunk;
+    return b < (uint32_t)kBigMaxB ? b : (uint32_t)kBigMaxB;
+}
+
+// the chunked f64 sums of a large candidate inside one workgroup of kPB
+// threads: NV values per member (7: mass, mass x position, mass x velocity;
+// 1: mass), kept members only; out[k] for k < NV, valid after the call
+template <int NV>
+__device__ void chunked_sums(uint32_t s0, uint32_t e0, const uint8_t* __restrict__ keep,
+                             const float4* __restrict__ qw, const float4* __restrict__ vv,
+                             double (*part)[8], double (*wred)[kPB / kWave], double* out) {
+    const uint32_t n = e0 - s0, nb = big_blocks(n), chunk = (n + nb - 1) / nb;
+    const int tid = threadIdx.x, sub = tid / kB, lt = tid % kB, w = tid / kWave;
+    constexpr int kSub = kPB / kB;
+    for (uint32_t r = 0; r < (nb + kSub - 1) / kSub; ++r) {
+        const uint32_t b = r * kSub + (uint32_t)sub;
+        double v[NV];
+        for (int k = 0; k < NV; ++k) v[k] = 0.0;
+        if (b < nb) {
+            const uint32_t c0 = s0 + min(n, b * chunk), c1 = s0 + min(n, (b + 1) * chunk);
+            for (uint32_t t = c0 + lt; t < c1; t += kB) {
+                if (!keep[t]) continue;
+                const float4 q = qw[t];
+                const double wi = (double)q.w;
+                v[0] += wi;
+                if (NV == 7) {
+                    const float4 u = vv[t];
+                    v[1] += wi * (double)q.x; v[2] += wi * (double)q.y; v[3] += wi * (double)q.z;
+                    v[4] += wi * (double)u.x; v[5] += wi * (double)u.y; v[6] += wi * (double)u.z;
+                }
+            }
+        }
+        for (int k = 0; k < NV; ++k)
+            for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, kWave);
+        if (__lane_id() == 0) for (int k = 0; k < NV; ++k) wred[k][w] = v[k];
+        __syncthreads();
+        if (b < nb && lt < NV) {
+            double sacc = 0.0;
+            for (int j = 0; j < kBW; ++j) sacc += wred[lt][sub * kBW + j];
+            part[b][lt] = sacc;
+        }
+        __syncthreads();
+    }
+    if (tid < NV) {
+        double sacc = 0.0;
+        for (uint32_t j = 0; j < nb; ++j) sacc += part[j][tid];
+        out[tid] = sacc;
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ alist,
                                                   const uint32_t* __restrict__ hoff,
                                                   const float4* __restrict__ qw,
@@ -1448,13 +1512,20 @@ __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ al
     __shared__ uint32_t sh[4];
     __shared__ double bc[7];
     __shared__ int bad_any;
+    __shared__ double part[kBigMaxB][8];
     const int32_t h = alist[blockIdx.x];
     const uint32_t s0 = h ? hoff[h - 1] : 0u, e0 = hoff[h];
     const int tid = threadIdx.x, w = tid / kWave;
     const float b32 = (float)P.b;
+    const bool large = e0 - s0 >= (uint32_t)kBigMin;
     int np_bound = npb[h];
     int deleted = 0, st = -2;  // -2: still iterating
     for (;;) {
+        if (large) {
+            chunked_sums<7>(s0, e0, keep, qw, vv, part, red, bc);
+            if (tid == 0) bad_any = 0;
+            __syncthreads();
+        } else {
         // centre of mass and bulk velocity, f64, over kept members
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
         for (uint32_t t = s0 + tid; t < e0; t += kPB) {
@@ -1477,6 +1548,7 @@ __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ al
             bc[tid] = sacc;
         }
         __syncthreads();
+        }
         const double ws = bc[0];
         const float qc32[3] = {(float)(bc[1] / ws), (float)(bc[2] / ws), (float)(bc[3] / ws)};
         const float vc32[3] = {(float)(bc[4] / ws), (float)(bc[5] / ws), (float)(bc[6] / ws)};
@@ -1580,6 +1652,11 @@ __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ al
         }
     }
     // kept mass for the next potential (zero-mode term)
+    if (large) {
+        chunked_sums<1>(s0, e0, keep, qw, vv, part, red, bc);
+        if (tid == 0) mkeep[h] = bc[0];
+        return;
+    }
     double m = 0.0;
     for (uint32_t t = s0 + tid; t < e0; t += kPB)
         if (keep[t]) m += (double)qw[t].w;
@@ -1590,6 +1667,292 @@ __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ al
         double sm = 0.0;
         for (int j = 0; j < kPW; ++j) sm += red[0][j];
         mkeep[h] = sm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same removal loop for a large candidate on several workgroups (one
+// cooperative launch, so that all its workgroups are resident together).
+// Workgroup b of a candidate owns a contiguous chunk of its members and the
+// matching stretch of the unbound list; the candidate's workgroups meet at a
+// barrier in global memory (bounded spin) after each reduction. Values that
+// other workgroups wrote are read with device-scope atomic loads, and sums
+// over workgroups are taken in workgroup order, so every workgroup derives
+// the same COM, threshold and decisions.
+
+
+struct BigScratch {
+    double part[kBigMaxB][8];
+    uint32_t ucnt[kBigMaxB], tcnt[kBigMaxB];
+    uint32_t hist[4][256];
+    uint32_t bar_count, bar_gen, bad, timeout;
+};
+
+__device__ __forceinline__ uint32_t ld_dev(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_dev(const double* p) {
+    return __longlong_as_double(
+        __hip_atomic_load(reinterpret_cast<const long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// barrier of the nb workgroups of one candidate; false if the wait exceeded
+// its bound (then the timeout flag is set and every workgroup stops)
+__device__ bool big_barrier(BigScratch* S, uint32_t nb, int* ok_s) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        const uint32_t gen = ld_dev(&S->bar_gen);
+        if (atomicAdd(&S->bar_count, 1u) == nb - 1) {
+            __hip_atomic_store(&S->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            atomicAdd(&S->bar_gen, 1u);
+        } else {
+            uint32_t spins = 0;
+            while (ld_dev(&S->bar_gen) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 24) || ld_dev(&S->timeout)) { ok = 0; break; }
+            }
+        }
+        if (!ok) __hip_atomic_store(&S->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        *ok_s = ok;
+    }
+    __syncthreads();
+    return *ok_s != 0;
+}
+
+// rank of flagged threads in the workgroup (kB threads), and their total
+__device__ __forceinline__ uint32_t wg_rank(bool flag, uint32_t* wcnt, uint32_t& total) {
+    const int w = threadIdx.x / kWave;
+    const uint64_t bal = __ballot(flag);
+    const uint32_t in_wave = __popcll(bal & lanemask_lt());
+    if (__lane_id() == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int k = 0; k < kBW; ++k) {
+        const uint32_t c = wcnt[k];
+        before += k < w ? c : 0u;
+        total += c;
+    }
+    __syncthreads();
+    return before + in_wave;
+}
+
+__global__ void __launch_bounds__(kB) k_pm_big(const int2* __restrict__ bmap, const int32_t* __restrict__ big_h,
+                                               const int32_t* __restrict__ big_nb, BigScratch* __restrict__ scr,
+                                               const uint32_t* __restrict__ hoff, const float4* __restrict__ qw,
+                                               const float4* __restrict__ vv, const float* __restrict__ phi,
+                                               uint32_t* __restrict__ ul, uint32_t* __restrict__ uk,
+                                               uint8_t* __restrict__ keep, int32_t* __restrict__ status,
+                                               int32_t* __restrict__ npb, uint8_t* __restrict__ active,
+                                               double* __restrict__ mkeep, int32_t last_pass,
+                                               int32_t* __restrict__ err, Params P) {
+    __shared__ double red[8][kBW];
+    __shared__ uint32_t lhist[256];
+    __shared__ uint32_t wcnt[kBW];
+    __shared__ double bc[8];
+    __shared__ uint32_t sel[2];
+    __shared__ int ok_s, bad_s;
+    const int2 bm = bmap[blockIdx.x];
+    const int32_t kb = bm.x;
+    const uint32_t b = (uint32_t)bm.y;
+    const int32_t h = big_h[kb];
+    const uint32_t nb = (uint32_t)big_nb[kb];
+    BigScratch* S = scr + kb;
+    const uint32_t s0 = h ? hoff[h - 1] : 0u, e0 = hoff[h];
+    const uint32_t n = e0 - s0, chunk = (n + nb - 1) / nb;
+    const uint32_t c0 = s0 + min(n, b * chunk), c1 = s0 + min(n, (b + 1) * chunk);
+    const int tid = threadIdx.x, w = tid / kWave;
+    const float b32 = (float)P.b;
+    int np_bound = npb[h];
+    int deleted = 0, st = -2;
+    bool ok = true;
+    for (;;) {
+        // the select histograms start from zero each iteration; every
+        // workgroup finished reading the previous ones before the last barrier
+        if (b == 0)
+            for (int i = tid; i < 4 * 256; i += kB) (&S->hist[0][0])[i] = 0u;
+        // centre of mass and bulk velocity, f64: partial sums per workgroup
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t t = c0 + tid; t < c1; t += kB) {
+            if (!keep[t]) continue;
+            const float4 q = qw[t];
+            const float4 u = vv[t];
+            const double wi = (double)q.w;
+            v[0] += wi;
+            v[1] += wi * (double)q.x; v[2] += wi * (double)q.y; v[3] += wi * (double)q.z;
+            v[4] += wi * (double)u.x; v[5] += wi * (double)u.y; v[6] += wi * (double)u.z;
+        }
+        for (int k = 0; k < 7; ++k)
+            for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, kWave);
+        if (__lane_id() == 0) for (int k = 0; k < 7; ++k) red[k][w] = v[k];
+        __syncthreads();
+        if (tid < 7) {
+            double sacc = 0.0;
+            for (int j = 0; j < kBW; ++j) sacc += red[tid][j];
+            S->part[b][tid] = sacc;
+        }
+        if (!(ok = big_barrier(S, nb, &ok_s))) break;
+        if (tid < 7) {
+            double sacc = 0.0;
+            for (uint32_t j = 0; j < nb; ++j) sacc += ld_dev(&S->part[j][tid]);
+            bc[tid] = sacc;
+        }
+        __syncthreads();
+        const double ws = bc[0];
+        const float qc32[3] = {(float)(bc[1] / ws), (float)(bc[2] / ws), (float)(bc[3] / ws)};
+        const float vc32[3] = {(float)(bc[4] / ws), (float)(bc[5] / ws), (float)(bc[6] / ws)};
+        // energies; this workgroup's unbound members listed in index order
+        // from c0 on
+        if (tid == 0) bad_s = 0;
+        __syncthreads();
+        uint32_t nu_b = 0;
+        for (uint32_t base = c0; base < c1; base += kB) {
+            const uint32_t t = base + tid;
+            bool unb = false;
+            uint32_t key = 0;
+            if (t < c1 && keep[t]) {
+                const float4 q = qw[t];
+                const float4 u = vv[t];
+                const float d0 = __fadd_rn(__fsub_rn(u.x, vc32[0]), __fmul_rn(b32, __fsub_rn(q.x, qc32[0])));
+                const float d1 = __fadd_rn(__fsub_rn(u.y, vc32[1]), __fmul_rn(b32, __fsub_rn(q.y, qc32[1])));
+                const float d2 = __fadd_rn(__fsub_rn(u.z, vc32[2]), __fmul_rn(b32, __fsub_rn(q.z, qc32[2])));
+                float sq = __fmul_rn(d0, d0);
+                sq = __fadd_rn(sq, __fmul_rn(d1, d1));
+                sq = __fadd_rn(sq, __fmul_rn(d2, d2));
+                const float E = __fadd_rn(sq / 2.0f, phi[t]);
+                if (!isfinite(E)) bad_s = 1;
+                else if (E >= 0.0f) { unb = true; key = ekey(E); }
+            }
+            uint32_t tot;
+            const uint32_t r = wg_rank(unb, wcnt, tot);
+            if (unb) { ul[c0 + nu_b + r] = t; uk[c0 + nu_b + r] = key; }
+            nu_b += tot;
+        }
+        if (tid == 0) {
+            S->ucnt[b] = nu_b;
+            if (bad_s) atomicOr(&S->bad, 1u);
+        }
+        if (!(ok = big_barrier(S, nb, &ok_s))) break;
+        uint32_t nu = 0;
+        for (uint32_t j = 0; j < nb; ++j) nu += ld_dev(&S->ucnt[j]);
+        if (ld_dev(&S->bad)) { st = -1; break; }
+        if (nu == 0) {
+            if (deleted == 0) st = 0;
+            break;  // status OK, or refresh the potential
+        }
+        int nbd = (int)ceil(0.02 * (double)np_bound);
+        nbd = nbd < 1 ? 1 : nbd;
+        nbd = nbd > (int)nu ? (int)nu : nbd;
+        // radix select of the nbd-th largest key over all workgroups' lists
+        uint32_t prefix = 0, mask = 0, need = (uint32_t)nbd;
+        for (int byte = 3; byte >= 0; --byte) {
+            for (int i = tid; i < 256; i += kB) lhist[i] = 0;
+            __syncthreads();
+            for (uint32_t k = tid; k < nu_b; k += kB) {
+                const uint32_t key = uk[c0 + k];
+                if ((key & mask) == prefix) atomicAdd(&lhist[(key >> (8 * byte)) & 255u], 1u);
+            }
+            __syncthreads();
+            for (int i = tid; i < 256; i += kB)
+                if (lhist[i]) atomicAdd(&S->hist[byte][i], lhist[i]);
+            if (!(ok = big_barrier(S, nb, &ok_s))) break;
+            if (tid == 0) {
+                uint32_t rem = need;
+                int dsel = 0;
+                for (int d = 255; d >= 0; --d) {
+                    const uint32_t c = ld_dev(&S->hist[byte][d]);
+                    if (c >= rem) { dsel = d; break; }
+                    rem -= c;
+                }
+                sel[0] = (uint32_t)dsel;
+                sel[1] = rem;
+            }
+            __syncthreads();
+            prefix |= sel[0] << (8 * byte);
+            mask |= 255u << (8 * byte);
+            need = sel[1];
+            __syncthreads();
+        }
+        if (!ok) break;
+        // delete keys > T; keys == T (ties) are deleted in index order, the
+        // `need` first over all workgroups
+        const uint32_t T = prefix;
+        uint32_t ties = 0;
+        for (uint32_t base = 0; base < nu_b; base += kB) {
+            const uint32_t k = base + tid;
+            bool tie = false;
+            if (k < nu_b) {
+                const uint32_t key = uk[c0 + k];
+                if (key > T) keep[ul[c0 + k]] = 0;
+                else if (key == T) tie = true;
+            }
+            uint32_t tot;
+            wg_rank(tie, wcnt, tot);
+            ties += tot;
+        }
+        if (tid == 0) S->tcnt[b] = ties;
+        if (!(ok = big_barrier(S, nb, &ok_s))) break;
+        uint32_t taken = 0;
+        for (uint32_t j = 0; j < b; ++j) taken += ld_dev(&S->tcnt[j]);
+        if (taken < need) {
+            for (uint32_t base = 0; base < nu_b; base += kB) {
+                const uint32_t k = base + tid;
+                const bool tie = k < nu_b && uk[c0 + k] == T;
+                uint32_t tot;
+                const uint32_t r = wg_rank(tie, wcnt, tot);
+                if (tie && taken + r < need) keep[ul[c0 + k]] = 0;
+                taken += tot;
+            }
+        }
+        np_bound -= nbd;
+        deleted += nbd;
+        if (np_bound < P.np_min) { st = 1; break; }
+        if ((double)deleted >= 0.2 * (double)np_bound) break;
+    }
+    if (!ok) {
+        if (b == 0 && tid == 0) {
+            atomicCAS(err, 0, IERR_HIP);
+            status[h] = 5;
+            active[h] = 0;
+        }
+        return;
+    }
+    // kept mass for the next potential (zero-mode term)
+    double m = 0.0;
+    for (uint32_t t = c0 + tid; t < c1; t += kB)
+        if (keep[t]) m += (double)qw[t].w;
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_down(m, o, kWave);
+    if (__lane_id() == 0) red[7][w] = m;
+    __syncthreads();
+    if (tid == 0) {
+        double sm = 0.0;
+        for (int j = 0; j < kBW; ++j) sm += red[7][j];
+        S->part[b][7] = sm;
+    }
+    if (!big_barrier(S, nb, &ok_s)) {
+        if (b == 0 && tid == 0) atomicCAS(err, 0, IERR_HIP);
+        return;
+    }
+    if (b == 0 && tid == 0) {
+        double sm = 0.0;
+        for (uint32_t j = 0; j < nb; ++j) sm += ld_dev(&S->part[j][7]);
+        mkeep[h] = sm;
+        npb[h] = np_bound;
+        if (st == -1) {
+            atomicCAS(err, 0, 1);
+            status[h] = 5;
+            active[h] = 0;
+        } else if (st >= 0) {
+            status[h] = st;
+            active[h] = 0;
+        } else if (last_pass) {
+            status[h] = 5;
+            active[h] = 0;
+        }
     }
 }
 
@@ -2141,6 +2504,19 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
 
     // the pass loop over a set of mesh / large-direct halos
     // (bound_fortran.f90:2613-2864): potential, then the removal loop
+    // workgroups of k_pm_big that can be resident at once (0: large
+    // candidates stay on k_pm_inner)
+    int64_t big_cap = 0;
+    {
+        int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+        PYFOF_HIP(hipGetDevice(&dev));
+        PYFOF_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+        PYFOF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        PYFOF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_pm_big), kB, 0));
+        if (A.multi_wg && coop && per_cu > 0) big_cap = (int64_t)per_cu * ncu;
+    }
+    const uint32_t* hoff_p = A.hoff;
+
     auto iterate = [&](std::vector<int32_t> act_list) -> int32_t {
         if (act_list.empty()) return IERR_OK;
         std::vector<uint8_t> on(nh, 0);
@@ -2295,11 +2671,51 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                 k_record_phi<<<blocks(M), kB, 0, s>>>(A.mhalo, active, phi, M, phi_rec);
                 PYFOF_LAUNCH_CHECK();
             }
-            PYFOF_HIP(hipMemcpyAsync(d_alist, act_list.data(), 4 * act_list.size(), hipMemcpyHostToDevice, s));
-            k_pm_inner<<<(uint32_t)act_list.size(), kPB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, ulist, ukey, keep,
-                                                                status, npb, active, mkeep, pass == 20,
-                                                                err, P);
-            PYFOF_LAUNCH_CHECK();
+            // removal loop: one workgroup per candidate, except large ones,
+            // which get several (k_pm_big, cooperative launches)
+            std::vector<int32_t> small_list, big_list;
+            for (int32_t h : act_list)
+                (big_cap > 0 && hsize(h) >= kBigMin ? big_list : small_list).push_back(h);
+            if (!small_list.empty()) {
+                PYFOF_HIP(hipMemcpyAsync(d_alist, small_list.data(), 4 * small_list.size(), hipMemcpyHostToDevice, s));
+                k_pm_inner<<<(uint32_t)small_list.size(), kPB, 0, s>>>(d_alist, A.hoff, qw, vv, phi, ulist, ukey,
+                                                                      keep, status, npb, active, mkeep, pass == 20,
+                                                                      err, P);
+                PYFOF_LAUNCH_CHECK();
+            }
+            std::vector<int2> bmap;
+            std::vector<int32_t> bh, bnb;
+            for (size_t i = 0; i < big_list.size();) {
+                // as many candidates per launch as stay resident together
+                bmap.clear();
+                bh.clear();
+                bnb.clear();
+                for (; i < big_list.size(); ++i) {
+                    const int64_t n = hsize(big_list[i]);
+                    const int32_t nbk = (int32_t)big_blocks((uint32_t)n);
+                    if (!bh.empty() && (int64_t)bmap.size() + nbk > big_cap) break;
+                    for (int32_t b = 0; b < nbk; ++b) bmap.push_back(make_int2((int32_t)bh.size(), b));
+                    bh.push_back(big_list[i]);
+                    bnb.push_back(nbk);
+                }
+                int2* d_bmap = C.buf("bigmap").as<int2>(bmap.size());
+                int32_t* d_bh = C.buf("bigh").as<int32_t>(bh.size());
+                int32_t* d_bnb = C.buf("bignb").as<int32_t>(bnb.size());
+                BigScratch* d_scr = C.buf("bigscr").as<BigScratch>(bh.size());
+                const uint64_t words = sizeof(BigScratch) / 4 * bh.size();
+                k_fill32<<<blocks(words), kB, 0, s>>>(reinterpret_cast<uint32_t*>(d_scr), words, 0u);
+                PYFOF_LAUNCH_CHECK();
+                PYFOF_HIP(hipMemcpyAsync(d_bmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice, s));
+                PYFOF_HIP(hipMemcpyAsync(d_bh, bh.data(), 4 * bh.size(), hipMemcpyHostToDevice, s));
+                PYFOF_HIP(hipMemcpyAsync(d_bnb, bnb.data(), 4 * bnb.size(), hipMemcpyHostToDevice, s));
+                int32_t last = pass == 20;
+                void* kargs[] = {&d_bmap, &d_bh, &d_bnb, &d_scr, &hoff_p, &qw, &vv, &phi,
+                                 &ulist, &ukey, &keep, &status, &npb, &active, &mkeep, &last, &err, &P};
+                PYFOF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pm_big),
+                                                     dim3((uint32_t)bmap.size()), dim3(kB), kargs, 0, s));
+                // the host vectors are read by the copies above: wait before reuse
+                PYFOF_HIP(hipStreamSynchronize(s));
+            }
             int32_t herr = 0;
             PYFOF_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
             PYFOF_HIP(hipMemcpyAsync(on.data(), active, nh, hipMemcpyDeviceToHost, s));

@@ -32,6 +32,7 @@ struct BoundArgs {
     int32_t direct_max = 1000;
     int32_t direct_f32 = 0;
     int32_t batched = 1;
+    bool multi_wg = true;  // large candidates on several workgroups (k_pm_big)
     float* phi_rec = nullptr;
 };
 

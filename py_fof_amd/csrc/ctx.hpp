@@ -82,6 +82,10 @@ struct Ctx {
     int32_t direct_f32 = 0;
     int32_t batched = 1;
     bool record_phi = false;
+    // large candidates may use multi-workgroup cooperative launches; off for
+    // contexts that share the GPU with others (the request server's workers),
+    // where the co-residency those launches need is not assured
+    bool solo = true;
     uint64_t rec_n = 0;  // recorded first-pass potentials of the last call
     DevBuf rec_phi, rec_idx;
     BoundStats last_bound;  // build the fof_with_info catalog on every call

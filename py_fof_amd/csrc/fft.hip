@@ -243,7 +243,10 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             const int idx = tid + k * kFftBlock;
             if (idx < total) {
                 const int e = idx / NL, c = idx - e * NL;
-                pre[k] = data[sb + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride];
+                // elements above in_lim are the zero padding: never stored
+                // by the forward passes, so they are not read
+                pre[k] = e <= d.in_lim ? data[sb + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride]
+                                       : make_float2(0.0f, 0.0f);
             }
         }
     };
@@ -253,7 +256,6 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
         double inv = 0.0;
         if (MODE == kConvXFwd) inv = 1.0 / fscale[slot_halo[slots[ib]]];
         if (MODE == kConvZ) {
-            // every element of a z column is an input (in_lim = N)
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int idx = tid + k * kFftBlock;

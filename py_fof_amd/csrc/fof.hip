@@ -1460,6 +1460,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             B.direct_max = C.direct_max;
             B.direct_f32 = C.direct_f32;
             B.batched = C.batched;
+            B.multi_wg = C.solo;
             C.rec_n = 0;
             if (C.record_phi) {
                 B.phi_rec = C.rec_phi.as<float>(std::max<uint32_t>(cand.nmem, 1));

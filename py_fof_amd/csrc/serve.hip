@@ -68,7 +68,10 @@ struct Server {
 
     Server(int dev, int64_t hbm_budget, int slots) : device(dev), budget(hbm_budget) {
         PYFOF_HIP(hipSetDevice(dev));
-        for (int k = 0; k < slots; ++k) ctxs.push_back(new Ctx(dev));
+        for (int k = 0; k < slots; ++k) {
+            ctxs.push_back(new Ctx(dev));
+            ctxs.back()->solo = slots == 1;
+        }
         for (int k = 0; k < slots; ++k) workers.emplace_back([this, k] { run(k); });
     }
 

@@ -111,3 +111,35 @@ def test_f32_potential_error_and_status():
     assert np.sqrt(np.mean(rel ** 2)) <= 1e-6
     diff = int(np.sum(r64["in_halo"] != r32["in_halo"]))
     assert diff <= 1e-3 * max(1, int(r64["in_halo"].sum())), diff
+
+
+def test_result_independent_of_recycled_device_memory():
+    # buffers are reused and never cleared between calls: a kernel that read
+    # memory no pass had written (the zero padding of the convolution grids)
+    # would give results that depend on what earlier contexts left behind
+    from py_fof_amd import _native
+    import py_fof_amd as pf
+    sizes = np.array([300, 1500, 5000, 12000, 40000, 90000])
+    xs, vs = _halos(sizes, 23)
+
+    def run(ctx):
+        ctx.set_option(_native.OPT_RECORD_PHI, 1)
+        r = pf._fof_core(xs, vs, L_Box=L, linking_length=0.25, np_halo_min=10, bound_check=True,
+                         return_ang_mom=True, softening=0.1, ng_global=4096.0, _ctx=ctx)
+        return r, ctx.potential()
+
+    first = _native.Context(0)
+    ref = run(first)
+    # a context that fills its grids and frees them, then a new one
+    other = _native.Context(0)
+    big, bigv = _halos(np.array([200000, 60000, 30000]), 24)
+    pf._fof_core(big, bigv, L_Box=L, linking_length=0.25, np_halo_min=10, bound_check=True,
+                 softening=0.1, ng_global=4096.0, _ctx=other)
+    other.close()
+    late = _native.Context(0)
+    got = run(late)
+    for a, b in zip(ref[0], got[0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ref[1][0], got[1][0]) and np.array_equal(ref[1][1], got[1][1])
+    first.close()
+    late.close()
