@@ -76,12 +76,17 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
  *     one launch per step; 0 unbinds one candidate at a time (plain form).
  *   PYFOF_OPT_RECORD_PHI (default 0): keep the first-pass potential of every
  *     candidate member for pyfof_ctx_potential.
+ *   PYFOF_OPT_POISON_ALLOC (default 0; process-wide, not per context): a
+ *     debugging aid. 0 = off; 1 + b fills every device allocation made from
+ *     then on with the byte b (0..255), so that a read of memory no kernel
+ *     has written changes the result. Allocation becomes slower.
  * Returns 0, or 1 for an unknown option or an out-of-range value. */
 #define PYFOF_OPT_CATALOG 1
 #define PYFOF_OPT_DIRECT_MAX 2
 #define PYFOF_OPT_DIRECT_FP32 3
 #define PYFOF_OPT_BATCHED 4
 #define PYFOF_OPT_RECORD_PHI 5
+#define PYFOF_OPT_POISON_ALLOC 6
 int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value);
 
 /* First-pass potentials recorded by the last bound_check call on ctx (with

@@ -19,6 +19,7 @@ OPT_DIRECT_MAX = 2
 OPT_DIRECT_FP32 = 3
 OPT_BATCHED = 4
 OPT_RECORD_PHI = 5
+OPT_POISON_ALLOC = 6  # process-wide: 0 off, 1 + b fills new device allocations with byte b
 
 EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",

@@ -1,5 +1,6 @@
 // Per-device context: stream, events and the reusable HBM workspace.
 #pragma once
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -14,6 +15,9 @@ namespace pyfof {
 
 // Uploads the constant tables of the FoF kernels to the current device.
 void fof_init_device_tables();
+
+// PYFOF_OPT_POISON_ALLOC: 0, or 1 + the byte every new allocation is filled with
+inline std::atomic<int> g_poison_alloc{0};
 
 struct DevBuf {
     void* p = nullptr;
@@ -32,6 +36,12 @@ struct DevBuf {
                 throw HipError(hipErrorOutOfMemory, "device allocation failed");
             }
             cap = want;
+            const int poison = g_poison_alloc.load();
+            if (poison) {
+                // completes before any stream uses the buffer
+                PYFOF_HIP(hipMemset(p, poison - 1, want));
+                PYFOF_HIP(hipDeviceSynchronize());
+            }
         }
         return p;
     }

@@ -1712,6 +1712,13 @@ int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
         case PYFOF_OPT_DIRECT_FP32: C.direct_f32 = value != 0; return IERR_OK;
         case PYFOF_OPT_BATCHED: C.batched = value != 0; return IERR_OK;
         case PYFOF_OPT_RECORD_PHI: C.record_phi = value != 0; return IERR_OK;
+        case PYFOF_OPT_POISON_ALLOC:
+            if (value < 0 || value > 256) {
+                set_error("PYFOF_OPT_POISON_ALLOC takes 0 or 1 + a byte value (at most 256)");
+                return IERR_INVALID;
+            }
+            g_poison_alloc.store((int)value);
+            return IERR_OK;
         default: break;
     }
     set_error("unknown option");

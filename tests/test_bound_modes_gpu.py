@@ -113,33 +113,40 @@ def test_f32_potential_error_and_status():
     assert diff <= 1e-3 * max(1, int(r64["in_halo"].sum())), diff
 
 
-def test_result_independent_of_recycled_device_memory():
-    # buffers are reused and never cleared between calls: a kernel that read
-    # memory no pass had written (the zero padding of the convolution grids)
-    # would give results that depend on what earlier contexts left behind
-    from py_fof_amd import _native
+def test_no_reads_of_unwritten_device_memory():
+    # buffers are reused and never cleared between calls, so a kernel that
+    # read memory no pass had written (say the zero padding of a convolution
+    # grid) would depend on what earlier calls left there. New contexts whose
+    # allocations are filled with 0x00 and with 0xFF bytes must agree exactly,
+    # for the unbinding (all four grid classes, two-level off and on) and for
+    # the FoF linking (dense and sparse cells).
+    from py_fof_amd import _native, synth
     import py_fof_amd as pf
-    sizes = np.array([300, 1500, 5000, 12000, 40000, 90000])
-    xs, vs = _halos(sizes, 23)
+    xs, vs = _halos(np.array([300, 1500, 5000, 12000, 40000, 90000]), 23)
+    xc = synth.clumpy_plummer(n=200_000, L=48.0, n_clumps=120, seed=3)
 
-    def run(ctx):
-        ctx.set_option(_native.OPT_RECORD_PHI, 1)
-        r = pf._fof_core(xs, vs, L_Box=L, linking_length=0.25, np_halo_min=10, bound_check=True,
-                         return_ang_mom=True, softening=0.1, ng_global=4096.0, _ctx=ctx)
-        return r, ctx.potential()
+    def run(fill):
+        probe = _native.Context(0)
+        probe.set_option(_native.OPT_POISON_ALLOC, 1 + fill)
+        probe.close()
+        ctx = _native.Context(0)
+        try:
+            ctx.set_option(_native.OPT_RECORD_PHI, 1)
+            r = pf._fof_core(xs, vs, L_Box=L, linking_length=0.25, np_halo_min=10, bound_check=True,
+                             return_ang_mom=True, softening=0.1, ng_global=4096.0, _ctx=ctx)
+            phi = ctx.potential()
+            r2 = pf._fof_core(xs[-1:], vs[-1:], L_Box=L, linking_length=0.6, np_halo_min=10,
+                              bound_check=True, return_ang_mom=True, softening=0.05, ng_global=4096.0,
+                              _ctx=ctx)
+            f = pf._fof_core([xc], L_Box=48.0, linking_length=0.2, np_halo_min=10, _ctx=ctx)
+            return [a for a in list(r) + list(phi) + list(r2) + list(f) if a is not None]
+        finally:
+            ctx.close()
 
-    first = _native.Context(0)
-    ref = run(first)
-    # a context that fills its grids and frees them, then a new one
-    other = _native.Context(0)
-    big, bigv = _halos(np.array([200000, 60000, 30000]), 24)
-    pf._fof_core(big, bigv, L_Box=L, linking_length=0.25, np_halo_min=10, bound_check=True,
-                 softening=0.1, ng_global=4096.0, _ctx=other)
-    other.close()
-    late = _native.Context(0)
-    got = run(late)
-    for a, b in zip(ref[0], got[0]):
+    try:
+        zero = run(0x00)
+        ones = run(0xFF)
+    finally:
+        _native.Context(0).set_option(_native.OPT_POISON_ALLOC, 0)
+    for a, b in zip(zero, ones):
         assert np.array_equal(a, b)
-    assert np.array_equal(ref[1][0], got[1][0]) and np.array_equal(ref[1][1], got[1][1])
-    first.close()
-    late.close()
