@@ -962,11 +962,10 @@ __global__ void k_tl_targets(const float4* __restrict__ qw, const uint32_t* __re
 // class into their inner grids (local slot = slot's class index - s_lo)
 // sort keys of the source entries: slot rank (ratio-2 slots first, then
 // ratio 4, each by slot), then the 4^3-node block of the stencil's first
-// node; counts per slot rank for the chunk ranges
+// node
 __global__ void k_tl_src_keys(const int32_t* __restrict__ src_slot, const int4* __restrict__ src_first,
                               const TlSlot* __restrict__ slots, uint32_t nsrc, int32_t nslot2,
-                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                              uint32_t* __restrict__ rank_cnt) {
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t e = blockIdx.x * kB + threadIdx.x;
     if (e >= nsrc) return;
     const TlSlot S = slots[src_slot[e]];
@@ -975,7 +974,14 @@ __global__ void k_tl_src_keys(const int32_t* __restrict__ src_slot, const int4* 
     keys[e] = (rank << 18) | ((uint32_t)(f.z >> 2) & 63u) << 12 | ((uint32_t)(f.y >> 2) & 63u) << 6 |
               ((uint32_t)(f.x >> 2) & 63u);
     vals[e] = e;
-    atomicAdd(&rank_cnt[rank], 1u);
+}
+
+// end of each slot rank's run in the sorted source keys (rank = key >> 18)
+__global__ void k_rank_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t* __restrict__ ends) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = skeys[i] >> 18;
+    if (i + 1 == n || (skeys[i + 1] >> 18) != r) ends[r] = i + 1;
 }
 
 // deposit of the band sources of one chunk of inner grids (bound_fortran.f90:
@@ -1221,11 +1227,27 @@ __global__ void __launch_bounds__(kB) k_halo_mass(const float4* __restrict__ qw,
     }
 }
 
+// positions of the members in C2-cell order (qs[p] = qw[clist[p]]), mass 0
+// for members no longer kept
+__global__ void k_c2_sorted_pos(const float4* __restrict__ qw, const uint8_t* __restrict__ keep,
+                                const uint32_t* __restrict__ clist, uint32_t n, float4* __restrict__ qs) {
+    const uint32_t p = blockIdx.x * kB + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t t = clist[p];
+    float4 q = qw[t];
+    if (!keep[t]) q.w = 0.0f;
+    qs[p] = q;
+}
+
 // Full C2 residual over kept members (bound_fortran.f90:3403-3483), as a
 // gather: member i sums over the kept members j of its 27 neighbour cells.
 // Threads take members in cell order (clist), so the lanes of a wavefront
-// share neighbour cells: equal trip counts and broadcast loads.
-__global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+// share neighbour cells: equal trip counts and broadcast loads. Partners are
+// read from qs, the positions in the same cell order: one contiguous load
+// each; a partner of mass 0 (not kept) adds nothing and is skipped, which
+// leaves the sum bit for bit as skipping the removed members does.
+__global__ void k_c2_full(const float4* __restrict__ qw, const float4* __restrict__ qs,
+                          const uint32_t* __restrict__ mhalo,
                           const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
                           const uint8_t* __restrict__ keep, const uint8_t* __restrict__ active,
                           const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ cstart,
@@ -1265,9 +1287,9 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const uint32_t* __restr
                 if (x < 0 || x >= H.nc[0]) continue;
                 const uint32_t c = (uint32_t)H.c2off + (uint32_t)x + (uint32_t)H.nc[0] * ((uint32_t)y + (uint32_t)H.nc[1] * (uint32_t)z);
                 for (uint32_t p = cstart[c]; p < cstart[c + 1]; ++p) {
-                    const uint32_t j = clist[p];
-                    if (j == t || !keep[j]) continue;
-                    const float4 qj = qw[j];
+                    if (p == ps) continue;
+                    const float4 qj = qs[p];
+                    if (qj.w == 0.0f) continue;
                     if (H.tl) {  // two-level: the target's own cutoff 3.5 sp / r
                         acc = __fadd_rn(acc, tl_pair(qi, qj, Ke, Ka, factor));
                         continue;
@@ -2444,21 +2466,26 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                     uint32_t* sv = C.buf("tlsv").as<uint32_t>(nsrc);
                     uint32_t* sk2 = C.buf("tlsk2").as<uint32_t>(nsrc);
                     uint32_t* sv2 = C.buf("tlsv2").as<uint32_t>(nsrc);
-                    uint32_t* rcnt = C.buf("tlrcnt").as<uint32_t>(nrank);
-                    void* stmp = C.buf("tlstmp").get(radix_temp_bytes(nsrc) + 256);
-                    k_fill32<<<blocks(nrank), kB, 0, s>>>(rcnt, nrank, 0u);
-                    PYFOF_LAUNCH_CHECK();
+                    uint32_t* rend = C.buf("tlrend").as<uint32_t>(nrank);
+                    void* stmp = C.buf("tlstmp").get(std::max(radix_temp_bytes(nsrc), scan_temp_bytes(nrank)) + 256);
                     k_tl_src_keys<<<blocks(nsrc), kB, 0, s>>>(src_slot, src_first, d_tlslots, nsrc, tl_nslot[0], sk,
-                                                              sv, rcnt);
+                                                              sv);
                     PYFOF_LAUNCH_CHECK();
                     const bool salt = radix_sort_pairs(sk, sv, sk2, sv2, nsrc, bits_for(((uint64_t)nrank << 18) - 1),
                                                        stmp, s);
                     src_order = salt ? sv2 : sv;
-                    std::vector<uint32_t> rc(nrank);
-                    PYFOF_HIP(hipMemcpyAsync(rc.data(), rcnt, 4 * (size_t)nrank, hipMemcpyDeviceToHost, s));
+                    // rank r's entries end at rend[r] (empty ranks: the end of
+                    // the previous one, by a max-scan)
+                    k_fill32<<<blocks(nrank), kB, 0, s>>>(rend, nrank, 0u);
+                    PYFOF_LAUNCH_CHECK();
+                    k_rank_ends<<<blocks(nsrc), kB, 0, s>>>(salt ? sk2 : sk, nsrc, rend);
+                    PYFOF_LAUNCH_CHECK();
+                    inclusive_max_scan_u32(rend, rend, nrank, stmp, s);
+                    std::vector<uint32_t> re(nrank);
+                    PYFOF_HIP(hipMemcpyAsync(re.data(), rend, 4 * (size_t)nrank, hipMemcpyDeviceToHost, s));
                     PYFOF_HIP(hipStreamSynchronize(s));
                     rank_start.assign(nrank + 1, 0u);
-                    for (uint32_t r = 0; r < nrank; ++r) rank_start[r + 1] = rank_start[r] + rc[r];
+                    for (uint32_t r = 0; r < nrank; ++r) rank_start[r + 1] = re[r];
                 }
                 // inner kernels (bound_init_inner_gk) and targets
                 for (int c = 0; c < 2; ++c) {
@@ -2596,7 +2623,10 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                     }
                 }
                 if (pass == 1) {
-                    k_c2_full<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
+                    float4* qs = C.buf("c2qs").as<float4>(M);
+                    k_c2_sorted_pos<<<blocks(M), kB, 0, s>>>(qw, keep, clist, M, qs);
+                    PYFOF_LAUNCH_CHECK();
+                    k_c2_full<<<blocks(M), kB, 0, s>>>(qw, qs, A.mhalo, d_pmof, d_pm, keep, active, cell_of, cstart,
                                                        clist, M, A.a, A.app, pr, phi_c2);
                     PYFOF_LAUNCH_CHECK();
                 } else {
