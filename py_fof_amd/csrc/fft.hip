@@ -209,6 +209,7 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
                                                         const float* __restrict__ G,
                                                         const float2* __restrict__ tw, ConvDesc d) {
     __shared__ float2 buf[2][NL * N];  // LDS sized for this grid: more workgroups per CU for small N
+    __shared__ float2 tws[N];          // twiddles, read by every stage
     constexpr int total = NL * N;
     constexpr int kPer = (total + kFftBlock - 1) / kFftBlock;  // elements per thread at most
     const AxisDesc& ax = d.ax;
@@ -221,6 +222,8 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
     const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
     const int line0 = ax.x_contig ? i0 * NL : 0;  // y of line 0 in the x passes
     const int tid = threadIdx.x;
+    // visible to the stages after the barrier that follows the first load
+    for (int i = tid; i < N; i += kFftBlock) tws[i] = tw[i];
     float gk[kPer];
     if (MODE == kConvZ) {
 #pragma unroll
@@ -283,9 +286,9 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
         }
         int cur;
         if (MODE == kConvXFwd || MODE == kConvYFwd) {
-            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tw);
+            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tws);
         } else if (MODE == kConvZ) {
-            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tw);
+            cur = stages_ct<false, NL, N, 0, 1>(buf, 0, tws);
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const int idx = tid + k * kFftBlock;
@@ -296,9 +299,9 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
                 }
             }
             __syncthreads();
-            cur = stages_ct<true, NL, N, 0, 1>(buf, cur, tw);
+            cur = stages_ct<true, NL, N, 0, 1>(buf, cur, tws);
         } else {
-            cur = stages_ct<true, NL, N, 0, 1>(buf, 0, tw);
+            cur = stages_ct<true, NL, N, 0, 1>(buf, 0, tws);
         }
         for (int idx = tid; idx < total; idx += kFftBlock) {
             int c, e;
