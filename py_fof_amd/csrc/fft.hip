@@ -197,9 +197,9 @@ struct ConvDesc {
     int32_t out_lim;   // elements e > out_lim are not stored
 };
 
-// NL lines per workgroup (8 or 16: a row segment of 16 float2 is one 128 B
-// line). The z pass runs every listed slot in one workgroup: the spectrum G
-// of its lines is read once into registers and reused for all slots.
+// NL lines per workgroup. The z pass runs every listed slot in one
+// workgroup: the spectrum G of its lines is read once into registers and
+// reused for all slots.
 template <int MODE, int NL, int N>
 __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ data,
                                                         const unsigned long long* __restrict__ acc,
@@ -317,25 +317,29 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
 template <int N>
 void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
                  int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G, hipStream_t s) {
-    constexpr int NLYZ = N % 16 == 0 ? 16 : kLines;  // 128 B row segments when N allows
+    // 8 lines per workgroup in every pass: 16 (whole 128 B row segments)
+    // doubles the LDS per workgroup and halves the resident workgroups, which
+    // measured slower (C3 unbinding 151 vs 131 ms per call)
+    constexpr int NLY = kLines;
+    constexpr int NLZ = kLines;
     constexpr int m = N / 2 + 1, mc = m + 1;
     constexpr int64_t n2 = (int64_t)N * N;
     constexpr int ygroups = (mc + kLines - 1) / kLines;
     const AxisDesc ax_x{(int64_t)kLines * N, n2, ygroups, mc, N, 1, 1};
-    const AxisDesc ax_y{NLYZ, n2, N / NLYZ, mc, 1, N, 0};
-    const AxisDesc ax_z{NLYZ, N, N / NLYZ, N, 1, n2, 0};
+    const AxisDesc ax_y{NLY, n2, N / NLY, mc, 1, N, 0};
+    const AxisDesc ax_z{NLZ, N, N / NLZ, N, 1, n2, 0};
     const ConvDesc dxf{ax_x, m, m, N - 1}, dxi{ax_x, m, N - 1, m};
     const ConvDesc dyf{ax_y, N, m, N - 1}, dyi{ax_y, N, N - 1, m}, dz{ax_z, N, m, m};
     const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
-    const uint32_t gy = (uint32_t)((N / NLYZ) * mc) * (uint32_t)nslots;
-    const uint32_t gzz = (uint32_t)((N / NLYZ) * N);
+    const uint32_t gy = (uint32_t)((N / NLY) * mc) * (uint32_t)nslots;
+    const uint32_t gzz = (uint32_t)((N / NLZ) * N);
     k_fft_conv<kConvXFwd, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvYFwd, NLYZ, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
+    k_fft_conv<kConvYFwd, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvZ, NLYZ, N><<<gzz, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dz);
+    k_fft_conv<kConvZ, NLZ, N><<<gzz, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dz);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvYInv, NLYZ, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyi);
+    k_fft_conv<kConvYInv, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyi);
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvXInv, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxi);
     PYFOF_LAUNCH_CHECK();
