@@ -182,11 +182,13 @@ __device__ __forceinline__ int stages_ct(float2 (*buf)[NL * N], int cur, const f
 // m = n/2 + 1 and only those nodes are read back, so each pass touches only
 // the lines that can be non-zero or are needed later, and zero-fills the rest
 // in LDS:
-//   x forward  lines y, z <= m, input x <= m read from the int64 deposit
-//   y forward  lines z <= m, input y <= m
-//   z          all lines: forward, x G(k), inverse; output z <= m
-//   y inverse  lines z <= m; output y <= m
-//   x inverse  lines y, z <= m; output x <= m
+//   x forward  lines y, z <= m, input x <= m read from the int64 deposit;
+//              output kx <= N/2 (the grid is real: X(N - k) = conj X(k))
+//   y forward  lines kx <= N/2, z <= m, input y <= m
+//   z          lines kx <= N/2: forward, x G(k), inverse; output z <= m
+//   y inverse  lines kx <= N/2, z <= m; output y <= m
+//   x inverse  lines y, z <= m; input kx <= N/2, the rest by conjugation;
+//              output x <= m
 
 enum { kConvXFwd = 0, kConvYFwd = 1, kConvZ = 2, kConvYInv = 3, kConvXInv = 4 };
 
@@ -220,7 +222,8 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
     const int32_t ib0 = MODE == kConvZ ? 0 : g / (ax.c0 * ax.c1);
     const int32_t ib1 = MODE == kConvZ ? nslots : ib0 + 1;
     const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
-    const int line0 = ax.x_contig ? i0 * NL : 0;  // y of line 0 in the x passes
+    // first line of the workgroup: y in the x passes, x in the y and z passes
+    const int line0 = i0 * NL;
     const int tid = threadIdx.x;
     // visible to the stages after the barrier that follows the first load
     for (int i = tid; i < N; i += kFftBlock) tws[i] = tw[i];
@@ -246,10 +249,11 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             const int idx = tid + k * kFftBlock;
             if (idx < total) {
                 const int e = idx / NL, c = idx - e * NL;
-                // elements above in_lim are the zero padding: never stored
-                // by the forward passes, so they are not read
-                pre[k] = e <= d.in_lim ? data[sb + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride]
-                                       : make_float2(0.0f, 0.0f);
+                // elements above in_lim are the zero padding and lines above
+                // line_lim the conjugate half: never stored, so not read
+                pre[k] = e <= d.in_lim && line0 + c <= d.line_lim
+                             ? data[sb + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride]
+                             : make_float2(0.0f, 0.0f);
             }
         }
     };
@@ -275,7 +279,7 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             if (ax.x_contig) { c = idx / N; e = idx - c * N; }
             else { e = idx / NL; c = idx - e * NL; }
             float2 v = make_float2(0.0f, 0.0f);
-            if (e <= d.in_lim && (!ax.x_contig || line0 + c <= d.line_lim)) {
+            if (e <= d.in_lim && line0 + c <= d.line_lim) {
                 const int64_t o = sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride;
                 if (MODE == kConvXFwd) v = make_float2((float)((double)(long long)acc[o] * inv), 0.0f);
                 else v = data[o];
@@ -283,6 +287,17 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             buf[0][c * N + e] = v;
         }
         __syncthreads();
+        if (MODE == kConvXInv) {
+            // the stored half kx <= N/2 of a Hermitian line: X(N - k) = conj X(k)
+            for (int idx = tid; idx < total; idx += kFftBlock) {
+                const int c = idx / N, e = idx - c * N;
+                if (e > N / 2) {
+                    const float2 v = buf[0][c * N + N - e];
+                    buf[0][c * N + e] = make_float2(v.x, -v.y);
+                }
+            }
+            __syncthreads();
+        }
         }
         int cur;
         if (MODE == kConvXFwd || MODE == kConvYFwd) {
@@ -307,7 +322,7 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
             int c, e;
             if (ax.x_contig) { c = idx / N; e = idx - c * N; }
             else { e = idx / NL; c = idx - e * NL; }
-            if (e > d.out_lim || (ax.x_contig && line0 + c > d.line_lim)) continue;
+            if (e > d.out_lim || line0 + c > d.line_lim) continue;
             data[sbase + goff + (int64_t)c * ax.line_stride + (int64_t)e * ax.elem_stride] = buf[cur][c * N + e];
         }
         __syncthreads();  // buf is reused by the next slot
@@ -325,14 +340,20 @@ void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, 
     constexpr int m = N / 2 + 1, mc = m + 1;
     constexpr int64_t n2 = (int64_t)N * N;
     constexpr int ygroups = (mc + kLines - 1) / kLines;
+    // the grids are real and G is real and even, so every spectrum is
+    // Hermitian along x: the y and z passes keep kx <= N/2 only and the x
+    // inverse rebuilds the other half
+    constexpr int hx = N / 2;
+    constexpr int xgroups_y = (hx + 1 + NLY - 1) / NLY, xgroups_z = (hx + 1 + NLZ - 1) / NLZ;
     const AxisDesc ax_x{(int64_t)kLines * N, n2, ygroups, mc, N, 1, 1};
-    const AxisDesc ax_y{NLY, n2, N / NLY, mc, 1, N, 0};
-    const AxisDesc ax_z{NLZ, N, N / NLZ, N, 1, n2, 0};
-    const ConvDesc dxf{ax_x, m, m, N - 1}, dxi{ax_x, m, N - 1, m};
-    const ConvDesc dyf{ax_y, N, m, N - 1}, dyi{ax_y, N, N - 1, m}, dz{ax_z, N, m, m};
+    const AxisDesc ax_y{NLY, n2, xgroups_y, mc, 1, N, 0};
+    const AxisDesc ax_z{NLZ, N, xgroups_z, N, 1, n2, 0};
+    // {axis, line_lim, in_lim, out_lim}
+    const ConvDesc dxf{ax_x, m, m, hx}, dxi{ax_x, m, hx, m};
+    const ConvDesc dyf{ax_y, hx, m, N - 1}, dyi{ax_y, hx, N - 1, m}, dz{ax_z, hx, m, m};
     const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
-    const uint32_t gy = (uint32_t)((N / NLY) * mc) * (uint32_t)nslots;
-    const uint32_t gzz = (uint32_t)((N / NLZ) * N);
+    const uint32_t gy = (uint32_t)(xgroups_y * mc) * (uint32_t)nslots;
+    const uint32_t gzz = (uint32_t)(xgroups_z * N);
     k_fft_conv<kConvXFwd, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvYFwd, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
