@@ -650,19 +650,25 @@ __global__ void k_tsc_setup(const float4* __restrict__ qw, const uint32_t* __res
 // depend on the order of the additions. Members come in C2-cell order (order
 // = the class's range of the C2 member list), so the 256 members of a
 // workgroup are neighbours: when they belong to one halo and their stencils
-// fit in a box of kDepBox nodes, they are summed into that box in LDS first
+// fit in a box of BOX nodes, they are summed into that box in LDS first
 // and the box is added to the grid once per node; otherwise each member adds
 // its 27 nodes to the grid directly.
-constexpr int kDepBox = 4096;
+// workgroup size and LDS box (nodes) of the two deposits: the two-level
+// sources are sparser per node block, so a 256-entry run spans more nodes
+// than a 4096-node box; runs of 64 fit a 1024-node box far more often
+// (measured: C3 unbinding 113.5 -> 109.9 ms per call)
+constexpr int kDepTB = 256, kDepBox = 4096;
+constexpr int kTlDepTB = 64, kTlDepBox = 1024;
 
-__global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
+template <int TB, int BOX>
+__global__ void __launch_bounds__(TB) k_deposit(const float4* __restrict__ qw, const uint32_t* __restrict__ mhalo,
                                                 const int32_t* __restrict__ pm_of_halo, const PmHalo* __restrict__ pm,
                                                 const uint8_t* __restrict__ keep, const int4* __restrict__ first,
                                                 const float* __restrict__ wt, const uint8_t* __restrict__ active,
                                                 const double* __restrict__ fscale, int32_t cls,
                                                 const uint32_t* __restrict__ order, uint32_t count,
                                                 unsigned long long* __restrict__ acc) {
-    __shared__ unsigned long long box[kDepBox];
+    __shared__ unsigned long long box[BOX];
     __shared__ int32_t blo[3], bhi[3], bslot, bn, mixed;
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) { blo[d] = 0x7fffffff; bhi[d] = -0x7fffffff; }
@@ -671,7 +677,7 @@ __global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, c
         mixed = 0;
     }
     __syncthreads();
-    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    const uint32_t ps = blockIdx.x * TB + threadIdx.x;
     bool dep = false;
     uint32_t t = 0, h = 0;
     int32_t slot = -1, n = 0;
@@ -703,7 +709,7 @@ __global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, c
     const float* w9 = wt + 9 * (size_t)t;
     const double fs = dep ? fscale[h] : 0.0;
     const int ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1, ez = bhi[2] - blo[2] + 1;
-    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= kDepBox;
+    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= BOX;
     if (!in_lds) {
         if (!dep) return;
         unsigned long long* g = acc + (size_t)slot * n * n * n;
@@ -718,7 +724,7 @@ __global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, c
         return;
     }
     const int vol = ex * ey * ez;
-    for (int i = threadIdx.x; i < vol; i += kB) box[i] = 0ull;
+    for (int i = threadIdx.x; i < vol; i += TB) box[i] = 0ull;
     __syncthreads();
     if (dep) {
         for (int sz = 0; sz < 3; ++sz)
@@ -733,7 +739,7 @@ __global__ void __launch_bounds__(kB) k_deposit(const float4* __restrict__ qw, c
     __syncthreads();
     const int nn = bn;
     unsigned long long* g = acc + (size_t)bslot * nn * nn * nn;
-    for (int i = threadIdx.x; i < vol; i += kB) {
+    for (int i = threadIdx.x; i < vol; i += TB) {
         const unsigned long long v = box[i];
         if (v == 0ull) continue;  // no member reached this node (all terms are >= 0)
         const int x = blo[0] + i % ex, y = blo[1] + (i / ex) % ey, z = blo[2] + i / (ex * ey);
@@ -988,8 +994,9 @@ __global__ void k_rank_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint
 // 895-904), as int64 fixed point. With order (the entries sorted by slot and
 // position) the launch covers the chunk's range only and a workgroup's
 // entries are neighbours in one inner grid: their stencils are summed in an
-// LDS box when it fits (kDepBox nodes), as in k_deposit.
-__global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ src_slot,
+// LDS box when it fits (kTlDepBox nodes), as in k_deposit.
+template <int TB, int BOX>
+__global__ void __launch_bounds__(TB) k_tl_deposit(const int32_t* __restrict__ src_slot,
                                                    const uint32_t* __restrict__ src_member,
                                                    const int4* __restrict__ src_first,
                                                    const float* __restrict__ src_wt,
@@ -1001,7 +1008,7 @@ __global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ s
                                                    const float4* __restrict__ qw,
                                                    const double* __restrict__ fscale,
                                                    unsigned long long* __restrict__ acc) {
-    __shared__ unsigned long long box[kDepBox];
+    __shared__ unsigned long long box[BOX];
     __shared__ int32_t blo[3], bhi[3], bslot, bn, mixed;
     if (threadIdx.x == 0) {
         for (int d = 0; d < 3; ++d) { blo[d] = 0x7fffffff; bhi[d] = -0x7fffffff; }
@@ -1010,7 +1017,7 @@ __global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ s
         mixed = 0;
     }
     __syncthreads();
-    const uint32_t ps = blockIdx.x * kB + threadIdx.x;
+    const uint32_t ps = blockIdx.x * TB + threadIdx.x;
     bool dep = false;
     uint32_t e = 0, t = 0;
     int32_t slot = -1, n = 0, halo = 0;
@@ -1041,7 +1048,7 @@ __global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ s
     const float* w9 = src_wt + 9 * (size_t)e;
     const double fs = dep ? fscale[halo] : 0.0;
     const int ex = bhi[0] - blo[0] + 1, ey = bhi[1] - blo[1] + 1, ez = bhi[2] - blo[2] + 1;
-    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= kDepBox;
+    const bool in_lds = bslot >= 0 && !mixed && (int64_t)ex * ey * ez <= BOX;
     if (!in_lds) {
         if (!dep) return;
         unsigned long long* g = acc + (size_t)slot * n * n * n;
@@ -1056,7 +1063,7 @@ __global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ s
         return;
     }
     const int vol = ex * ey * ez;
-    for (int i = threadIdx.x; i < vol; i += kB) box[i] = 0ull;
+    for (int i = threadIdx.x; i < vol; i += TB) box[i] = 0ull;
     __syncthreads();
     if (dep) {
         for (int sz = 0; sz < 3; ++sz)
@@ -1071,7 +1078,7 @@ __global__ void __launch_bounds__(kB) k_tl_deposit(const int32_t* __restrict__ s
     __syncthreads();
     const int nn = bn;
     unsigned long long* g = acc + (size_t)bslot * nn * nn * nn;
-    for (int i = threadIdx.x; i < vol; i += kB) {
+    for (int i = threadIdx.x; i < vol; i += TB) {
         const unsigned long long v = box[i];
         if (v == 0ull) continue;
         const int x = blo[0] + i % ex, y = blo[1] + (i / ex) % ey, z = blo[2] + i / (ex * ey);
@@ -2570,7 +2577,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                     const dim3 gz((uint32_t)blocks((uint64_t)mc * mc * mc), (uint32_t)sl.size());
                     k_zero_region<<<gz, kB, 0, s>>>(acc[g], d_slots, n, mc);
                     PYFOF_LAUNCH_CHECK();
-                    k_deposit<<<blocks(cls_count[g]), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt,
+                    k_deposit<kDepTB, kDepBox><<<(uint32_t)((cls_count[g] + kDepTB - 1) / kDepTB), kDepTB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt,
                                                                   active, fscale, g, clist + cls_first[g],
                                                                   cls_count[g], acc[g]);
                     PYFOF_LAUNCH_CHECK();
@@ -2608,7 +2615,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
                             scount = rank_start[r_hi] - sbeg;
                         }
                         if (scount > 0) {
-                            k_tl_deposit<<<blocks(scount), kB, 0, s>>>(src_slot, src_member, src_first, src_wt,
+                            k_tl_deposit<kTlDepTB, kTlDepBox><<<(scount + kTlDepTB - 1) / kTlDepTB, kTlDepTB, 0, s>>>(src_slot, src_member, src_first, src_wt,
                                                                        src_order, sbeg, scount, d_tlslots, ratio, cs,
                                                                        cs + tl_chunk[c], keep, active, qw, fscale,
                                                                        acc_in[c]);
