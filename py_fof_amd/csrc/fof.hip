@@ -149,19 +149,61 @@ __global__ void __launch_bounds__(kBlk) k_gather(const double* __restrict__ xp,
 }
 
 // marks cells with at least kDenseMin particles in regions with sub-cells
-__global__ void k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
+// aux[c]: the dense index of a dense cell (>= 0); kAuxMid for a cell whose
+// own pairs k_cell_intra resolves; -1 otherwise. Negative values read as
+// "not dense" everywhere.
+constexpr int32_t kAuxMid = -2;
+constexpr uint32_t kMidMin = 16, kMidMax = 255;  // particles of a k_cell_intra cell
+constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
+
+__global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
                              const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg,
                              int32_t nreg, int32_t* __restrict__ aux, uint32_t* __restrict__ dense_cells,
-                             uint32_t* __restrict__ ndense) {
+                             uint32_t* __restrict__ ndense, uint32_t* __restrict__ mid_cells,
+                             uint32_t* __restrict__ nmid) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dense = false, mid = false;
+    if (c < ncell) {
+        const uint32_t cnt = cell_ptr[c + 1] - cell_ptr[c];
+        if (cnt >= kMidMin) {
+            const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+            dense = !g.skip && cnt >= (uint32_t)kDenseMin && g.prune && g.k[0] * g.k[1] * g.k[2] > 1;
+            mid = !dense && !g.skip && cnt <= kMidMax;
+        }
+    }
+    // one counter update per workgroup and list: wave counts by ballot,
+    // their prefix in LDS, lanes placed by ballot rank
+    __shared__ uint32_t wd[kMarkBlk / kWave], wm[kMarkBlk / kWave], bases[2];
+    const int w = threadIdx.x / kWave;
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    const uint64_t bd = __ballot(dense), bm = __ballot(mid);
+    if (__lane_id() == 0) {
+        wd[w] = (uint32_t)__popcll(bd);
+        wm[w] = (uint32_t)__popcll(bm);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t td = 0, tm = 0;
+        for (int k = 0; k < kMarkBlk / kWave; ++k) {
+            const uint32_t a = wd[k], b = wm[k];
+            wd[k] = td;
+            wm[k] = tm;
+            td += a;
+            tm += b;
+        }
+        bases[0] = td ? atomicAdd(ndense, td) : 0u;
+        bases[1] = tm ? atomicAdd(nmid, tm) : 0u;
+    }
+    __syncthreads();
+    const uint32_t based = bases[0] + wd[w], basem = bases[1] + wm[w];
     if (c >= ncell) return;
     int32_t d = -1;
-    if (cell_ptr[c + 1] - cell_ptr[c] >= (uint32_t)kDenseMin) {
-        const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
-        if (!g.skip && g.prune && g.k[0] * g.k[1] * g.k[2] > 1) {
-            d = (int32_t)atomicAdd(ndense, 1u);
-            dense_cells[d] = c;
-        }
+    if (dense) {
+        d = (int32_t)(based + (uint32_t)__popcll(bd & lt));
+        dense_cells[d] = c;
+    } else if (mid) {
+        d = kAuxMid;
+        mid_cells[basem + (uint32_t)__popcll(bm & lt)] = c;
     }
     aux[c] = d;
 }
@@ -292,10 +334,12 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
     }
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
     if (s >= e) return;
-    if (aux[jc] >= 0) {
+    const int32_t ac = aux[jc];
+    if (ac >= 0) {
         L.dmask |= 1u << o;
         return;
     }
+    if (own && ac == kAuxMid) return;  // own pairs resolved by k_cell_intra
     uint32_t qq = own ? p + 1 : s;
     // two partners at a time: their positions, and then the parents of the
     // linked ones, are loaded together; a partner whose parent is already the
@@ -581,6 +625,45 @@ __device__ __forceinline__ void lds_unite(int32_t* lp, int32_t a, int32_t b) {
         const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
         if (atomicCAS(&lp[hi], hi, lo) == hi) return;
     }
+}
+
+// One wave per cell of kMidMin..kMidMax particles that is not dense: every
+// pair of its members is tested (the same predicate as k_link's own-cell
+// loop) and joined in a union-find in LDS; each member's parent becomes the
+// smallest position of its local component, a valid state for the global
+// union-find (parents point to smaller positions). k_link then skips these
+// cells' own pairs.
+template <bool PERIODIC>
+__device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const float4* __restrict__ xc,
+                           int32_t* parent, float4* pos, int32_t* lp) {
+    const int lane = __lane_id();
+    for (uint32_t i = lane; i < n; i += kWave) {
+        pos[i] = xc[s + i];
+        lp[i] = (int32_t)i;
+    }
+    __syncthreads();
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        const float4 a = pos[i];
+        for (uint32_t j = i + 1 + lane; j < n; j += kWave)
+            if (pair_linked<PERIODIC>(a, pos[j], P)) lds_unite(lp, (int32_t)i, (int32_t)j);
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += kWave) parent[s + i] = (int32_t)s + lds_find(lp, (int32_t)i);
+}
+
+__global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict__ mid_cells,
+                                                      const uint32_t* __restrict__ cell_ptr,
+                                                      const uint32_t* __restrict__ cell_off,
+                                                      const RegionDev* __restrict__ reg, int32_t nreg,
+                                                      const float4* __restrict__ xc, int32_t* parent,
+                                                      LinkParams P) {
+    __shared__ float4 pos[kMidMax];
+    __shared__ int32_t lp[kMidMax];
+    const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
+    const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+    const uint32_t s = cell_ptr[c], n = cell_ptr[c + 1] - s;
+    if (g.is_periodic) cell_intra<true>(s, n, P, xc, parent, pos, lp);
+    else cell_intra<false>(s, n, P, xc, parent, pos, lp);
 }
 
 template <bool PERIODIC>
@@ -1326,12 +1409,14 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* scal = C.scal.as<uint32_t>(64);
         k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
         PYFOF_LAUNCH_CHECK();
-        k_dense_mark<<<nblk(ncell), kBlk, 0, s>>>(cell_ptr, (uint32_t)ncell, d_coff, d_reg, H, aux,
-                                                   dense_cells, scal);
+        uint32_t* mid_cells = C.mid.as<uint32_t>(N / kMidMin + 1);
+        k_dense_mark<<<(uint32_t)((ncell + kMarkBlk - 1) / kMarkBlk), kMarkBlk, 0, s>>>(cell_ptr, (uint32_t)ncell, d_coff, d_reg, H, aux,
+                                                   dense_cells, scal, mid_cells, scal + 8);
         PYFOF_LAUNCH_CHECK();
-        uint32_t ndense = 0;
-        PYFOF_HIP(hipMemcpyAsync(&ndense, scal, 4, hipMemcpyDeviceToHost, s));
+        uint32_t cnts[9];
+        PYFOF_HIP(hipMemcpyAsync(cnts, scal, sizeof(cnts), hipMemcpyDeviceToHost, s));
         PYFOF_HIP(hipStreamSynchronize(s));
+        const uint32_t ndense = cnts[0], nmid = cnts[8];
         uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
         uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);
@@ -1365,6 +1450,10 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // per-particle masks of dense cells to scan, in the buffer that holds
         // the labels after linking
         uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
+        if (nmid > 0) {
+            k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
+            PYFOF_LAUNCH_CHECK();
+        }
         k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         if (ndense > 0) {
