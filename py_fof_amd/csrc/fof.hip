@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restr
         if (cnt >= kMidMin) {
             const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
             dense = !g.skip && cnt >= (uint32_t)kDenseMin && g.prune && g.k[0] * g.k[1] * g.k[2] > 1;
-            mid = !dense && !g.skip && cnt <= kMidMax;
+            mid = !dense && !g.skip && g.prune && cnt <= kMidMax;
         }
     }
     // one counter update per workgroup and list: wave counts by ballot,
@@ -321,7 +321,7 @@ __device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, co
 // register footprint of this kernel, and so its occupancy, low.
 template <bool PERIODIC>
 __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
-                                           const int32_t jijk[3], const double q[3], bool own,
+                                           const int32_t jijk[3], const double q[3], bool own, bool mid_self,
                                            const RegionDev& g, const LinkParams& P,
                                            const uint32_t* __restrict__ cell_ptr,
                                            const int32_t* __restrict__ aux,
@@ -339,7 +339,9 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
         L.dmask |= 1u << o;
         return;
     }
-    if (own && ac == kAuxMid) return;  // own pairs resolved by k_cell_intra
+    // a mid cell's own pairs are k_cell_intra's, its pairs with non-dense
+    // neighbours k_cell_pair's
+    if (mid_self) return;
     uint32_t qq = own ? p + 1 : s;
     // two partners at a time: their positions, and then the parents of the
     // linked ones, are loaded together; a partner whose parent is already the
@@ -395,7 +397,8 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, g, P, cell_ptr, aux, xc, L);
+    const bool mid_self = aux[c] == kAuxMid;
+    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, mid_self, g, P, cell_ptr, aux, xc, L);
     auto neighbour = [&](int o) {
         int32_t jijk[3];
         double sh[3];
@@ -416,7 +419,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
             }
         }
         const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, g, P, cell_ptr, aux, xc, L);
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, mid_self, g, P, cell_ptr, aux, xc, L);
     };
     if (!g.prune) {
         for (int o = 0; o < 27; ++o)
@@ -664,6 +667,104 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
     const uint32_t s = cell_ptr[c], n = cell_ptr[c + 1] - s;
     if (g.is_periodic) cell_intra<true>(s, n, P, xc, parent, pos, lp);
     else cell_intra<false>(s, n, P, xc, parent, pos, lp);
+}
+
+// One wave per mid cell A (kAuxMid): the pairs between A and each forward
+// neighbour cell B that is not dense (the cell-pair rule of k_link: the lower
+// cell takes the pair). Only members within R = b + m of the shared face,
+// edge or corner can link, on either side, so each side is filtered to those
+// first (the same margins as k_link's pruning) and the remaining pairs are
+// tested by all lanes. k_link keeps, for A's particles, only the dense
+// neighbours (its mask for k_link_dense).
+template <bool PERIODIC>
+__device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
+                          const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
+                          const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
+                          uint16_t* ia, uint16_t* ib, uint32_t* cnt) {
+    const int lane = __lane_id();
+    const uint32_t lc = c - g.cell_off;
+    int32_t ijk[3];
+    double lo[3];
+    cell_lower_corner(g, lc, lo, ijk);
+    const uint32_t sA = cell_ptr[c], nA = cell_ptr[c + 1] - sA;
+    for (uint32_t i = lane; i < nA; i += kWave) pa[i] = xc[sA + i];
+    __syncthreads();
+    for (int o = 0; o < 27; ++o) {
+        if (o == 13) continue;
+        int32_t jijk[3];
+        double sh[3];
+        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
+        const uint32_t jl = local_cell(g, jijk);
+        if (jl <= lc) continue;
+        const uint32_t cB = g.cell_off + jl;
+        if (aux[cB] >= 0) continue;  // dense: through k_link's mask
+        const uint32_t sB = cell_ptr[cB], nB = cell_ptr[cB + 1] - sB;
+        if (nB == 0) continue;
+        const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+        double loB[3];
+        for (int d = 0; d < 3; ++d) loB[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
+        if (lane == 0) cnt[0] = 0u;
+        __syncthreads();
+        for (uint32_t i = lane; i < nA; i += kWave) {
+            const float4 a = pa[i];
+            const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) {
+                if (off[d] < 0) ok = ok && x[d] - P.R <= lo[d] + 2.0 * P.m;
+                else if (off[d] > 0) ok = ok && x[d] + P.R >= lo[d] + g.cell[d] - 2.0 * P.m;
+            }
+            if (ok) ia[atomicAdd(&cnt[0], 1u)] = (uint16_t)i;
+        }
+        __syncthreads();
+        const uint32_t na = cnt[0];
+        if (na == 0) continue;
+        // B in chunks of at most kMidMax members (a non-dense B can be larger)
+        for (uint32_t b0 = 0; b0 < nB; b0 += kMidMax) {
+            const uint32_t b1 = min(nB, b0 + kMidMax);
+            if (lane == 0) cnt[1] = 0u;
+            __syncthreads();
+            for (uint32_t i = b0 + lane; i < b1; i += kWave) {
+                const float4 q = xc[sB + i];
+                const double x[3] = {(double)q.x, (double)q.y, (double)q.z};
+                bool ok = true;
+                for (int d = 0; d < 3; ++d) {
+                    if (off[d] > 0) ok = ok && x[d] - P.R <= loB[d] + 2.0 * P.m;
+                    else if (off[d] < 0) ok = ok && x[d] + P.R >= loB[d] + g.cell[d] - 2.0 * P.m;
+                }
+                if (ok) {
+                    const uint32_t k = atomicAdd(&cnt[1], 1u);
+                    ib[k] = (uint16_t)(i - b0);
+                    pb[k] = q;
+                }
+            }
+            __syncthreads();
+            const uint32_t nb = cnt[1];
+            for (uint32_t t = lane; t < na * nb; t += kWave) {
+                const uint32_t i = t / nb, j = t - i * nb;
+                if (pair_linked<PERIODIC>(pa[ia[i]], pb[j], P)) {
+                    LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
+                    L.link((int32_t)(sB + b0 + ib[j]));
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict__ mid_cells,
+                                                     const uint32_t* __restrict__ cell_ptr,
+                                                     const uint32_t* __restrict__ cell_off,
+                                                     const RegionDev* __restrict__ reg, int32_t nreg,
+                                                     const int32_t* __restrict__ aux,
+                                                     const float4* __restrict__ xc, int32_t* parent,
+                                                     LinkParams P) {
+    __shared__ float4 pa[kMidMax], pb[kMidMax];
+    __shared__ uint16_t ia[kMidMax], ib[kMidMax];
+    __shared__ uint32_t cnt[2];
+    const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
+    const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+    if (g.is_periodic) cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, cnt);
+    else cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, cnt);
 }
 
 template <bool PERIODIC>
@@ -1452,6 +1553,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
+            PYFOF_LAUNCH_CHECK();
+            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
