@@ -1280,6 +1280,10 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const float4* __restric
     Ke.init(eps);
     Ka.init(ai);
     const double spd = (double)sp, inv_sp = 1.0 / spd;
+    // f32 screen: the f32 sum of squares is within ~4e-7 of r^2, so pairs
+    // above cut^2 (1 + 1e-6) are outside the cutoff in the f64 test as well
+    // (and add nothing); the rest take the f64 test unchanged
+    const float cut32 = nextafterf((float)((H.tl ? ai * ai : r2cut) * (1.0 + 1e-6)), INFINITY);
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -1297,6 +1301,10 @@ __global__ void k_c2_full(const float4* __restrict__ qw, const float4* __restric
                     if (p == ps) continue;
                     const float4 qj = qs[p];
                     if (qj.w == 0.0f) continue;
+                    {
+                        const float ex = qj.x - qi.x, ey = qj.y - qi.y, ez = qj.z - qi.z;
+                        if (ex * ex + ey * ey + ez * ez > cut32) continue;
+                    }
                     if (H.tl) {  // two-level: the target's own cutoff 3.5 sp / r
                         acc = __fadd_rn(acc, tl_pair(qi, qj, Ke, Ka, factor));
                         continue;
@@ -1371,6 +1379,10 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
     Ke.init(eps);
     Ka.init(ai);
     const double spd = (double)sp, inv_sp = 1.0 / spd;
+    // f32 screen: the f32 sum of squares is within ~4e-7 of r^2, so pairs
+    // above cut^2 (1 + 1e-6) are outside the cutoff in the f64 test as well
+    // (and add nothing); the rest take the f64 test unchanged
+    const float cut32 = nextafterf((float)((H.tl ? ai * ai : r2cut) * (1.0 + 1e-6)), INFINITY);
     const uint32_t lc = cell_of[t] - (uint32_t)H.c2off;
     const int cx = (int)(lc % (uint32_t)H.nc[0]);
     const int cy = (int)((lc / (uint32_t)H.nc[0]) % (uint32_t)H.nc[1]);
@@ -1386,6 +1398,10 @@ __global__ void k_c2_update(const float4* __restrict__ qw, const uint32_t* __res
                 const uint32_t c = (uint32_t)H.c2off + (uint32_t)x + (uint32_t)H.nc[0] * ((uint32_t)y + (uint32_t)H.nc[1] * (uint32_t)z);
                 for (uint32_t p = dstart[c]; p < dstart[c + 1]; ++p) {
                     const float4 qj = qw[dlist[p]];
+                    {
+                        const float ex = qj.x - qi.x, ey = qj.y - qi.y, ez = qj.z - qi.z;
+                        if (ex * ex + ey * ey + ez * ez > cut32) continue;
+                    }
                     if (H.tl) {
                         acc = __fsub_rn(acc, tl_pair(qi, qj, Ke, Ka, factor));
                         continue;
