@@ -321,7 +321,7 @@ __device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, co
 // register footprint of this kernel, and so its occupancy, low.
 template <bool PERIODIC>
 __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
-                                           const int32_t jijk[3], const double q[3], bool own, bool mid_self,
+                                           const int32_t jijk[3], const double q[3], bool own,
                                            const RegionDev& g, const LinkParams& P,
                                            const uint32_t* __restrict__ cell_ptr,
                                            const int32_t* __restrict__ aux,
@@ -334,14 +334,10 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
     }
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
     if (s >= e) return;
-    const int32_t ac = aux[jc];
-    if (ac >= 0) {
+    if (aux[jc] >= 0) {
         L.dmask |= 1u << o;
         return;
     }
-    // a mid cell's own pairs are k_cell_intra's, its pairs with non-dense
-    // neighbours k_cell_pair's
-    if (mid_self) return;
     uint32_t qq = own ? p + 1 : s;
     // two partners at a time: their positions, and then the parents of the
     // linked ones, are loaded together; a partner whose parent is already the
@@ -397,8 +393,9 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    const bool mid_self = aux[c] == kAuxMid;
-    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, mid_self, g, P, cell_ptr, aux, xc, L);
+    // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
+    if (aux[c] == kAuxMid) return 0u;
+    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, g, P, cell_ptr, aux, xc, L);
     auto neighbour = [&](int o) {
         int32_t jijk[3];
         double sh[3];
@@ -419,7 +416,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
             }
         }
         const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, mid_self, g, P, cell_ptr, aux, xc, L);
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, g, P, cell_ptr, aux, xc, L);
     };
     if (!g.prune) {
         for (int o = 0; o < 27; ++o)
@@ -669,16 +666,20 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
     else cell_intra<false>(s, n, P, xc, parent, pos, lp);
 }
 
-// One wave per mid cell A (kAuxMid): the pairs between A and each forward
-// neighbour cell B that is not dense (the cell-pair rule of k_link: the lower
-// cell takes the pair). Only members within R = b + m of the shared face,
-// edge or corner can link, on either side, so each side is filtered to those
-// first (the same margins as k_link's pruning) and the remaining pairs are
-// tested by all lanes. k_link keeps, for A's particles, only the dense
-// neighbours (its mask for k_link_dense).
+// One wave per mid cell A (kAuxMid): every pair of A's members with a
+// neighbour cell, under k_link's rules: a forward cell B that is not dense
+// (the lower cell takes the pair), and a dense clique cell in either
+// direction (taken from the non-dense side). Only members within R = b + m
+// of the shared face, edge or corner can link, so A's side is filtered to
+// those first (the same margins as k_link's pruning). Against a non-dense B,
+// B's side is filtered the same way and the remaining pairs are tested by all
+// lanes; against a dense B, each remaining member scans B's sub-cells as in
+// k_link_dense. k_link does nothing for A's particles.
 template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
+                          const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
+                          const int32_t* __restrict__ ncomp, const int32_t* __restrict__ croot,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
                           uint16_t* ia, uint16_t* ib, uint32_t* cnt) {
     const int lane = __lane_id();
@@ -695,9 +696,11 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         double sh[3];
         if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
         const uint32_t jl = local_cell(g, jijk);
-        if (jl <= lc) continue;
         const uint32_t cB = g.cell_off + jl;
-        if (aux[cB] >= 0) continue;  // dense: through k_link's mask
+        const bool denseB = aux[cB] >= 0;
+        // non-dense pairs once (forward cells); pairs with a dense clique cell
+        // in both directions (k_link's rule, from the non-dense side)
+        if (jl <= lc && !(denseB && g.clique)) continue;
         const uint32_t sB = cell_ptr[cB], nB = cell_ptr[cB + 1] - sB;
         if (nB == 0) continue;
         const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
@@ -718,6 +721,17 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         __syncthreads();
         const uint32_t na = cnt[0];
         if (na == 0) continue;
+        if (denseB) {
+            // the members that reach B scan its sub-cells (a cell of one
+            // component already sharing the member's root is skipped at once)
+            for (uint32_t i = 0; i < na; ++i) {
+                const uint32_t pm = sA + ia[i];
+                dense_entry<PERIODIC>(pm, o, pa[ia[i]], g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
+                                      xc, parent);
+            }
+            __syncthreads();
+            continue;
+        }
         // B in chunks of at most kMidMax members (a non-dense B can be larger)
         for (uint32_t b0 = 0; b0 < nB; b0 += kMidMax) {
             const uint32_t b1 = min(nB, b0 + kMidMax);
@@ -756,6 +770,10 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ cell_off,
                                                      const RegionDev* __restrict__ reg, int32_t nreg,
                                                      const int32_t* __restrict__ aux,
+                                                     const uint32_t* __restrict__ sub_ptr,
+                                                     const uint32_t* __restrict__ sub_list,
+                                                     const int32_t* __restrict__ ncomp,
+                                                     const int32_t* __restrict__ croot,
                                                      const float4* __restrict__ xc, int32_t* parent,
                                                      LinkParams P) {
     __shared__ float4 pa[kMidMax], pb[kMidMax];
@@ -763,8 +781,10 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
     __shared__ uint32_t cnt[2];
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
-    if (g.is_periodic) cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, cnt);
-    else cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, cnt);
+    if (g.is_periodic)
+        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, cnt);
+    else
+        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, cnt);
 }
 
 template <bool PERIODIC>
@@ -1554,7 +1574,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
-            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
+            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, sub_ptr, sub_list,
+                                               ncomp, croot, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
