@@ -3,8 +3,9 @@
 // Both work on tiles of 4096 elements per 256-thread workgroup (4 waves of 64).
 // The radix sort is the classic reduce-then-scan form: a per-tile digit
 // histogram, one device scan of the digit-major [digit][tile] table, then a
-// scatter in which each tile ranks its items stably (lane order within a wave
-// via 8 ballots, wave order within a round, rounds in item order).
+// scatter in which each tile ranks its items stably: each wave ranks its own
+// contiguous quarter with ballots and a per-wave running count, and one
+// barrier combines the waves in order.
 #include "prims.hpp"
 
 namespace pyfof {
@@ -133,6 +134,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
     }
 }
 
+// Each wave ranks a contiguous quarter of the tile, 64 items per round: lanes
+// with equal digits are matched by one ballot per digit bit, and a running
+// count per digit in the wave's own LDS row gives each item its rank among
+// the wave's earlier items (the rows need no barrier: a wave's lanes run in
+// lockstep). One barrier then turns the four rows into prefixes in wave
+// order, so the rank of an item follows (wave, round, lane) = item order:
+// the sort is stable.
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ vals,
                                                           uint32_t* __restrict__ okeys,
@@ -140,20 +148,21 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
                                                           int shift,
                                                           const uint32_t* __restrict__ hist,
                                                           uint32_t ntiles) {
-    __shared__ uint32_t run[kRadix];
-    __shared__ uint32_t wcnt[kBlock / kWave][kRadix];
-    uint32_t* wf = &wcnt[0][0];
+    constexpr int kWaves = kBlock / kWave;
+    constexpr int kRounds = kTile / kBlock;  // rounds of 64 items per wave
+    __shared__ uint32_t wc[kWaves][kRadix];
     const int t = threadIdx.x, w = t / kWave;
-    for (int d = t; d < kRadix; d += kBlock) run[d] = hist[(uint64_t)d * ntiles + blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t* wf = &wc[0][0];
+    for (int i = t; i < kWaves * kRadix; i += kBlock) wf[i] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kRounds * kWave);
     const uint64_t lt = lanemask_lt();
-    for (int j = 0; j < kIpt; ++j) {
-        for (int i = t; i < (kBlock / kWave) * kRadix; i += kBlock) wf[i] = 0;
-        __syncthreads();
-        const uint64_t idx = base + (uint64_t)j * kBlock + t;
+    uint32_t kk[kRounds], vv[kRounds], rk[kRounds];
+#pragma unroll
+    for (int j = 0; j < kRounds; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kWave + __lane_id();
         const bool valid = idx < n;
         const uint32_t k = valid ? keys[idx] : 0u;
-        const uint32_t v = valid ? vals[idx] : 0u;
         const uint32_t d = (k >> shift) & (kRadix - 1);
         uint64_t m = __ballot(valid);
 #pragma unroll
@@ -162,26 +171,31 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
             const uint64_t bal = __ballot(valid && bit);
             m &= bit ? bal : ~bal;
         }
-        const uint32_t rank = __popcll(m & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(m);
-        __syncthreads();
-        for (int dd = t; dd < kRadix; dd += kBlock) {
-            uint32_t r = run[dd];
+        const uint32_t r = __popcll(m & lt);
+        const uint32_t before = wc[w][d];
+        if (valid && r == 0) wc[w][d] = before + (uint32_t)__popcll(m);
+        kk[j] = k;
+        vv[j] = valid ? vals[idx] : 0u;
+        rk[j] = before + r;
+    }
+    __syncthreads();
+    for (int dd = t; dd < kRadix; dd += kBlock) {
+        uint32_t acc = hist[(uint64_t)dd * ntiles + blockIdx.x];
 #pragma unroll
-            for (int k2 = 0; k2 < kBlock / kWave; ++k2) {
-                const uint32_t c = wcnt[k2][dd];
-                wcnt[k2][dd] = r;
-                r += c;
-            }
-            run[dd] = r;
+        for (int w2 = 0; w2 < kWaves; ++w2) {
+            const uint32_t c = wc[w2][dd];
+            wc[w2][dd] = acc;
+            acc += c;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t dst = wcnt[w][d] + rank;
-            okeys[dst] = k;
-            ovals[dst] = v;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRounds; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kWave + __lane_id();
+        if (idx >= n) continue;
+        const uint32_t dst = wc[w][(kk[j] >> shift) & (kRadix - 1)] + rk[j];
+        okeys[dst] = kk[j];
+        ovals[dst] = vv[j];
     }
 }
 
