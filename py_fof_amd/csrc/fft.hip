@@ -219,8 +219,11 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
     const int32_t g = blockIdx.x;
     const int32_t i0 = g % ax.c0;
     const int32_t i1 = (g / ax.c0) % ax.c1;
-    const int32_t ib0 = MODE == kConvZ ? 0 : g / (ax.c0 * ax.c1);
-    const int32_t ib1 = MODE == kConvZ ? nslots : ib0 + 1;
+    // z pass: blockIdx.y picks a group of slots (several groups when the
+    // columns alone would not fill the GPU)
+    const int32_t zper = (nslots + (int32_t)gridDim.y - 1) / (int32_t)gridDim.y;
+    const int32_t ib0 = MODE == kConvZ ? (int32_t)blockIdx.y * zper : g / (ax.c0 * ax.c1);
+    const int32_t ib1 = MODE == kConvZ ? min(nslots, ib0 + zper) : ib0 + 1;
     const int64_t goff = (int64_t)i0 * ax.s0 + (int64_t)i1 * ax.s1;
     // first line of the workgroup: y in the x passes, x in the y and z passes
     const int line0 = i0 * NL;
@@ -354,11 +357,13 @@ void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, 
     const uint32_t gx = (uint32_t)(ygroups * mc) * (uint32_t)nslots;
     const uint32_t gy = (uint32_t)(xgroups_y * mc) * (uint32_t)nslots;
     const uint32_t gzz = (uint32_t)(xgroups_z * N);
+    // about 2048 workgroups in the z pass: slot groups when the columns are few
+    const uint32_t zgroups = (uint32_t)std::max(1, std::min(nslots, (int32_t)((2048 + gzz - 1) / gzz)));
     k_fft_conv<kConvXFwd, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvYFwd, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvZ, NLZ, N><<<gzz, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dz);
+    k_fft_conv<kConvZ, NLZ, N><<<dim3(gzz, zgroups), kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dz);
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvYInv, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyi);
     PYFOF_LAUNCH_CHECK();
