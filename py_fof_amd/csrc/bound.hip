@@ -1539,6 +1539,36 @@ __device__ void chunked_sums(uint32_t s0, uint32_t e0, const uint8_t* __restrict
     __syncthreads();
 }
 
+// Radix-select step over a 256-bin digit histogram h (LDS), by the first 256
+// threads of the workgroup (all threads must call it): the digit d holding
+// the rem-th largest key counted from the top digit, and the rank left within
+// it. This is the first d, from 255 down, whose count from the top reaches
+// rem, found with a reversed prefix sum instead of a serial loop; as in that
+// loop, d = 0 and rem minus the total if rem exceeds the total.
+__device__ void select_digit256(const uint32_t* h, uint32_t rem, uint32_t* sel, uint32_t* wsum) {
+    const int t = threadIdx.x;
+    uint32_t c = 0, incl = 0;
+    if (t < 256) {
+        c = h[255 - t];
+        incl = c;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, kWave);
+            if ((int)__lane_id() >= o) incl += y;
+        }
+        if (__lane_id() == kWave - 1) wsum[t / kWave] = incl;
+    }
+    __syncthreads();
+    if (t < 256) {
+        uint32_t off = 0;
+        for (int w = 0; w < t / kWave; ++w) off += wsum[w];
+        incl += off;
+        const uint32_t excl = incl - c;
+        if (excl < rem && rem <= incl) { sel[0] = (uint32_t)(255 - t); sel[1] = rem - excl; }
+        if (t == 255 && rem > incl) { sel[0] = 0u; sel[1] = rem - incl; }
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ alist,
                                                   const uint32_t* __restrict__ hoff,
                                                   const float4* __restrict__ qw,
@@ -1640,17 +1670,7 @@ __global__ void __launch_bounds__(kPB) k_pm_inner(const int32_t* __restrict__ al
                 if ((key & mask) == prefix) atomicAdd(&hist[(key >> (8 * byte)) & 255u], 1u);
             }
             __syncthreads();
-            if (tid == 0) {
-                uint32_t rem = need;
-                int dsel = 0;
-                for (int d = 255; d >= 0; --d) {
-                    if (hist[d] >= rem) { dsel = d; break; }
-                    rem -= hist[d];
-                }
-                sh[1] = (uint32_t)dsel;
-                sh[2] = rem;
-            }
-            __syncthreads();
+            select_digit256(hist, need, &sh[1], wcnt);
             prefix |= sh[1] << (8 * byte);
             mask |= 255u << (8 * byte);
             need = sh[2];
@@ -1905,18 +1925,9 @@ __global__ void __launch_bounds__(kB) k_pm_big(const int2* __restrict__ bmap, co
             for (int i = tid; i < 256; i += kB)
                 if (lhist[i]) atomicAdd(&S->hist[byte][i], lhist[i]);
             if (!(ok = big_barrier(S, nb, &ok_s))) break;
-            if (tid == 0) {
-                uint32_t rem = need;
-                int dsel = 0;
-                for (int d = 255; d >= 0; --d) {
-                    const uint32_t c = ld_dev(&S->hist[byte][d]);
-                    if (c >= rem) { dsel = d; break; }
-                    rem -= c;
-                }
-                sel[0] = (uint32_t)dsel;
-                sel[1] = rem;
-            }
+            for (int i = tid; i < 256; i += kB) lhist[i] = ld_dev(&S->hist[byte][i]);
             __syncthreads();
+            select_digit256(lhist, need, sel, wcnt);
             prefix |= sel[0] << (8 * byte);
             mask |= 255u << (8 * byte);
             need = sel[1];
