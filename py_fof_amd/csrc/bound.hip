@@ -12,9 +12,11 @@
 //                        residual (C2), then k_pm_inner (removal loop)
 //   write-back           valid = status OK and np_bound >= np_halo_min
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "bound.hpp"
@@ -2159,17 +2161,56 @@ GreenCache& green_for(Ctx& C, int n, hipStream_t s, double range = 0.0) {
 
 }  // namespace
 
-int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& res) {
+// bound_fortran.f90:279-289: the first error of a call is kept with its
+// message, and printed at verbose >= 1
+static int32_t bound_fail(const BoundArgs& A, BoundResult& res, int32_t code, const char* msg) {
+    res.err_msg = msg;
+    if (A.verbose >= 1) {
+        printf(" [BOUND_ERROR] code=%12d  msg= %s\n", code, msg);
+        fflush(stdout);
+    }
+    return code;
+}
+
+// bound_fortran.f90:2062-2080, printed at verbose >= 1 when a candidate is
+// rejected for its span; box and h0 are the fixed 300 Mpc/h and 0.67 the
+// reference's unbinding module uses for this report
+static void span_report(const BoundArgs& A, const SelCtx& sc, uint32_t ident, double span) {
+    const double box = 300.0, h0 = 0.67;
+    const double l_ng = box / (h0 * A.ng_global);
+    printf(" ===============================================================\n");
+    printf(" bound grid selection: candidate rejected, no valid result written\n");
+    printf("   subroutine           = bound_classify\n");
+    printf("   reason               = candidate span exceeds D_max = 15 Mpc\n");
+    printf("   candidate identifier = %12u\n", ident);
+    printf("   actual span          = %25.17g  standard grid cells\n", span);
+    printf("   actual span          = %25.17g  Mpc (comoving)\n", span * l_ng);
+    printf("   upper limit D_max    = 15 Mpc = %25.17g  standard grid cells\n", A.s15);
+    printf("   box [Mpc/h], h0      = %25.17g %25.17g\n", box, h0);
+    printf("   ng, ng_global        = %12lld %25.17g\n", (long long)A.ng, A.ng_global);
+    printf("   n_max, m_max         = %12d %12d\n", sc.n_max, sc.m_max);
+    printf("   spacing_max          = %25.17g\n", sc.spacing_max);
+    printf("   nlev ladder size     = %12d  ngrid = %12d\n", sc.ntot, sc.ngrid);
+    printf("   L_ng [Mpc]           = %25.17g\n", l_ng);
+    printf(" ===============================================================\n");
+    fflush(stdout);
+}
+
+static int32_t run_bound_impl(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& res) {
     hipStream_t s = C.stream;
     res = BoundResult();
     const uint32_t nh = A.nh, M = A.nmem;
     if (nh == 0 || M == 0) return IERR_OK;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto fail = [&](int32_t code, const char* msg) { return bound_fail(A, res, code, msg); };
     SelCtx sc;
     sel_init(sc, A.s15, A.ng);
     // bound_fortran.f90:1559-1563, 1586-1590
-    if (sc.ngrid != 4) return IERR_INVALID;
-    if (!(A.a > 0.0) || !std::isfinite(A.a) || !std::isfinite(A.b)) return IERR_INVALID;
-    if (A.direct_max < kDirectMax || A.direct_max > kDirectLargeMax) return IERR_INVALID;
+    if (sc.ngrid != 4) return fail(IERR_INVALID, "bound_check_batch: fixed groups require ng >= 256");
+    if (!(A.a > 0.0) || !std::isfinite(A.a) || !std::isfinite(A.b))
+        return fail(IERR_INVALID, "bound_check_batch: invalid physical context");
+    if (A.direct_max < kDirectMax || A.direct_max > kDirectLargeMax)
+        return fail(IERR_INVALID, "direct-summation threshold out of range");
     Params P{A.app, A.a, A.b, A.L, A.Lh, A.np_min};
 
     float4* qw = C.buf("qw").as<float4>(M);
@@ -2199,8 +2240,16 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         float span = lh[3] - lh[0];
         span = std::max(span, lh[4] - lh[1]);
         span = std::max(span, lh[5] - lh[2]);
-        if (!std::isfinite(span)) return IERR_INVALID;
-        if (n > 1000 && span > (float)A.s15) return IERR_SPAN;
+        if (!std::isfinite(span)) {
+            printf(" bound_classify: non-finite span\n");  // bound_fortran.f90:2966-2970
+            fflush(stdout);
+            return fail(IERR_INVALID, "bound_classify: non-finite span");
+        }
+        if (n > 1000 && span > (float)A.s15) {
+            const int32_t e = fail(IERR_SPAN, "candidate span exceeds D_max = 15 Mpc");
+            if (A.verbose >= 1) span_report(A, sc, h ? hoff[h - 1] + 1 : 1u, (double)span);
+            return e;
+        }
         if (n <= A.direct_max) {
             if (n <= kDirectMax) {
                 dsmall.push_back((int32_t)h);
@@ -2212,10 +2261,11 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         }
         int gid;
         double sp;
-        if (sel_grid(sc, n, (double)span, gid, sp) != 0) return IERR_INVALID;
-        if (gid == 0) return IERR_INVALID;  // n > 1000 always selects a grid
+        if (sel_grid(sc, n, (double)span, gid, sp) != 0 || gid == 0)  // n > 1000 always selects a grid
+            return fail(IERR_INVALID, "no enabled grid class or direct combination can cover candidate");
         const float sp32 = (float)sp;
-        if ((double)(sc.bn[gid - 1] / 2 - 3) * (double)sp32 < (double)span) return IERR_INVALID;
+        if ((double)(sc.bn[gid - 1] / 2 - 3) * (double)sp32 < (double)span)
+            return fail(IERR_INVALID, "actual single precision spacing does not cover span");
         PmHalo H;
         memset(&H, 0, sizeof(H));
         H.origin = make_float4(lh[0] - sp32, lh[1] - sp32, lh[2] - sp32, sp32);
@@ -2227,6 +2277,12 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         pm_halo.push_back((int32_t)h);
     }
     res.n_direct = (uint32_t)(dsmall.size() + dlarge.size());
+    // bound_fortran.f90:1679-1681. The batch is one phase on the GPU: every
+    // candidate is processed concurrently, by workgroups of kB threads.
+    if (A.verbose >= 1) {
+        printf("[SCHEDULE] phase=1 owners=%u team_threads=%d\n", nh, kB);
+        fflush(stdout);
+    }
 
     int32_t* status = C.buf("bstatus").as<int32_t>(nh);
     int32_t* npb = C.buf("bnpb").as<int32_t>(nh);
@@ -2358,7 +2414,7 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
             }
             cls_count[g] = cls_end_prev - cls_first[g];
         }
-        if (c2cells >= 0x7fffffffull) return IERR_TOO_LARGE;
+        if (c2cells >= 0x7fffffffull) return fail(IERR_TOO_LARGE, "residual cell table exceeds 32-bit indices");
         PYFOF_HIP(hipMemcpyAsync(d_pm, pm.data(), sizeof(PmHalo) * npm, hipMemcpyHostToDevice, s));
         // member -> C2 cell; members sorted by (cell, member index)
         uint32_t* ck = C.buf("c2k").as<uint32_t>(M);
@@ -2852,7 +2908,38 @@ int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& re
         else ++res.n_max_iter;
     }
     res.n_direct_large = (uint32_t)dlarge.size();
+    // bound_fortran.f90:2861: printed for every candidate that ends at the
+    // outer iteration cap, whatever the verbosity (candidate numbers from 1)
+    if (res.n_max_iter) {
+        std::vector<int32_t> npb_h(nh);
+        PYFOF_HIP(hipMemcpyAsync(npb_h.data(), npb, 4 * (size_t)nh, hipMemcpyDeviceToHost, s));
+        PYFOF_HIP(hipStreamSynchronize(s));
+        for (uint32_t h = 0; h < nh; ++h)
+            if (st_h[h] == 5)
+                printf(" bound_check: MAX_ITER, candidate =%12u retained members =%12d\n", h + 1, npb_h[h]);
+    }
+    // bound_fortran.f90:1797-1799, 1808-1811
+    if (A.verbose >= 1) {
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        printf("[BOUND_PHASE] phase=1 total_sec=%12.4f\n", sec);
+        printf("[BOUND_STATUS] OK/TOO_FEW/MAX_ITER=%u %u %u  total_sec=%12.4f\n", res.n_ok, res.n_too_few,
+               res.n_max_iter, sec);
+    }
+    if (res.n_max_iter || A.verbose >= 1) fflush(stdout);
     return IERR_OK;
+}
+
+int32_t run_bound(Ctx& C, const BoundArgs& A, uint8_t* keep_out, BoundResult& res) {
+    const int32_t e = run_bound_impl(C, A, keep_out, res);
+    if (e != IERR_OK && res.err_msg.empty()) {
+        // found on the device, where the error word holds the code only
+        const char* msg = e == IERR_NONFINITE ? "non-finite potential or energy"
+                          : e == IERR_INVALID ? "a member or source node outside its grid, or no removal threshold"
+                          : e == IERR_HIP     ? "workgroups of a large candidate did not meet at a barrier"
+                                              : "device error";
+        bound_fail(A, res, e, msg);
+    }
+    return e;
 }
 
 }  // namespace pyfof

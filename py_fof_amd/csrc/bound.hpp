@@ -8,6 +8,7 @@ struct BoundArgs {
     // physical context (bound_fortran.f90:291-330)
     double app, a, b, s15;
     int64_t ng;
+    double ng_global = 512.0;  // standard-grid size, for the span error report
     int32_t np_min, verbose;
     double L, Lh;
     // particle data in HBM (vp/mp may be null: zero velocities, unit masses)

@@ -61,6 +61,7 @@ struct BoundStats {
     uint32_t n_direct = 0, n_pm[4] = {0, 0, 0, 0};
     int32_t passes = 0;
     uint32_t n_direct_large = 0;
+    std::string err_msg;  // reason of a failed call, empty on success
 };
 
 // Green function of one isolated grid size (depends on n only), cached.
@@ -160,6 +161,7 @@ struct CallArgs {
     double* ang_mom;
     int32_t* root_out;
     int32_t mem_kind;
+    double ng_global = 512.0;  // only read by the unbinding's error report
 };
 
 // Runs one call on C (validation, then the pipeline); returns ierr. The

@@ -1666,7 +1666,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             Cat cand = build_catalog(nh);
             BoundArgs B;
             B.app = A.softening; B.a = A.a_val; B.b = A.b_val; B.s15 = A.s_15; B.ng = A.ng;
-            B.np_min = A.np_min; B.verbose = A.verbose; B.L = A.L; B.Lh = P.Lh;
+            B.np_min = A.np_min; B.verbose = A.verbose; B.L = A.L; B.Lh = P.Lh; B.ng_global = A.ng_global;
             B.xp = xp; B.vp = vp; B.mp = mp; B.N = N;
             B.nh = nh; B.nmem = cand.nmem; B.hoff = cand.hoff; B.members = cand.members;
             B.mhalo = cand.mkeys; B.anchor = anchor; B.in_halo = o_in; B.halo_id = o_id;
@@ -1686,7 +1686,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             const int32_t berr = run_bound(C, B, bkeep, br);
             C.last_bound = br;
             if (berr != IERR_OK) {
-                set_error("unbinding failed");
+                set_error("unbinding failed: " + br.err_msg);
                 return berr;
             }
             // anchors and regions of the valid halos, in their new numbering
@@ -1846,7 +1846,7 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
     CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
                verbose, softening, a_val, b_val, s_15_val, ng_val, total_np, in_halo, halo_id,
                ang_mom, root_out, mem_kind};
-    (void)ng_global_val;
+    A.ng_global = ng_global_val;
     return fof_entry(ctx->c, A, n_threads);
 }
 
@@ -1866,7 +1866,7 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
     CallArgs A{xp, vp, mp, np_halo, nhalo, L_Box, linking_length, np_halo_min, bound_check,
                verbose, softening, a_val, b_val, s_15_val, ng_val, total_np, in_halo, halo_id,
                ang_mom, nullptr, mem_kind};
-    (void)ng_global_val;
+    A.ng_global = ng_global_val;
     return fof_entry(C, A, n_threads);
 }
 

@@ -153,6 +153,7 @@ CallArgs args_of(const pyfof_job_desc* d) {
     a.b_val = d->b_val;
     a.s_15 = d->s_15_val;
     a.ng = d->ng_val;
+    a.ng_global = d->ng_global_val;
     a.total_np = d->total_np;
     a.in_halo = d->in_halo;
     a.halo_id = d->halo_id;

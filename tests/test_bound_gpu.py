@@ -339,3 +339,60 @@ def test_small_and_nonpositive_np_halo_min(np_min):
         vs.append(np.asfortranarray(rng.standard_normal(xp.shape) * 0.1))
     inh, hid, am, _ = _gpu(xs, vs, np_halo_min=np_min)
     _same_as_oracle(inh, hid, am, _oracle(xs, vs, np_min=np_min))
+
+
+# ---------------------------------------------------------------- diagnostics
+
+def test_verbose_schedule_phase_and_status_lines(capfd):
+    # bound_fortran.f90:1679-1681, 1797-1799, 1808-1811: printed at verbose >= 1 only
+    import re
+    pf = _lib()
+    xp, vp, _, _ = _with_highv(1, 50, 10)
+    _, _, _, st = _gpu(xp, vp, verbose=0)
+    assert "[SCHEDULE]" not in capfd.readouterr().out
+    _, _, _, st = _gpu(xp, vp, verbose=1)
+    out = capfd.readouterr().out
+    m = re.search(r"^\[SCHEDULE\] phase=1 owners=(\d+) team_threads=(\d+)$", out, re.M)
+    assert m and int(m.group(1)) == st["nhalo"] and int(m.group(2)) > 0
+    assert re.search(r"^\[BOUND_PHASE\] phase=1 total_sec=\s*\d+\.\d{4}$", out, re.M)
+    m = re.search(r"^\[BOUND_STATUS\] OK/TOO_FEW/MAX_ITER=(\d+) (\d+) (\d+)  total_sec=\s*\d+\.\d{4}$",
+                  out, re.M)
+    assert m
+    ok, few, mx = (int(g) for g in m.groups())
+    assert (ok, few, mx) == (st["n_ok"], st["n_too_few"], st["n_max_iter"])
+    assert ok + few + mx == st["nhalo"] and ok >= 1
+    assert "MAX_ITER, candidate" not in out or mx > 0
+    # phases print in order: schedule, phase end, batch status
+    assert out.index("[SCHEDULE]") < out.index("[BOUND_PHASE]") < out.index("[BOUND_STATUS]")
+    del pf
+
+
+def test_verbose_error_reports(capfd):
+    # bound_fortran.f90:279-289 ([BOUND_ERROR] at verbose >= 1) and the span
+    # rejection report, bound_fortran.f90:2062-2080; the candidate identifier is
+    # its 1-based offset in the packed member list (fof_fortran.f90:291)
+    pf = _lib()
+    x = np.vstack([np.linspace(10.0, 35.0, 1050), np.zeros(1050), np.zeros(1050)])
+    with pytest.raises(ValueError, match="Candidate halo span exceeds 15 Mpc limit"):
+        pf.fof(x, vp_list=np.zeros((3, 1050)), L_Box=L, linking_length=B, np_halo_min=NP_MIN,
+               bound_check=True, verbose=1)
+    out = capfd.readouterr().out
+    assert "[BOUND_ERROR] code=" in out and "candidate span exceeds D_max = 15 Mpc" in out
+    line = [s for s in out.splitlines() if "[BOUND_ERROR]" in s][0]
+    assert int(line.split("code=")[1].split()[0]) == 2
+    assert "bound grid selection: candidate rejected, no valid result written" in out
+    ident = [s for s in out.splitlines() if "candidate identifier" in s][0]
+    assert int(ident.split("=")[1]) == 1
+    span = [s for s in out.splitlines() if "standard grid cells" in s and "actual span" in s][0]
+    assert abs(float(span.split("=")[1].split()[0]) - 25.0) < 1e-4
+    # quiet at verbose 0
+    with pytest.raises(ValueError):
+        pf.fof(x, vp_list=np.zeros((3, 1050)), L_Box=L, linking_length=B, np_halo_min=NP_MIN,
+               bound_check=True, verbose=0)
+    assert "[BOUND_ERROR]" not in capfd.readouterr().out
+    xp, vp, _, _ = _with_highv(1, 50, 10)
+    with pytest.raises(ValueError, match="Invalid input parameters"):
+        pf.fof(xp, vp_list=vp, L_Box=L, linking_length=B, np_halo_min=NP_MIN, bound_check=True,
+               ng=128, verbose=1)
+    out = capfd.readouterr().out
+    assert "fixed groups require ng >= 256" in out
