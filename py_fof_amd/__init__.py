@@ -10,6 +10,8 @@ import numpy as np
 
 from . import _native
 from ._native import MEM_DEVICE, MEM_HOST, Context, NativeLibraryError  # noqa: F401
+from . import _device
+from ._units import unit_scalars
 
 __all__ = ["fof", "fof_with_info", "Context", "NativeLibraryError"]
 __version__ = "0.1.0"
@@ -40,10 +42,11 @@ def _error_for(ierr):
 
 
 class _Prepared:
-    """Native arguments of one call plus what the result needs afterwards."""
+    """Native arguments of one call plus what the result needs afterwards.
+    `device` is None for host arrays, else the torch device of the inputs."""
     __slots__ = ("empty", "xp", "vp", "mp", "np_halo", "nhalo", "total_np", "L", "link",
                  "np_min", "bound_check", "verbose", "softening", "a", "b", "s15", "n_threads",
-                 "ng", "ng_global", "need_am", "ang_mom_scale")
+                 "ng", "ng_global", "need_am", "ang_mom_scale", "device")
 
 
 def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
@@ -63,6 +66,16 @@ def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
             raise ValueError("n_threads cannot exceed 256")
         n_threads_val = int(n_threads)
 
+    if _device.is_device_input(xp_list):
+        P = _Prepared()
+        U = _device.prepare(P, xp_list, vp_list, mp_list, units, bound_check, softening, L_Box,
+                            linking_length, z, box, h0, omega_m, omega_l, n_sim, ng_global,
+                            include_hubble_flow)
+        if U is not None:
+            _scalars_into(P, U, np_halo_min, bound_check, verbose, n_threads_val, ng, ng_global)
+            P.need_am = bool(return_ang_mom) and P.vp is not None
+        return P
+
     if isinstance(xp_list, np.ndarray) and xp_list.ndim == 2:
         xp_list_norm = [xp_list]
     elif isinstance(xp_list, (list, tuple)):
@@ -79,6 +92,7 @@ def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
     total_np = int(np.sum(np_halo))
 
     P = _Prepared()
+    P.device = None
     P.nhalo = nhalo
     P.total_np = total_np
     P.empty = total_np == 0
@@ -137,77 +151,41 @@ def _prepare(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
         if not np.all(np.isfinite(mp_raw)) or not np.all(mp_raw > 0):
             raise ValueError("All particle masses in mp_list must be positive and finite")
 
-    z_val = float(z)
-    if 1.0 + z_val <= 0.0:
-        raise ValueError(f"1 + z must be positive, got {1.0 + z_val}")
-    a_val = 1.0 / (1.0 + z_val)
-
-    if include_hubble_flow:
-        t0 = 1.0 / (150.0 * float(h0) * np.sqrt(float(omega_m)))
-        hubble = 100.0 * float(h0) * np.sqrt(float(omega_m) / (a_val ** 3) + float(omega_l))
-        b_val = hubble * (a_val ** 2) * t0
-    else:
-        b_val = 0.0
-
-    if units == 'cube':
-        softening_grid = 0.1 if softening is None else float(softening)
-        if bound_check and softening_grid <= 0:
-            raise ValueError("softening must be positive")
-        ng_eff_global = float(ng_global) if ng_global is not None else float(L_Box)
-        s_15_val = 15.0 * float(h0) * ng_eff_global / float(box)
-        xp_code = xp_halo
-        L_Box_code = float(L_Box)
-        link_code = float(linking_length)
-        vp_code = vp_halo
+    U = unit_scalars(units, bound_check, softening, L_Box, linking_length, z, box, h0, omega_m,
+                     omega_l, n_sim, ng_global, include_hubble_flow, mp_raw is not None, total_np)
+    if not U.physical:
+        xp_code, vp_code = xp_halo, vp_halo
         mp_code = mp_raw  # None: unit masses inside the library
-        ang_mom_scale = 1.0
     else:
-        if bound_check:
-            if softening is None or float(softening) <= 0:
-                raise ValueError("softening must be explicitly provided and > 0 when units='physical' and bound_check=True")
-            if mp_raw is None and n_sim is None:
-                raise ValueError("Either mp_list or n_sim must be provided when units='physical' and bound_check=True")
-
-        ng_eff = float(round(float(n_sim) ** (1.0 / 3.0))) if (n_sim is not None and float(n_sim) > 0) else (float(ng_global) if ng_global is not None else 512.0)
-        box_mpc_h = float(L_Box)
-        dx = box_mpc_h / ng_eff
-
-        xp_code = np.asfortranarray(xp_halo / dx, dtype=np.float64)
-        L_Box_code = ng_eff
-        link_code = float(linking_length) / dx
-        softening_grid = 0.1 if softening is None else (float(softening) / dx)
-        s_15_val = 15.0 * float(h0) * ng_eff / box_mpc_h
-
-        v_scale = (a_val * ng_eff) / (150.0 * np.sqrt(float(omega_m)) * box_mpc_h)
-        vp_code = np.asfortranarray(vp_halo * v_scale, dtype=np.float64) if vp_halo is not None else None
-
-        rho_crit_0 = 2.77536627e11
-        m_grid = float(omega_m) * rho_crit_0 * (box_mpc_h ** 3) / (ng_eff ** 3)
-
+        xp_code = np.asfortranarray(xp_halo / U.dx, dtype=np.float64)
+        vp_code = np.asfortranarray(vp_halo * U.v_scale, dtype=np.float64) if vp_halo is not None else None
         if mp_raw is not None:
-            mp_code = np.ascontiguousarray(mp_raw / m_grid, dtype=np.float64)
-        elif n_sim is not None:
-            mp_code = np.full(total_np, (ng_eff ** 3) / float(n_sim), dtype=np.float64)
+            mp_code = np.ascontiguousarray(mp_raw / U.m_grid, dtype=np.float64)
+        elif U.mp_fill is not None:
+            mp_code = np.full(total_np, U.mp_fill, dtype=np.float64)
         else:
             mp_code = None
-            m_grid = 1.0
-
-        ang_mom_scale = m_grid * dx / v_scale
 
     P.xp, P.vp, P.mp, P.np_halo = xp_code, vp_code, mp_code, np_halo
-    P.L, P.link, P.np_min = L_Box_code, link_code, int(np_halo_min)
-    P.bound_check, P.verbose = int(bool(bound_check)), int(verbose)
-    P.softening, P.a, P.b, P.s15 = softening_grid, a_val, b_val, s_15_val
-    P.n_threads = n_threads_val
-    P.ng = int(ng) if ng is not None else 256
-    P.ng_global = float(ng_global) if ng_global is not None else 512.0
+    _scalars_into(P, U, np_halo_min, bound_check, verbose, n_threads_val, ng, ng_global)
     # the moment is only observable with return_ang_mom; with zero velocities it is 0
     P.need_am = bool(return_ang_mom) and vp_code is not None
-    P.ang_mom_scale = ang_mom_scale
     return P
 
 
+def _scalars_into(P, U, np_halo_min, bound_check, verbose, n_threads_val, ng, ng_global):
+    P.L, P.link, P.np_min = U.L, U.link, int(np_halo_min)
+    P.bound_check, P.verbose = int(bool(bound_check)), int(verbose)
+    P.softening, P.a, P.b, P.s15 = U.softening, U.a, U.b, U.s15
+    P.n_threads = n_threads_val
+    P.ng = int(ng) if ng is not None else 256
+    P.ng_global = float(ng_global) if ng_global is not None else 512.0
+    P.ang_mom_scale = U.ang_mom_scale
+
+
 def _outputs(P):
+    if P.device is not None:
+        return _device.outputs(P)
     return (np.zeros(P.total_np, dtype=np.int32), np.zeros(P.total_np, dtype=np.int32),
             np.zeros((3, P.nhalo), dtype=np.float64, order='F'))
 
@@ -218,6 +196,8 @@ def _finish(P, ierr, in_halo_out, halo_id_out, ang_mom_out, message=""):
         if P.verbose:
             print(f"[py_fof_amd] ierr={ierr}: {message}")
         raise _error_for(ierr)
+    if P.device is not None:
+        return in_halo_out.bool(), halo_id_out, ang_mom_out * P.ang_mom_scale
     return in_halo_out.astype(bool), halo_id_out, ang_mom_out * P.ang_mom_scale
 
 
@@ -239,6 +219,9 @@ def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
                  n_sim=n_sim, units=units, include_hubble_flow=include_hubble_flow, ng=ng,
                  ng_global=ng_global)
     extras = {}
+    if P.empty and P.device is not None:
+        out = _device.empty_result(P, _want_catalog, _want_roots)
+        return out if (_want_catalog or _want_roots) else out[:3]
     if P.empty:
         in_halo = np.zeros(0, dtype=bool)
         halo_id = np.zeros(0, dtype=np.int32)
@@ -250,17 +233,25 @@ def _fof_core(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
         return (in_halo, halo_id, ang_mom, extras) if (_want_catalog or _want_roots) else (in_halo, halo_id, ang_mom)
 
     in_halo_out, halo_id_out, ang_mom_out = _outputs(P)
-    roots = np.zeros(P.total_np, dtype=np.int32) if _want_roots else None
-    ctx = _ctx if _ctx is not None else _native.default_context()
+    if P.device is None:
+        roots = np.zeros(P.total_np, dtype=np.int32) if _want_roots else None
+        ctx = _ctx if _ctx is not None else _native.default_context()
+        mem = MEM_HOST
+    else:
+        roots = _device.like_int32(P) if _want_roots else None
+        ctx = _ctx if _ctx is not None else _native.default_context(P.device.index or 0)
+        if (P.device.index or 0) != ctx.device:
+            raise ValueError(f"inputs are on {P.device}, the context on device {ctx.device}")
+        mem = MEM_DEVICE
     ctx.set_option(_native.OPT_CATALOG, 1 if _want_catalog else 0)
     ierr = ctx.fof(P.xp, P.vp, P.mp, P.np_halo, P.L, P.link, P.np_min, P.bound_check, P.verbose,
                    P.softening, P.a, P.b, P.s15, P.n_threads, P.total_np, P.ng, P.ng_global,
-                   in_halo_out, halo_id_out, ang_mom_out if P.need_am else None, roots, MEM_HOST)
+                   in_halo_out, halo_id_out, ang_mom_out if P.need_am else None, roots, mem)
     out = _finish(P, ierr, in_halo_out, halo_id_out, ang_mom_out,
                   _native.last_error() if ierr else "")
     if _want_catalog or _want_roots:
         if _want_catalog:
-            extras["catalog"] = ctx.catalog()
+            extras["catalog"] = ctx.catalog() if P.device is None else _device.catalog(ctx, P.device)
         if _want_roots:
             extras["roots"] = roots
         return out + (extras,)
@@ -277,6 +268,8 @@ def fof(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
 
     Returns ``in_halo`` (bool, shape (total_np,)) and, with
     ``return_ang_mom=True``, also the (3, nhalo) per-region angular momentum.
+    xp_list (and vp_list, mp_list) may also be torch tensors on the GPU; the
+    results are then device tensors and nothing goes through host memory.
     """
     in_halo, _, ang_mom = _fof_core(
         xp_list=xp_list, vp_list=vp_list, L_Box=L_Box, linking_length=linking_length,
@@ -299,7 +292,9 @@ def fof_with_info(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
     """FoF with the halo catalog (py_fof/__init__.py:450-518).
 
     The catalog (members grouped by halo id, ascending indices) is built on
-    the device by a stable radix sort instead of a host argsort.
+    the device by a stable radix sort instead of a host argsort. With device
+    tensors as input, in_halo, ang_mom and each halo's indices are device
+    tensors; counts and bound_status are host arrays, as in the reference.
     """
     in_halo, halo_id, ang_mom, extras = _fof_core(
         xp_list=xp_list, vp_list=vp_list, L_Box=L_Box, linking_length=linking_length,
@@ -315,6 +310,8 @@ def fof_with_info(xp_list, vp_list=None, L_Box=512.0, linking_length=0.2,
         nhalo_found = 0
         np_per_halo = np.array([], dtype=int)
         halo_indices = []
+    elif not isinstance(off, np.ndarray):
+        nhalo_found, np_per_halo, halo_indices = _device.split_catalog(off, idx)
     else:
         counts = np.diff(off).astype(np.int64)
         idx64 = idx.astype(np.int64)

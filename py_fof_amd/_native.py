@@ -5,6 +5,7 @@ call raises. Build it with ``python -m py_fof_amd.build``.
 """
 import ctypes
 import os
+import sys
 import threading
 
 import numpy as np
@@ -39,11 +40,27 @@ class NativeLibraryError(RuntimeError):
     pass
 
 
+def _torch_first():
+    """The torch wheel carries its own HIP runtime beside the system one this
+    library links. Both share the device, but the runtime that initialises
+    second finds no GPU when it is torch's. When torch is already imported,
+    its runtime is initialised here, before this library's first HIP call."""
+    t = sys.modules.get("torch")
+    if t is None:
+        return
+    try:
+        if t.cuda.device_count() > 0:
+            t.cuda.init()
+    except Exception:
+        pass
+
+
 def load(path=LIB_PATH):
     """Loads and types the library once; raises NativeLibraryError if absent."""
     global _lib
     with _lock:
         if _lib is None:
+            _torch_first()
             _lib = _load_from(path)
         return _lib
 
@@ -137,19 +154,28 @@ class Context:
         if err:
             raise RuntimeError("pyfof_ctx_set_option failed: " + last_error())
 
-    def catalog(self):
-        """(offsets int64[nh+1], indices int32[total]) of the last call."""
+    def catalog_size(self):
+        """(halos, members) of the last call's catalog."""
         nh = ctypes.c_int32(0)
         nm = ctypes.c_int64(0)
         err = self.lib.pyfof_catalog_size(self.h, ctypes.byref(nh), ctypes.byref(nm))
         if err:
             raise RuntimeError("pyfof_catalog_size failed: " + last_error())
-        off = np.zeros(nh.value + 1, np.int64)
-        idx = np.zeros(max(nm.value, 1), np.int32)
-        err = self.lib.pyfof_catalog_copy(self.h, off.ctypes.data, idx.ctypes.data, MEM_HOST)
+        return nh.value, nm.value
+
+    def catalog_copy(self, offsets, indices, mem_kind):
+        """Copies the catalog into caller buffers (numpy or device tensors)."""
+        err = self.lib.pyfof_catalog_copy(self.h, _ptr(offsets), _ptr(indices), int(mem_kind))
         if err:
             raise RuntimeError("pyfof_catalog_copy failed: " + last_error())
-        return off, idx[:nm.value]
+
+    def catalog(self):
+        """(offsets int64[nh+1], indices int32[total]) of the last call."""
+        nh, nm = self.catalog_size()
+        off = np.zeros(nh + 1, np.int64)
+        idx = np.zeros(max(nm, 1), np.int32)
+        self.catalog_copy(off, idx, MEM_HOST)
+        return off, idx[:nm]
 
     def timings(self):
         buf = (ctypes.c_double * 8)()

@@ -17,8 +17,7 @@ import threading
 
 import numpy as np
 
-from . import _native
-from . import _finish, _outputs, _prepare
+from . import _device, _finish, _native, _outputs, _prepare
 
 __all__ = ["Server", "Job", "ServerError"]
 
@@ -135,6 +134,7 @@ class Server:
         self._h = self._lib.pyfof_server_create(int(device), int(hbm_budget), int(slots))
         if not self._h:
             raise ServerError(self._lib.pyfof_server_last_error().decode(errors="replace"))
+        self.device = int(device)
         self._lock = threading.Lock()
 
     def close(self):
@@ -167,27 +167,32 @@ class Server:
         ret_am = bool(kwargs.get("return_ang_mom", False))
         P = _prepare(xp_list, vp_list=vp_list, **kwargs)
         if P.empty:
-            outs = (np.zeros(0, dtype=bool), np.zeros(0, dtype=np.int32),
-                    np.zeros((3, P.nhalo), dtype=np.float64))
+            if P.device is not None:
+                outs = _device.empty_result(P, False, False)[:3]
+            else:
+                outs = (np.zeros(0, dtype=bool), np.zeros(0, dtype=np.int32),
+                        np.zeros((3, P.nhalo), dtype=np.float64))
             job = Job(self, -1, None, outs, ret_am, None)
             job._done = True
             return job
+        if P.device is not None and (P.device.index or 0) != self.device:
+            raise ValueError(f"inputs are on {P.device}, the server on device {self.device}")
         outs = _outputs(P)
         in_halo, halo_id, ang_mom = outs
         d = _JobDesc()
-        d.xp = P.xp.ctypes.data
-        d.vp = P.vp.ctypes.data if P.vp is not None else None
-        d.mp = P.mp.ctypes.data if P.mp is not None else None
+        d.xp = _native._ptr(P.xp)
+        d.vp = _native._ptr(P.vp)
+        d.mp = _native._ptr(P.mp)
         d.np_halo = P.np_halo.ctypes.data
-        d.in_halo = in_halo.ctypes.data
-        d.halo_id = halo_id.ctypes.data
-        d.ang_mom = ang_mom.ctypes.data if P.need_am else None
+        d.in_halo = _native._ptr(in_halo)
+        d.halo_id = _native._ptr(halo_id)
+        d.ang_mom = _native._ptr(ang_mom) if P.need_am else None
         d.L_Box, d.linking_length, d.softening = P.L, P.link, P.softening
         d.a_val, d.b_val, d.s_15_val, d.ng_global_val = P.a, P.b, P.s15, P.ng_global
         d.ng_val = P.ng
         d.nhalo, d.np_halo_min, d.bound_check = P.nhalo, P.np_min, P.bound_check
         d.verbose, d.n_threads, d.total_np = P.verbose, P.n_threads, P.total_np
-        d.mem_kind = _native.MEM_HOST
+        d.mem_kind = _native.MEM_HOST if P.device is None else _native.MEM_DEVICE
         keep = (P.xp, P.vp, P.mp, P.np_halo, outs)
         with self._lock:
             jid = self._lib.pyfof_submit(self._h, ctypes.byref(d))
@@ -203,5 +208,6 @@ class Server:
         d = _JobDesc()
         d.vp = 1 if P.vp is not None else None
         d.mp = 1 if P.mp is not None else None
-        d.total_np, d.bound_check, d.mem_kind = P.total_np, P.bound_check, _native.MEM_HOST
+        d.total_np, d.bound_check = P.total_np, P.bound_check
+        d.mem_kind = _native.MEM_HOST if P.device is None else _native.MEM_DEVICE
         return int(self._lib.pyfof_job_footprint(ctypes.byref(d)))
