@@ -19,7 +19,7 @@ struct LinkParams {
     int32_t intra_stage;  // dense-cell members staged in LDS by k_dense_intra
 };
 
-constexpr int kDenseMin = 128;    // cells with at least this many particles get sub-cells
+constexpr int kDenseMin = 256;    // cells with at least this many particles get sub-cells
 constexpr int kMaxSubPerDim = 16; // sub-cells per dim; at most 4096 per cell
 
 // gfortran real MODULO: exact fmod, shifted into the sign of p.

@@ -136,7 +136,7 @@ def test_dense_clump_subcells_and_angmom():
 def test_dense_clique_subcells_nonperiodic_and_coarse_k():
     rng = np.random.default_rng(31)
     # isolated dense blob: L_mesh clamps to b, two clique sub-cells per axis
-    x = 200.0 + 0.35 * rng.standard_normal((3, 20000))
+    x = 200.0 + 0.28 * rng.standard_normal((3, 20000))
     g, r = _run_both([np.asfortranarray(x)], 512.0, 0.2, 10)
     _check(g, r)
     assert _dense_cells([np.asfortranarray(x)], 512.0, 0.2) > 0
