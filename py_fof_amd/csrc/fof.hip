@@ -681,46 +681,78 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
                           const int32_t* __restrict__ ncomp, const int32_t* __restrict__ croot,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
-                          uint16_t* ia, uint16_t* ib, uint32_t* cnt) {
+                          uint16_t* ia, uint32_t* ib, uint8_t* fa) {
     const int lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t lc = c - g.cell_off;
     int32_t ijk[3];
     double lo[3];
     cell_lower_corner(g, lc, lo, ijk);
     const uint32_t sA = cell_ptr[c], nA = cell_ptr[c + 1] - sA;
-    for (uint32_t i = lane; i < nA; i += kWave) pa[i] = xc[sA + i];
-    __syncthreads();
-    for (int o = 0; o < 27; ++o) {
-        if (o == 13) continue;
+    // lane o < 27 looks up neighbour o: its cell, class and members, all in
+    // one round of loads. Non-dense pairs are taken once (forward cells),
+    // pairs with a dense clique cell in both directions (k_link's rule, from
+    // the non-dense side).
+    uint32_t n_s = 0, n_n = 0;
+    int32_t n_j0 = 0, n_j1 = 0, n_j2 = 0;
+    bool n_dense = false, n_work = false;
+    if (lane < 27 && lane != 13) {
         int32_t jijk[3];
         double sh[3];
-        if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) continue;
-        const uint32_t jl = local_cell(g, jijk);
-        const uint32_t cB = g.cell_off + jl;
-        const bool denseB = aux[cB] >= 0;
-        // non-dense pairs once (forward cells); pairs with a dense clique cell
-        // in both directions (k_link's rule, from the non-dense side)
-        if (jl <= lc && !(denseB && g.clique)) continue;
-        const uint32_t sB = cell_ptr[cB], nB = cell_ptr[cB + 1] - sB;
-        if (nB == 0) continue;
-        const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
-        double loB[3];
-        for (int d = 0; d < 3; ++d) loB[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
-        if (lane == 0) cnt[0] = 0u;
-        __syncthreads();
-        for (uint32_t i = lane; i < nA; i += kWave) {
-            const float4 a = pa[i];
-            const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
-            bool ok = true;
-            for (int d = 0; d < 3; ++d) {
-                if (off[d] < 0) ok = ok && x[d] - P.R <= lo[d] + 2.0 * P.m;
-                else if (off[d] > 0) ok = ok && x[d] + P.R >= lo[d] + g.cell[d] - 2.0 * P.m;
+        if (neighbour_cell(g, ijk, lane, jijk, sh, P.L)) {
+            const uint32_t jl = local_cell(g, jijk);
+            const uint32_t cB = g.cell_off + jl;
+            const bool dB = aux[cB] >= 0;
+            if (!(jl <= lc && !(dB && g.clique))) {
+                n_s = cell_ptr[cB];
+                n_n = cell_ptr[cB + 1] - n_s;
+                n_work = n_n > 0;
+                n_dense = dB;
+                n_j0 = jijk[0];
+                n_j1 = jijk[1];
+                n_j2 = jijk[2];
             }
-            if (ok) ia[atomicAdd(&cnt[0], 1u)] = (uint16_t)i;
         }
-        __syncthreads();
-        const uint32_t na = cnt[0];
+    }
+    // A's members and their face bits: bit 2d when the member lies within
+    // R = b + m of the lower face of axis d (with the 2m pruning margin),
+    // bit 2d+1 for the upper face
+    for (uint32_t i = lane; i < nA; i += kWave) {
+        const float4 a = xc[sA + i];
+        pa[i] = a;
+        const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
+        uint32_t f = 0;
+        for (int d = 0; d < 3; ++d) {
+            if (x[d] - P.R <= lo[d] + 2.0 * P.m) f |= 1u << (2 * d);
+            if (x[d] + P.R >= lo[d] + g.cell[d] - 2.0 * P.m) f |= 2u << (2 * d);
+        }
+        fa[i] = (uint8_t)f;
+    }
+    __syncthreads();
+    uint64_t work = __ballot(n_work);
+    while (work) {
+        const int o = __ffsll((long long)work) - 1;
+        work &= work - 1ull;
+        const uint32_t sB = __shfl(n_s, o, kWave), nB = __shfl(n_n, o, kWave);
+        const bool denseB = __shfl((int32_t)n_dense, o, kWave) != 0;
+        const int32_t jijk[3] = {__shfl(n_j0, o, kWave), __shfl(n_j1, o, kWave), __shfl(n_j2, o, kWave)};
+        const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+        uint32_t need = 0;
+        for (int d = 0; d < 3; ++d) {
+            if (off[d] < 0) need |= 1u << (2 * d);
+            else if (off[d] > 0) need |= 2u << (2 * d);
+        }
+        // A's members that can reach B, in member order
+        uint32_t na = 0;
+        for (uint32_t i0 = 0; i0 < nA; i0 += kWave) {
+            const uint32_t i = i0 + (uint32_t)lane;
+            const bool ok = i < nA && (fa[i] & need) == need;
+            const uint64_t m = __ballot(ok);
+            if (ok) ia[na + (uint32_t)__popcll(m & below)] = (uint16_t)i;
+            na += (uint32_t)__popcll(m);
+        }
         if (na == 0) continue;
+        __syncthreads();
         if (denseB) {
             // the members that reach B scan its sub-cells (a cell of one
             // component already sharing the member's root is skipped at once)
@@ -732,36 +764,48 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             __syncthreads();
             continue;
         }
-        // B in chunks of at most kMidMax members (a non-dense B can be larger)
-        for (uint32_t b0 = 0; b0 < nB; b0 += kMidMax) {
-            const uint32_t b1 = min(nB, b0 + kMidMax);
-            if (lane == 0) cnt[1] = 0u;
+        double loB[3];
+        for (int d = 0; d < 3; ++d) loB[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
+        // B's members that can reach A are staged kWave at a time (a
+        // non-dense B can be large); each full window is tested against A
+        auto flush = [&](uint32_t nb) {
             __syncthreads();
-            for (uint32_t i = b0 + lane; i < b1; i += kWave) {
-                const float4 q = xc[sB + i];
-                const double x[3] = {(double)q.x, (double)q.y, (double)q.z};
-                bool ok = true;
-                for (int d = 0; d < 3; ++d) {
-                    if (off[d] > 0) ok = ok && x[d] - P.R <= loB[d] + 2.0 * P.m;
-                    else if (off[d] < 0) ok = ok && x[d] + P.R >= loB[d] + g.cell[d] - 2.0 * P.m;
-                }
-                if (ok) {
-                    const uint32_t k = atomicAdd(&cnt[1], 1u);
-                    ib[k] = (uint16_t)(i - b0);
-                    pb[k] = q;
-                }
-            }
-            __syncthreads();
-            const uint32_t nb = cnt[1];
             for (uint32_t t = lane; t < na * nb; t += kWave) {
                 const uint32_t i = t / nb, j = t - i * nb;
                 if (pair_linked<PERIODIC>(pa[ia[i]], pb[j], P)) {
                     LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
-                    L.link((int32_t)(sB + b0 + ib[j]));
+                    L.link((int32_t)(sB + ib[j]));
                 }
             }
             __syncthreads();
+        };
+        uint32_t nb = 0;
+        for (uint32_t i0 = 0; i0 < nB; i0 += kWave) {
+            const uint32_t i = i0 + (uint32_t)lane;
+            bool ok = i < nB;
+            float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (ok) {
+                q = xc[sB + i];
+                const double x[3] = {(double)q.x, (double)q.y, (double)q.z};
+                for (int d = 0; d < 3; ++d) {
+                    if (off[d] > 0) ok = ok && x[d] - P.R <= loB[d] + 2.0 * P.m;
+                    else if (off[d] < 0) ok = ok && x[d] + P.R >= loB[d] + g.cell[d] - 2.0 * P.m;
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            if (nb + cnt > (uint32_t)kWave) {
+                flush(nb);
+                nb = 0;
+            }
+            if (ok) {
+                const uint32_t k = nb + (uint32_t)__popcll(m & below);
+                ib[k] = i;
+                pb[k] = q;
+            }
+            nb += cnt;
         }
+        if (nb) flush(nb);
     }
 }
 
@@ -776,15 +820,16 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      const int32_t* __restrict__ croot,
                                                      const float4* __restrict__ xc, int32_t* parent,
                                                      LinkParams P) {
-    __shared__ float4 pa[kMidMax], pb[kMidMax];
-    __shared__ uint16_t ia[kMidMax], ib[kMidMax];
-    __shared__ uint32_t cnt[2];
+    __shared__ float4 pa[kMidMax], pb[kWave];
+    __shared__ uint16_t ia[kMidMax];
+    __shared__ uint32_t ib[kWave];
+    __shared__ uint8_t fa[kMidMax];
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, cnt);
+        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, fa);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, cnt);
+        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, fa);
 }
 
 template <bool PERIODIC>
