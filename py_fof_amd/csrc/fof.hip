@@ -642,10 +642,18 @@ __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const fl
         lp[i] = (int32_t)i;
     }
     __syncthreads();
-    for (uint32_t i = 0; i + 1 < n; ++i) {
-        const float4 a = pos[i];
-        for (uint32_t j = i + 1 + lane; j < n; j += kWave)
-            if (pair_linked<PERIODIC>(a, pos[j], P)) lds_unite(lp, (int32_t)i, (int32_t)j);
+    // every unordered pair once, all lanes busy: pair t = (m - 1) n + k is
+    // (k, (k + m) mod n) for m = 1 .. (n - 1) / 2 and every k, then, for even
+    // n, (k, k + n / 2) for k < n / 2
+    const uint32_t total = n * ((n - 1) / 2) + ((n & 1u) ? 0u : n / 2);
+    uint32_t k = (uint32_t)lane, m = 1;
+    while (k >= n) { k -= n; ++m; }
+    for (uint32_t t = lane; t < total; t += kWave) {
+        uint32_t j = k + m;
+        if (j >= n) j -= n;
+        if (pair_linked<PERIODIC>(pos[k], pos[j], P)) lds_unite(lp, (int32_t)k, (int32_t)j);
+        k += kWave;
+        while (k >= n) { k -= n; ++m; }
     }
     __syncthreads();
     for (uint32_t i = lane; i < n; i += kWave) parent[s + i] = (int32_t)s + lds_find(lp, (int32_t)i);
