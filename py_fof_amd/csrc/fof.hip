@@ -1007,11 +1007,20 @@ __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
         int32_t rA = dsu_find(parent, croot[d]);
         int32_t rB = dsu_find(parent, croot[dB]);
         if (rA == rB) continue;
-        // per axis: number of (sub-cell of A, sub-cell of B) coordinate pairs
+        // Per axis, a sub-cell pair is a position and an option: along an axis
+        // parallel to the shared face, A's sub-cell coordinate (kk positions)
+        // and B's lateral offset from it (0, -1, +1, -2, +2); across it, one
+        // position and the depth pairs (k-1,0) (k-1,1) (k-2,0) toward +1,
+        // mirrored toward -1. Positions vary fastest, so the directly opposite
+        // sub-cells (all options 0) are searched first: where the two cells
+        // touch, one of them links at once.
         const int32_t co[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
-        int32_t npair[3];
-        for (int ax = 0; ax < 3; ++ax) npair[ax] = co[ax] == 0 ? 5 * kk[ax] : 3;
-        const uint32_t total = (uint32_t)npair[0] * (uint32_t)npair[1] * (uint32_t)npair[2];
+        int32_t npos[3], nopt[3];
+        for (int ax = 0; ax < 3; ++ax) {
+            npos[ax] = co[ax] == 0 ? kk[ax] : 1;
+            nopt[ax] = co[ax] == 0 ? 5 : 3;
+        }
+        const uint32_t total = (uint32_t)(npos[0] * npos[1] * npos[2]) * (uint32_t)(nopt[0] * nopt[1] * nopt[2]);
         const uint32_t* rowB = sub_ptr + (size_t)dB * P.sub_stride;
         const uint32_t csB = cell_ptr[cB];
         for (uint32_t base = 0; base < total; base += kWave) {
@@ -1019,21 +1028,27 @@ __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
             bool hit = false;
             if (it < total) {
                 uint32_t rem = it;
-                int32_t sa[3], sb[3];
+                int32_t pos[3], opt[3], sa[3], sb[3];
                 bool valid = true;
                 for (int ax = 0; ax < 3; ++ax) {
-                    const uint32_t pi = rem % (uint32_t)npair[ax];
-                    rem /= (uint32_t)npair[ax];
+                    pos[ax] = (int32_t)(rem % (uint32_t)npos[ax]);
+                    rem /= (uint32_t)npos[ax];
+                }
+                for (int ax = 0; ax < 3; ++ax) {
+                    opt[ax] = (int32_t)(rem % (uint32_t)nopt[ax]);
+                    rem /= (uint32_t)nopt[ax];
+                }
+                for (int ax = 0; ax < 3; ++ax) {
                     if (co[ax] == 0) {
-                        sa[ax] = (int32_t)(pi / 5u);
-                        sb[ax] = sa[ax] + (int32_t)(pi % 5u) - 2;
+                        const int32_t lat = (opt[ax] + 1) / 2 * ((opt[ax] & 1) ? -1 : 1);  // 0 -1 1 -2 2
+                        sa[ax] = pos[ax];
+                        sb[ax] = pos[ax] + lat;
                         if (sb[ax] < 0 || sb[ax] >= kk[ax]) valid = false;
                     } else {
-                        // (k-1,0) (k-1,1) (k-2,0) toward +1; mirrored toward -1
-                        const int32_t pa[3] = {kk[ax] - 1, kk[ax] - 1, kk[ax] - 2};
-                        const int32_t pb[3] = {0, 1, 0};
-                        sa[ax] = co[ax] > 0 ? pa[pi] : kk[ax] - 1 - pa[pi];
-                        sb[ax] = co[ax] > 0 ? pb[pi] : kk[ax] - 1 - pb[pi];
+                        const int32_t da = opt[ax] == 2 ? kk[ax] - 2 : kk[ax] - 1;
+                        const int32_t db = opt[ax] == 1 ? 1 : 0;
+                        sa[ax] = co[ax] > 0 ? da : kk[ax] - 1 - da;
+                        sb[ax] = co[ax] > 0 ? db : kk[ax] - 1 - db;
                     }
                 }
                 if (valid) {
