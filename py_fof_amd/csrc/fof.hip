@@ -678,16 +678,17 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
 // neighbour cell, under k_link's rules: a forward cell B that is not dense
 // (the lower cell takes the pair), and a dense clique cell in either
 // direction (taken from the non-dense side). Only members within R = b + m
-// of the shared face, edge or corner can link, so A's side is filtered to
-// those first (the same margins as k_link's pruning). Against a non-dense B,
-// B's side is filtered the same way and the remaining pairs are tested by all
-// lanes; against a dense B, each remaining member scans B's sub-cells as in
-// k_link_dense. k_link does nothing for A's particles.
+// of the shared face, edge or corner can link, so both sides are filtered to
+// those first (the same margins as k_link's pruning) and the remaining pairs
+// are tested by all lanes. Against a dense B, where most such pairs link,
+// each staged member carries its union-find parent (in .w, which the
+// predicates do not read; k_cell_intra and k_dense_intra leave every member
+// pointing at its local root): a pair whose two parents this wave has just
+// joined is connected already and is not tested, and a passing pair joins
+// the two parents. k_link does nothing for A's particles.
 template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
-                          const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
-                          const int32_t* __restrict__ ncomp, const int32_t* __restrict__ croot,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
                           uint16_t* ia, uint32_t* ib, uint8_t* fa) {
     const int lane = __lane_id();
@@ -727,7 +728,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     // bit 2d+1 for the upper face
     for (uint32_t i = lane; i < nA; i += kWave) {
         const float4 a = xc[sA + i];
-        pa[i] = a;
+        pa[i] = make_float4(a.x, a.y, a.z, __int_as_float(atomic_load_relaxed(&parent[sA + i])));
         const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
         uint32_t f = 0;
         for (int d = 0; d < 3; ++d) {
@@ -761,28 +762,43 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         }
         if (na == 0) continue;
         __syncthreads();
-        if (denseB) {
-            // the members that reach B scan its sub-cells (a cell of one
-            // component already sharing the member's root is skipped at once)
-            for (uint32_t i = 0; i < na; ++i) {
-                const uint32_t pm = sA + ia[i];
-                dense_entry<PERIODIC>(pm, o, pa[ia[i]], g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
-                                      xc, parent);
-            }
-            __syncthreads();
-            continue;
-        }
         double loB[3];
         for (int d = 0; d < 3; ++d) loB[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
         // B's members that can reach A are staged kWave at a time (a
         // non-dense B can be large); each full window is tested against A
+        int32_t ja = -1, jb = -1;  // the parent pair this wave joined last
         auto flush = [&](uint32_t nb) {
             __syncthreads();
-            for (uint32_t t = lane; t < na * nb; t += kWave) {
-                const uint32_t i = t / nb, j = t - i * nb;
-                if (pair_linked<PERIODIC>(pa[ia[i]], pb[j], P)) {
-                    LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
-                    L.link((int32_t)(sB + ib[j]));
+            if (!denseB) {
+                for (uint32_t t = lane; t < na * nb; t += kWave) {
+                    const uint32_t i = t / nb, j = t - i * nb;
+                    if (pair_linked<PERIODIC>(pa[ia[i]], pb[j], P)) {
+                        LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
+                        L.link((int32_t)(sB + ib[j]));
+                    }
+                }
+            } else {
+                for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
+                    const uint32_t t = t0 + (uint32_t)lane;
+                    int32_t xa = 0, xb = 0;
+                    bool hit = false;
+                    if (t < na * nb) {
+                        const uint32_t i = t / nb, j = t - i * nb;
+                        const float4 a = pa[ia[i]], q = pb[j];
+                        xa = __float_as_int(a.w);
+                        xb = __float_as_int(q.w);
+                        if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked<PERIODIC>(a, q, P);
+                    }
+                    uint64_t m = __ballot(hit);
+                    while (m) {
+                        const int l = __ffsll((long long)m) - 1;
+                        const int32_t la = __shfl(xa, l, kWave), lb = __shfl(xb, l, kWave);
+                        if (lane == l) dsu_unite(parent, la, lb);
+                        ja = la;
+                        jb = lb;
+                        hit = hit && !(xa == la && xb == lb);
+                        m = __ballot(hit);
+                    }
                 }
             }
             __syncthreads();
@@ -809,6 +825,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             if (ok) {
                 const uint32_t k = nb + (uint32_t)__popcll(m & below);
                 ib[k] = i;
+                if (denseB) q.w = __int_as_float(atomic_load_relaxed(&parent[sB + i]));
                 pb[k] = q;
             }
             nb += cnt;
@@ -822,10 +839,6 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ cell_off,
                                                      const RegionDev* __restrict__ reg, int32_t nreg,
                                                      const int32_t* __restrict__ aux,
-                                                     const uint32_t* __restrict__ sub_ptr,
-                                                     const uint32_t* __restrict__ sub_list,
-                                                     const int32_t* __restrict__ ncomp,
-                                                     const int32_t* __restrict__ croot,
                                                      const float4* __restrict__ xc, int32_t* parent,
                                                      LinkParams P) {
     __shared__ float4 pa[kMidMax], pb[kWave];
@@ -835,9 +848,9 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, fa);
+        cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent, pa, pb, ia, ib, fa);
+        cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa);
 }
 
 template <bool PERIODIC>
@@ -1642,8 +1655,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
-            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, sub_ptr, sub_list,
-                                               ncomp, croot, xc, parent, P);
+            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
