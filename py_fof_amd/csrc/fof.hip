@@ -318,7 +318,8 @@ __device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, co
 // [x - R, x + R] with R = b + m; the cell and sub-cell coordinates used to
 // prune are monotone in x. A cell with sub-cells (dense) is not scanned here:
 // bit o of the thread's mask is set and k_link_dense scans it, which keeps the
-// register footprint of this kernel, and so its occupancy, low.
+// register footprint of this kernel, and so its occupancy, low. A neighbouring
+// mid cell is skipped: k_cell_pair takes its pairs with this cell.
 template <bool PERIODIC>
 __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
                                            const int32_t jijk[3], const double q[3], bool own,
@@ -332,9 +333,11 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
             if (q[d] + P.R < lo - P.m || q[d] - P.R > lo + g.cell[d] + P.m) return;
         }
     }
+    const int32_t cls = aux[jc];
+    if (!own && cls == kAuxMid) return;
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
     if (s >= e) return;
-    if (aux[jc] >= 0) {
+    if (cls >= 0) {
         L.dmask |= 1u << o;
         return;
     }
@@ -675,9 +678,9 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
 }
 
 // One wave per mid cell A (kAuxMid): every pair of A's members with a
-// neighbour cell, under k_link's rules: a forward cell B that is not dense
-// (the lower cell takes the pair), and a dense clique cell in either
-// direction (taken from the non-dense side). Only members within R = b + m
+// neighbour cell B: a forward mid cell (the lower cell takes the pair), and
+// any other cell in either direction (k_link skips mid neighbours, and the
+// dense-cell kernels take no pairs with mid cells). Only members within R = b + m
 // of the shared face, edge or corner can link, so both sides are filtered to
 // those first (the same margins as k_link's pruning) and the remaining pairs
 // are tested by all lanes. Against a dense B, where most such pairs link,
@@ -699,9 +702,8 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     cell_lower_corner(g, lc, lo, ijk);
     const uint32_t sA = cell_ptr[c], nA = cell_ptr[c + 1] - sA;
     // lane o < 27 looks up neighbour o: its cell, class and members, all in
-    // one round of loads. Non-dense pairs are taken once (forward cells),
-    // pairs with a dense clique cell in both directions (k_link's rule, from
-    // the non-dense side).
+    // one round of loads. Pairs with another mid cell are taken once (forward
+    // cells), pairs with any other cell in both directions.
     uint32_t n_s = 0, n_n = 0;
     int32_t n_j0 = 0, n_j1 = 0, n_j2 = 0;
     bool n_dense = false, n_work = false;
@@ -711,8 +713,9 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         if (neighbour_cell(g, ijk, lane, jijk, sh, P.L)) {
             const uint32_t jl = local_cell(g, jijk);
             const uint32_t cB = g.cell_off + jl;
-            const bool dB = aux[cB] >= 0;
-            if (!(jl <= lc && !(dB && g.clique))) {
+            const int32_t clsB = aux[cB];
+            const bool dB = clsB >= 0;
+            if (!(jl <= lc && clsB == kAuxMid)) {
                 n_s = cell_ptr[cB];
                 n_n = cell_ptr[cB + 1] - n_s;
                 n_work = n_n > 0;
