@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(kBlk) k_gather(const double* __restrict__ xp,
 // own pairs k_cell_intra resolves; -1 otherwise. Negative values read as
 // "not dense" everywhere.
 constexpr int32_t kAuxMid = -2;
-constexpr uint32_t kMidMin = 16, kMidMax = 255;  // particles of a k_cell_intra cell
+constexpr uint32_t kMidMin = 8, kMidMax = 255;  // particles of a k_cell_intra cell
 constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
 
 __global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
