@@ -156,30 +156,48 @@ constexpr int32_t kAuxMid = -2;
 constexpr uint32_t kMidMin = 8, kMidMax = 255;  // particles of a k_cell_intra cell
 constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
 
+constexpr int kMarkPer = 8;                       // consecutive cells per thread
+
 __global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
                              const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg,
                              int32_t nreg, int32_t* __restrict__ aux, uint32_t* __restrict__ dense_cells,
                              uint32_t* __restrict__ ndense, uint32_t* __restrict__ mid_cells,
                              uint32_t* __restrict__ nmid) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    bool dense = false, mid = false;
-    if (c < ncell) {
-        const uint32_t cnt = cell_ptr[c + 1] - cell_ptr[c];
-        if (cnt >= kMidMin) {
-            const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
-            dense = !g.skip && cnt >= (uint32_t)kDenseMin && g.prune && g.k[0] * g.k[1] * g.k[2] > 1;
-            mid = !dense && !g.skip && g.prune && cnt <= kMidMax;
+    const uint64_t c0 = ((uint64_t)blockIdx.x * kMarkBlk + threadIdx.x) * kMarkPer;
+    uint32_t dbits = 0, mbits = 0;  // bit k: cell c0 + k is dense / mid
+    if (c0 < ncell) {
+        uint32_t prev = cell_ptr[c0];
+        for (int k = 0; k < kMarkPer && c0 + k < ncell; ++k) {
+            const uint32_t c = (uint32_t)(c0 + k);
+            const uint32_t next = cell_ptr[c + 1];
+            const uint32_t cnt = next - prev;
+            prev = next;
+            if (cnt >= kMidMin) {
+                const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+                const bool dense = !g.skip && cnt >= (uint32_t)kDenseMin && g.prune && g.k[0] * g.k[1] * g.k[2] > 1;
+                const bool mid = !dense && !g.skip && g.prune && cnt <= kMidMax;
+                dbits |= (uint32_t)dense << k;
+                mbits |= (uint32_t)mid << k;
+            }
         }
     }
-    // one counter update per workgroup and list: wave counts by ballot,
-    // their prefix in LDS, lanes placed by ballot rank
+    // one counter update per workgroup and list: the threads' counts are
+    // scanned within each wave, the waves' totals in LDS
+    const uint32_t nd = (uint32_t)__popc(dbits), nm = (uint32_t)__popc(mbits);
+    uint32_t sd = nd, sm = nm;
+    const int lane = __lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t td = __shfl_up(sd, o, kWave), tm = __shfl_up(sm, o, kWave);
+        if (lane >= o) {
+            sd += td;
+            sm += tm;
+        }
+    }
     __shared__ uint32_t wd[kMarkBlk / kWave], wm[kMarkBlk / kWave], bases[2];
     const int w = threadIdx.x / kWave;
-    const uint64_t lt = (1ull << __lane_id()) - 1ull;
-    const uint64_t bd = __ballot(dense), bm = __ballot(mid);
-    if (__lane_id() == 0) {
-        wd[w] = (uint32_t)__popcll(bd);
-        wm[w] = (uint32_t)__popcll(bm);
+    if (lane == kWave - 1) {
+        wd[w] = sd;
+        wm[w] = sm;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -195,17 +213,20 @@ __global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restr
         bases[1] = tm ? atomicAdd(nmid, tm) : 0u;
     }
     __syncthreads();
-    const uint32_t based = bases[0] + wd[w], basem = bases[1] + wm[w];
-    if (c >= ncell) return;
-    int32_t d = -1;
-    if (dense) {
-        d = (int32_t)(based + (uint32_t)__popcll(bd & lt));
-        dense_cells[d] = c;
-    } else if (mid) {
-        d = kAuxMid;
-        mid_cells[basem + (uint32_t)__popcll(bm & lt)] = c;
+    if (c0 >= ncell) return;
+    uint32_t bd = bases[0] + wd[w] + sd - nd, bm = bases[1] + wm[w] + sm - nm;
+    for (int k = 0; k < kMarkPer && c0 + k < ncell; ++k) {
+        const uint32_t c = (uint32_t)(c0 + k);
+        int32_t v = -1;
+        if ((dbits >> k) & 1u) {
+            v = (int32_t)bd;
+            dense_cells[bd++] = c;
+        } else if ((mbits >> k) & 1u) {
+            v = kAuxMid;
+            mid_cells[bm++] = c;
+        }
+        aux[c] = v;
     }
-    aux[c] = d;
 }
 
 __device__ inline void cell_lower_corner(const RegionDev& g, uint32_t lc, double lo[3], int32_t ijk[3]) {
@@ -1615,8 +1636,9 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
         PYFOF_LAUNCH_CHECK();
         uint32_t* mid_cells = C.mid.as<uint32_t>(N / kMidMin + 1);
-        k_dense_mark<<<(uint32_t)((ncell + kMarkBlk - 1) / kMarkBlk), kMarkBlk, 0, s>>>(cell_ptr, (uint32_t)ncell, d_coff, d_reg, H, aux,
-                                                   dense_cells, scal, mid_cells, scal + 8);
+        const uint64_t mark_cells = (uint64_t)kMarkBlk * kMarkPer;
+        k_dense_mark<<<(uint32_t)((ncell + mark_cells - 1) / mark_cells), kMarkBlk, 0, s>>>(
+            cell_ptr, (uint32_t)ncell, d_coff, d_reg, H, aux, dense_cells, scal, mid_cells, scal + 8);
         PYFOF_LAUNCH_CHECK();
         uint32_t cnts[9];
         PYFOF_HIP(hipMemcpyAsync(cnts, scal, sizeof(cnts), hipMemcpyDeviceToHost, s));
