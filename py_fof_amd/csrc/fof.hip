@@ -1236,27 +1236,31 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
         dense_cross<false>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
 }
 
-__global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t n) {
+// labels, and each root's own count: size[p] = 1 for a root, 0 otherwise
+// (plain stores; k_count then adds the other members)
+__global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t* __restrict__ size, uint32_t n) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) label[p] = dsu_find(parent, (int32_t)p);
+    if (p >= n) return;
+    const int32_t r = dsu_find(parent, (int32_t)p);
+    label[p] = r;
+    size[p] = r == (int32_t)p ? 1u : 0u;
 }
 
-// component sizes; runs of equal labels inside a wave are summed first
+// component sizes: the members that are not roots, with runs of equal labels
+// inside a wave summed first (a root, or a position past n, ends a run)
 __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < n;
-    const int32_t lab = valid ? label[p] : -1;
-    const int32_t prev = __shfl_up(lab, 1, kWave);
     const int lane = __lane_id();
-    const bool head = valid && (lane == 0 || prev != lab);
+    const int32_t l = p < n ? label[p] : -1;
+    const bool valid = p < n && l != (int32_t)p;
+    const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
+    const int32_t prev = __shfl_up(lab, 1, kWave);
+    const bool head = lane == 0 || prev != lab;
     const uint64_t heads = __ballot(head);
-    const uint64_t vmask = __ballot(valid);
-    if (head) {
+    if (valid && head) {
         const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
         const int next = above ? __ffsll((long long)above) - 1 : 64;
-        const int last_valid = 63 - __clzll((long long)vmask);
-        const int end = next < last_valid + 1 ? next : last_valid + 1;
-        atomicAdd(&size[lab], (uint32_t)(end - lane));
+        atomicAdd(&size[lab], (uint32_t)(next - lane));
     }
 }
 
@@ -1707,9 +1711,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* size = C.size.as<uint32_t>(N);
         uint32_t* flag = C.flag.as<uint32_t>(N);
         uint32_t* cidx = C.cidx.as<uint32_t>(N);
-        k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, N);
-        PYFOF_LAUNCH_CHECK();
-        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(size, N, 0u);
+        k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, size, N);
         PYFOF_LAUNCH_CHECK();
         k_count<<<nblk(N), kBlk, 0, s>>>(label, N, size);
         PYFOF_LAUNCH_CHECK();
