@@ -332,6 +332,24 @@ __device__ __forceinline__ bool pair_linked(const float4& a, const float4& q, co
     return pred_flat(a, q, P.rp2);
 }
 
+// pair_linked with an f32 screen for the periodic predicate. Its coordinate
+// differences are the f32 differences pred_periodic widens; the f32 sum of
+// their squares is within 2e-7 (relative) of the exact sum, so outside
+// b^2 (1 -+ 1e-6) it decides as the f64 test does. Near the threshold, or
+// where a difference may wrap (>= Lh / 2), the f64 test decides.
+template <bool PERIODIC>
+__device__ __forceinline__ bool pair_linked_screened(const float4& a, const float4& q, const LinkParams& P) {
+    if (PERIODIC) {
+        const float fx = fabsf(a.x - q.x), fy = fabsf(a.y - q.y), fz = fabsf(a.z - q.z);
+        if (fmaxf(fx, fmaxf(fy, fz)) < P.scr_wrap) {
+            const float s = __fadd_rn(__fadd_rn(__fmul_rn(fx, fx), __fmul_rn(fy, fy)), __fmul_rn(fz, fz));
+            if (s < P.scr_lo) return true;
+            if (s > P.scr_hi) return false;
+        }
+    }
+    return pair_linked<PERIODIC>(a, q, P);
+}
+
 // Tests the particle `me` (position p) against cell jc, reached by neighbour
 // offset o (13: its own cell). For the own cell only partners q > p are
 // tested. Pruning is exact: a pair passing the predicate is within b (+ float
@@ -675,7 +693,7 @@ __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const fl
     for (uint32_t t = lane; t < total; t += kWave) {
         uint32_t j = k + m;
         if (j >= n) j -= n;
-        if (pair_linked<PERIODIC>(pos[k], pos[j], P)) lds_unite(lp, (int32_t)k, (int32_t)j);
+        if (pair_linked_screened<PERIODIC>(pos[k], pos[j], P)) lds_unite(lp, (int32_t)k, (int32_t)j);
         k += kWave;
         while (k >= n) { k -= n; ++m; }
     }
@@ -796,7 +814,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             if (!denseB) {
                 for (uint32_t t = lane; t < na * nb; t += kWave) {
                     const uint32_t i = t / nb, j = t - i * nb;
-                    if (pair_linked<PERIODIC>(pa[ia[i]], pb[j], P)) {
+                    if (pair_linked_screened<PERIODIC>(pa[ia[i]], pb[j], P)) {
                         LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
                         L.link((int32_t)(sB + ib[j]));
                     }
@@ -811,7 +829,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                         const float4 a = pa[ia[i]], q = pb[j];
                         xa = __float_as_int(a.w);
                         xb = __float_as_int(q.w);
-                        if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked<PERIODIC>(a, q, P);
+                        if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked_screened<PERIODIC>(a, q, P);
                     }
                     uint64_t m = __ballot(hit);
                     while (m) {
@@ -1604,6 +1622,9 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         P.rp2 = A.b * A.b;
         P.m = m;
         P.R = A.b + m;
+        P.scr_lo = (float)(P.rp2 * (1.0 - 1e-6));
+        P.scr_hi = (float)(P.rp2 * (1.0 + 1e-6));
+        P.scr_wrap = (float)(0.5 * P.Lh);
         P.np_min = A.np_min;
         P.nregions = H;
         P.N = N;

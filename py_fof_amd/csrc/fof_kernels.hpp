@@ -17,6 +17,10 @@ struct LinkParams {
     uint32_t N, ncell;
     int32_t sub_stride;
     int32_t intra_stage;  // dense-cell members staged in LDS by k_dense_intra
+    // f32 screen of the periodic predicate: squared distances below scr_lo
+    // link and above scr_hi do not (b^2 (1 -+ 1e-6) rounded to float); pairs
+    // with a coordinate difference of scr_wrap (Lh / 2) or more use f64
+    float scr_lo, scr_hi, scr_wrap;
 };
 
 constexpr int kDenseMin = 256;    // cells with at least this many particles get sub-cells
