@@ -26,8 +26,16 @@ struct LinkParams {
 constexpr int kDenseMin = 256;    // cells with at least this many particles get sub-cells
 constexpr int kMaxSubPerDim = 16; // sub-cells per dim; at most 4096 per cell
 
-// gfortran real MODULO: exact fmod, shifted into the sign of p.
+// gfortran real MODULO: exact fmod, shifted into the sign of p. For p > 0
+// and -p < a < 2p the result is found without fmod, identically: a itself
+// on [0, p); a - p on [p, 2p), exact by Sterbenz's lemma, as fmod is; and on
+// (-p, 0) fmod returns a, which is then shifted by the same rounded a + p.
 __device__ __forceinline__ double f_modulo(double a, double p) {
+    if (p > 0.0) {
+        if (a >= 0.0 && a < p) return a;
+        if (a >= p && a < 2.0 * p) return a - p;
+        if (a < 0.0 && a > -p) return a + p;
+    }
     double r = fmod(a, p);
     if (r != 0.0 && ((r < 0.0) != (p < 0.0))) r += p;
     return r;
