@@ -361,7 +361,7 @@ __device__ __forceinline__ bool pair_linked_screened(const float4& a, const floa
 // mid cell is skipped: k_cell_pair takes its pairs with this cell.
 template <bool PERIODIC>
 __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
-                                           const int32_t jijk[3], const double q[3], bool own,
+                                           const int32_t jijk[3], const double q[3], bool own, bool back,
                                            const RegionDev& g, const LinkParams& P,
                                            const uint32_t* __restrict__ cell_ptr,
                                            const int32_t* __restrict__ aux,
@@ -372,9 +372,11 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
             if (q[d] + P.R < lo - P.m || q[d] - P.R > lo + g.cell[d] + P.m) return;
         }
     }
+    // class and member range loaded together, before any branch on them
     const int32_t cls = aux[jc];
-    if (!own && cls == kAuxMid) return;
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
+    if (!own && cls == kAuxMid) return;
+    if (back && cls < 0) return;  // a backward cell is visited for its sub-cells only
     if (s >= e) return;
     if (cls >= 0) {
         L.dmask |= 1u << o;
@@ -437,7 +439,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
     // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
     if (aux[c] == kAuxMid) return 0u;
-    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, g, P, cell_ptr, aux, xc, L);
+    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, false, g, P, cell_ptr, aux, xc, L);
     auto neighbour = [&](int o) {
         int32_t jijk[3];
         double sh[3];
@@ -447,7 +449,6 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
         // clique cell is taken here, from the sparse side, in both directions
         const bool fwd = jl > lc;
         if (!fwd && !g.clique) return;
-        if (!fwd && aux[g.cell_off + jl] < 0) return;
         if (!g.prune) {
             // tiny periodic grids: the same neighbour can be reached twice
             for (int o2 = 0; o2 < o; ++o2) {
@@ -458,7 +459,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
             }
         }
         const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, g, P, cell_ptr, aux, xc, L);
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, !fwd, g, P, cell_ptr, aux, xc, L);
     };
     if (!g.prune) {
         for (int o = 0; o < 27; ++o)
