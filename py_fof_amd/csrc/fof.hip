@@ -1265,21 +1265,45 @@ __global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t
     size[p] = r == (int32_t)p ? 1u : 0u;
 }
 
-// component sizes: the members that are not roots, with runs of equal labels
-// inside a wave summed first (a root, or a position past n, ends a run)
+// component sizes: the members that are not roots. Each wave walks
+// kCountChunks consecutive chunks of kWave positions; runs of equal labels
+// are summed per chunk, and the run that reaches a chunk's last lane is
+// carried into the next chunk instead of being added at once, so a large
+// component costs one atomic per wave rather than per chunk (its counter is
+// the contended one). A root, or a position past n, ends a run.
+constexpr int kCountChunks = 16;
+
 __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    const int32_t l = p < n ? label[p] : -1;
-    const bool valid = p < n && l != (int32_t)p;
-    const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
-    const int32_t prev = __shfl_up(lab, 1, kWave);
-    const bool head = lane == 0 || prev != lab;
-    const uint64_t heads = __ballot(head);
-    if (valid && head) {
-        const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-        const int next = above ? __ffsll((long long)above) - 1 : 64;
-        atomicAdd(&size[lab], (uint32_t)(next - lane));
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t base = wave * kWave * kCountChunks;
+    int32_t carry = -1;  // wave-uniform: the open run's label (>= 0) and count
+    uint32_t ccount = 0;
+    for (int k = 0; k < kCountChunks; ++k) {
+        const uint64_t p = base + (uint64_t)k * kWave + (uint64_t)lane;
+        const int32_t l = p < n ? label[p] : -1;
+        const bool valid = p < n && l != (int32_t)p;
+        const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
+        const int32_t prev = __shfl_up(lab, 1, kWave);
+        const bool head = lane == 0 || prev != lab;
+        const uint64_t heads = __ballot(head);
+        const int32_t lab0 = __shfl(lab, 0, kWave);
+        const int lasthead = 63 - __clzll((long long)heads);
+        const bool last = k + 1 == kCountChunks;
+        const bool merge0 = carry >= 0 && lab0 == carry;
+        if (carry >= 0 && !merge0 && lane == 0) atomicAdd(&size[carry], ccount);
+        uint32_t cnt = 0;
+        if (valid && head) {
+            const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+            const int next = above ? __ffsll((long long)above) - 1 : 64;
+            cnt = (uint32_t)(next - lane) + (lane == 0 && merge0 ? ccount : 0u);
+            if (last || lane != lasthead) atomicAdd(&size[lab], cnt);
+        }
+        // the run at the last head stays open (its label is valid when >= 0)
+        const int32_t labL = __shfl(lab, lasthead, kWave);
+        const uint32_t cntL = __shfl(cnt, lasthead, kWave);
+        carry = (!last && labL >= 0) ? labL : -1;
+        ccount = carry >= 0 ? cntL : 0u;
     }
 }
 
@@ -1735,7 +1759,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* cidx = C.cidx.as<uint32_t>(N);
         k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, size, N);
         PYFOF_LAUNCH_CHECK();
-        k_count<<<nblk(N), kBlk, 0, s>>>(label, N, size);
+        k_count<<<nblk((N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(label, N, size);
         PYFOF_LAUNCH_CHECK();
         k_cand_flags<<<nblk(N), kBlk, 0, s>>>(size, N, A.np_min, flag);
         PYFOF_LAUNCH_CHECK();
