@@ -1271,7 +1271,7 @@ __global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t
 // carried into the next chunk instead of being added at once, so a large
 // component costs one atomic per wave rather than per chunk (its counter is
 // the contended one). A root, or a position past n, ends a run.
-constexpr int kCountChunks = 16;
+constexpr int kCountChunks = 64;
 
 __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
     const int lane = __lane_id();
