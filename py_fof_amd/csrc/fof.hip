@@ -914,7 +914,7 @@ __device__ __forceinline__ bool cliques_linked(uint32_t a0, uint32_t a1, uint32_
 // are processed nearest first in classes separated by barriers, so that once
 // the close sub-cells have merged the far offsets are skipped on the root
 // check alone. Cells with at most kIntraStage members are staged in LDS.
-constexpr int kIntraStage = 1024;
+constexpr int kIntraStage = 768;
 constexpr int kFwdClassEnd[5] = {3, 9, 13, 16, 62};
 
 template <bool PERIODIC>
