@@ -1029,6 +1029,8 @@ __global__ void __launch_bounds__(kBlk) k_dense_intra(const uint32_t* __restrict
 // the two components link. If their roots differ, the sub-cell pairs across
 // the shared face/edge/corner (fine offset <= 2 per axis) are searched in
 // parallel, 64 at a time, until one pair of cliques links.
+constexpr int kCrossRound = 16;  // member tests per lane between wave checks in cell_cross
+
 template <bool PERIODIC>
 __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
                            const uint32_t* __restrict__ dense_cells, const uint32_t* __restrict__ cell_ptr,
@@ -1081,7 +1083,8 @@ __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
         const uint32_t csB = cell_ptr[cB];
         for (uint32_t base = 0; base < total; base += kWave) {
             const uint32_t it = base + (uint32_t)lane;
-            bool hit = false;
+            bool hit = false, active = false;
+            uint32_t ta = 0, a1 = 0, tb = 0, b0 = 0, b1 = 0;
             if (it < total) {
                 uint32_t rem = it;
                 int32_t pos[3], opt[3], sa[3], sb[3];
@@ -1110,10 +1113,30 @@ __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
                 if (valid) {
                     const uint32_t ia = (uint32_t)sa[0] + (uint32_t)kk[0] * ((uint32_t)sa[1] + (uint32_t)kk[1] * (uint32_t)sa[2]);
                     const uint32_t ib = (uint32_t)sb[0] + (uint32_t)kk[0] * ((uint32_t)sb[1] + (uint32_t)kk[1] * (uint32_t)sb[2]);
-                    const uint32_t a0 = csA + rowA[ia], a1 = csA + rowA[ia + 1];
-                    const uint32_t b0 = csB + rowB[ib], b1 = csB + rowB[ib + 1];
-                    if (a0 < a1 && b0 < b1) hit = cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P);
+                    ta = csA + rowA[ia];
+                    a1 = csA + rowA[ia + 1];
+                    b0 = csB + rowB[ib];
+                    b1 = csB + rowB[ib + 1];
+                    active = ta < a1 && b0 < b1;
+                    tb = b0;
                 }
+            }
+            // each lane scans its pair of sub-cells member by member, at most
+            // kCrossRound tests between checks of the wave: once any lane has
+            // linked, the others stop (one link joins the two components)
+            float4 xa = active ? xc[sub_list[ta]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (;;) {
+                for (int r = 0; r < kCrossRound && active; ++r) {
+                    if (pair_linked<PERIODIC>(xa, xc[sub_list[tb]], P)) {
+                        hit = true;
+                        active = false;
+                    } else if (++tb == b1) {
+                        tb = b0;
+                        if (++ta == a1) active = false;
+                        else xa = xc[sub_list[ta]];
+                    }
+                }
+                if (__ballot(hit) || !__ballot(active)) break;
             }
             const uint64_t hits = __ballot(hit);
             if (hits) {
