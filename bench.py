@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per call of the bbox + binning + linking kernels on the
 # 134,217,728-particle workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 # scaled by a 1 GiB copy (profiles/fof_nfw134M_hbm_traffic.md)
-TRAFFIC_BYTES_134M = 75.19e9
+TRAFFIC_BYTES_134M = 75.01e9
 PHASES = ["bbox", "binning", "linking", "relabel", "unbinding", "catalog"]
 
 
