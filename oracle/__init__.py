@@ -19,4 +19,8 @@ from .oracle import (  # noqa: F401
     oracle_select,
     oracle_depth,
     oracle_tl_info,
+    oracle_set_force_full,
+    oracle_c2_check,
+    oracle_tl_pp_check,
+    oracle_direct_check,
 )

@@ -864,6 +864,54 @@ static void tl_pp_update(const c2cells_t *cc, int n, const float *q, const char 
     }
 }
 
+/* Direct summation (bound_fortran.f90:2613-2647). Pass 1 visits every pair
+ * i < j once and adds the negative depth to both members, f64 sums of f32
+ * differences; keep == NULL or all kept is the reference's pass 1. A member
+ * not kept contributes nothing and receives nothing. */
+static void direct_full(int n, const float *q, const float *w, const char *keep, double app,
+                        double G_dir, double *phid)
+{
+    for (int i = 0; i < n; ++i) phid[i] = 0.0;
+    for (int i = 0; i < n - 1; ++i) {
+        if (keep && !keep[i]) continue;
+        for (int j = i + 1; j < n; ++j) {
+            if (keep && !keep[j]) continue;
+            double dx = (double)(q[3 * j] - q[3 * i]);
+            double dy = (double)(q[3 * j + 1] - q[3 * i + 1]);
+            double dz = (double)(q[3 * j + 2] - q[3 * i + 2]);
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double dd = G_dir * oracle_depth(r, app);
+            phid[i] = phid[i] - (double)w[j] * dd;
+            phid[j] = phid[j] - (double)w[i] * dd;
+        }
+    }
+}
+
+/* Later passes (bound_fortran.f90:2630-2641): the sources deleted since the
+ * last refresh are added back to every kept member. */
+static void direct_addback(int n, const float *q, const float *w, const char *keep, const int *just,
+                           int njust, double app, double G_dir, double *phid)
+{
+    for (int k = 0; k < njust; ++k) {
+        int j = just[k];
+        for (int i = 0; i < n; ++i) {
+            if (!keep[i]) continue;
+            double dx = (double)(q[3 * j] - q[3 * i]);
+            double dy = (double)(q[3 * j + 1] - q[3 * i + 1]);
+            double dz = (double)(q[3 * j + 2] - q[3 * i + 2]);
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            phid[i] = phid[i] + (double)w[j] * G_dir * oracle_depth(r, app);
+        }
+    }
+}
+
+/* Checker switch (not a reference feature): 1 recomputes every later pass's
+ * potential from scratch over the kept members (direct_full, c2_full,
+ * tl_pp_full) instead of the reference's incremental updates, so that a test
+ * can compare the two paths end to end. */
+static int g_force_full = 0;
+void oracle_set_force_full(int32_t on) { g_force_full = on ? 1 : 0; }
+
 /* energy ordering of bound_check_grid's heap sort: "greater" = larger energy,
  * or equal energy and smaller local index */
 static const float *g_energy;
@@ -952,35 +1000,12 @@ static int bound_check_halo(const bctx_t *c, const htask_t *t, const green_t *gr
         int n_source = 0;
         for (int i = 0; i < n; ++i) n_source += keep[i] ? 1 : 0;
         if (gid == 0) {
-            if (pass == 1) {
-                for (int i = 0; i < n; ++i) phid[i] = 0.0;
-                for (int i = 0; i < n - 1; ++i)
-                    for (int j = i + 1; j < n; ++j) {
-                        double dx = (double)(q[3 * j] - q[3 * i]);
-                        double dy = (double)(q[3 * j + 1] - q[3 * i + 1]);
-                        double dz = (double)(q[3 * j + 2] - q[3 * i + 2]);
-                        double r = sqrt(dx * dx + dy * dy + dz * dz);
-                        double dd = G_dir * oracle_depth(r, c->app);
-                        phid[i] = phid[i] - (double)w[j] * dd;
-                        phid[j] = phid[j] - (double)w[i] * dd;
-                    }
-            } else {
-                for (int k = 0; k < n_just; ++k) {
-                    int j = just[k];
-                    for (int i = 0; i < n; ++i) {
-                        if (!keep[i]) continue;
-                        double dx = (double)(q[3 * j] - q[3 * i]);
-                        double dy = (double)(q[3 * j + 1] - q[3 * i + 1]);
-                        double dz = (double)(q[3 * j + 2] - q[3 * i + 2]);
-                        double r = sqrt(dx * dx + dy * dy + dz * dz);
-                        phid[i] = phid[i] + (double)w[j] * G_dir * oracle_depth(r, c->app);
-                    }
-                }
-            }
+            if (pass == 1 || g_force_full) direct_full(n, q, w, keep, c->app, G_dir, phid);
+            else direct_addback(n, q, w, keep, just, n_just, c->app, G_dir, phid);
             n_just = 0;
             for (int i = 0; i < n; ++i) if (keep[i]) phi[i] = (float)phid[i];
         } else {
-            const int full_c2 = pass == 1;
+            const int full_c2 = pass == 1 || g_force_full;
             const size_t nn = (size_t)ng * ng * ng;
             for (size_t g = 0; g < nn; ++g) { grid[g].re = 0.0; grid[g].im = 0.0; }
             /* deposit in float into a float grid (kept as double storage of float values) */
@@ -1298,4 +1323,70 @@ int32_t oracle_fof_bound(const double *xp, const double *vp, const double *mp,
     for (int32_t h = 0; h < nhalo; ++h) { free(rc[h].cand_np); free(rc[h].member_ptr); free(rc[h].members); }
     free(rc); free(task); free(mem_pid); free(cand_reg); free(cand_n); free(cand_anchor);
     return ierr;
+}
+
+/* ------------------------------------------------------------------ */
+/* Checker hooks: one removal batch, the reference's incremental update   */
+/* against a full recomputation over the members still kept.              */
+/* q: (n, 3) f32 positions, w: f32 masses, keep0: members kept before the */
+/* batch, rem: the nrem members the batch deletes (kept in keep0).        */
+/* phi_inc: full sum over keep0 then the incremental update for rem;      */
+/* phi_full: full sum over keep0 minus rem. Returns 0, or 1 on bad input. */
+
+static char *apply_removal(int n, const char *keep0, int nrem, const int32_t *rem)
+{
+    char *k1 = (char *)malloc((size_t)(n > 0 ? n : 1));
+    memcpy(k1, keep0, (size_t)n);
+    for (int k = 0; k < nrem; ++k) {
+        if (rem[k] < 0 || rem[k] >= n || !k1[rem[k]]) { free(k1); return NULL; }
+        k1[rem[k]] = 0;
+    }
+    return k1;
+}
+
+int32_t oracle_c2_check(int32_t n, const float *q, const float *w, const char *keep0, int32_t nrem,
+                        const int32_t *rem, float sp, float app, double a, float *phi_inc,
+                        float *phi_full)
+{
+    char *k1 = apply_removal(n, keep0, nrem, rem);
+    if (!k1) return 1;
+    c2cells_t cc;
+    build_cells(&cc, n, q, sp);
+    c2_full(&cc, n, q, keep0, w, sp, app, a, phi_inc);
+    c2_incremental(&cc, n, q, k1, w, rem, nrem, sp, app, a, phi_inc);
+    c2_full(&cc, n, q, k1, w, sp, app, a, phi_full);
+    free_cells(&cc);
+    free(k1);
+    return 0;
+}
+
+/* pr: per-member refinement ratio (1, 2 or 4); cutoff a_i = 3.5 sp / pr_i */
+int32_t oracle_tl_pp_check(int32_t n, const float *q, const float *w, const char *keep0, int32_t nrem,
+                           const int32_t *rem, float sp, float app, double a, const int32_t *pr,
+                           float *phi_inc, float *phi_full)
+{
+    char *k1 = apply_removal(n, keep0, nrem, rem);
+    if (!k1) return 1;
+    c2cells_t cc;
+    build_cells(&cc, n, q, sp);
+    tl_pp_full(&cc, n, q, keep0, w, sp, app, a, pr, phi_inc);
+    tl_pp_update(&cc, n, q, k1, w, rem, nrem, sp, app, a, pr, phi_inc);
+    tl_pp_full(&cc, n, q, k1, w, sp, app, a, pr, phi_full);
+    free_cells(&cc);
+    free(k1);
+    return 0;
+}
+
+int32_t oracle_direct_check(int32_t n, const float *q, const float *w, const char *keep0, int32_t nrem,
+                            const int32_t *rem, double app, double a, double *phi_inc,
+                            double *phi_full)
+{
+    char *k1 = apply_removal(n, keep0, nrem, rem);
+    if (!k1) return 1;
+    const double G_dir = a / (6.0 * BPI);
+    direct_full(n, q, w, keep0, app, G_dir, phi_inc);
+    direct_addback(n, q, w, k1, rem, nrem, app, G_dir, phi_inc);
+    direct_full(n, q, w, k1, app, G_dir, phi_full);
+    free(k1);
+    return 0;
 }

@@ -232,3 +232,54 @@ def oracle_tl_info(q, spacing, origin, app):
 
 def oracle_depth(r, h):
     return load_oracle().oracle_depth(float(r), float(h))
+
+
+def oracle_set_force_full(on):
+    """Checker switch: every later pass recomputes its potential from scratch
+    over the kept members instead of the reference's incremental updates."""
+    lib = load_oracle()
+    lib.oracle_set_force_full.restype = None
+    lib.oracle_set_force_full.argtypes = [ctypes.c_int32]
+    lib.oracle_set_force_full(1 if on else 0)
+
+
+def _incremental_check(name, q, w, keep0, removed, extra, phi_dtype):
+    lib = load_oracle()
+    q = np.ascontiguousarray(np.asarray(q, np.float32).T)  # (n, 3)
+    n = q.shape[0]
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    k0 = np.ascontiguousarray(keep0, dtype=np.int8)
+    rem = np.ascontiguousarray(removed, dtype=np.int32)
+    ct = ctypes.c_float if phi_dtype == np.float32 else ctypes.c_double
+    inc = np.zeros(max(n, 1), phi_dtype)
+    full = np.zeros(max(n, 1), phi_dtype)
+    fn = getattr(lib, name)
+    fn.restype = ctypes.c_int32
+    rc = fn(int(n), _ptr(q, ctypes.c_float), _ptr(w, ctypes.c_float),
+            k0.ctypes.data_as(ctypes.c_char_p), int(len(rem)), _ptr(rem, ctypes.c_int32), *extra,
+            _ptr(inc, ct), _ptr(full, ct))
+    if rc != 0:
+        raise ValueError(f"{name}: invalid removal list")
+    return inc[:n], full[:n]
+
+
+def oracle_c2_check(q, w, keep0, removed, sp, app, a=1.0):
+    """C2 residual after one removal batch: (incremental, full recompute),
+    f32 (bound_fortran.f90:3403-3552)."""
+    extra = (ctypes.c_float(sp), ctypes.c_float(app), ctypes.c_double(a))
+    return _incremental_check("oracle_c2_check", q, w, keep0, removed, extra, np.float32)
+
+
+def oracle_tl_pp_check(q, w, keep0, removed, sp, app, ratios, a=1.0):
+    """Two-level short-range sums after one removal batch: (pp_update,
+    pp_full), f32 (bound_fortran.f90:1088-1498)."""
+    pr = np.ascontiguousarray(ratios, dtype=np.int32)
+    extra = (ctypes.c_float(sp), ctypes.c_float(app), ctypes.c_double(a), _ptr(pr, ctypes.c_int32))
+    return _incremental_check("oracle_tl_pp_check", q, w, keep0, removed, extra, np.float32)
+
+
+def oracle_direct_check(q, w, keep0, removed, app, a=1.0):
+    """Direct-sum potential after one removal batch: (add-back, fresh sum),
+    f64 (bound_fortran.f90:2613-2647)."""
+    extra = (ctypes.c_double(app), ctypes.c_double(a))
+    return _incremental_check("oracle_direct_check", q, w, keep0, removed, extra, np.float64)

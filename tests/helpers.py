@@ -98,3 +98,73 @@ def labels_to_canonical(labels):
     out[order] = mins[grp]
     del first
     return out
+
+
+def depth_kernel(r, h):
+    """Softened depth D(r, h) of the direct sum (bound_fortran.f90:3080-3102),
+    elementwise in f64; D(0, h) = 208 / (70 h). Used only to construct inputs."""
+    r = np.asarray(r, np.float64)
+    e = 2.0 * r / h
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inner = (208 - e ** 2 * (112 + e ** 2 * (-56 + e * (14 + e * (8 - 3 * e))))) / (70 * h)
+        outer = (128 - (e ** 2 * (448 + e * (-280 + e * (56 + e * (14 + e * (-8 + e)))))
+                        - 224 * e - 12 / e)) / (70 * h)
+        return np.where(r >= h, 1.0 / r, np.where(e <= 1.0, inner, outer))
+
+
+def max_iter_halo(n_shells=24, n_core_pairs=60, step=0.1, app=0.1, seed=7):
+    """A direct-summation halo built so that the removal loop deletes exactly
+    one shell per potential refresh and so reaches the 20-pass cap
+    (status 5, bound_fortran.f90:2857-2864).
+
+    An inversion-symmetric core (the origin plus n_core_pairs +/- pairs inside
+    radius 0.8 step) at rest, and shells k = 1..n_shells of six members at
+    +/- k step on the three axes, each with a tangential speed (the six
+    momenta cancel). Shell k's kinetic energy sits halfway between minus its
+    potential with shell k + 1 present and without it, so it is bound
+    (E = -delta/2) while shell k + 1 remains and unbound (E = +delta/2) once
+    shell k + 1 is gone; the outermost shell starts unbound. Every pass
+    therefore removes one shell and leaves the next one unbound for the next
+    refresh: after 20 passes 20 shells are gone, n_shells - 20 remain with the
+    core, and the candidate ends at MAX_ITER. Cube units, a = 1, masses 1,
+    linking length 0.2 > step links every member. Returns (xp, vp, retained).
+    """
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((3, n_core_pairs))
+    u /= np.sqrt((u * u).sum(0))
+    core_half = u * (0.8 * step * rng.uniform(0.2, 1.0, n_core_pairs) ** (1.0 / 3.0))
+    core = np.hstack([np.zeros((3, 1)), core_half, -core_half])
+    axes = np.eye(3)
+    tang = np.roll(np.eye(3), 1, axis=0)  # x -> y, y -> z, z -> x
+    shell_x, shell_dir = [], []
+    for k in range(1, n_shells + 1):
+        for ax in range(3):
+            for sgn in (1.0, -1.0):
+                shell_x.append(sgn * k * step * axes[:, ax])
+                shell_dir.append(sgn * tang[:, ax])
+    sx = np.array(shell_x).T
+    sd = np.array(shell_dir).T
+    x = np.hstack([core, sx])
+    nc = core.shape[1]
+    n = x.shape[1]
+    G = 1.0 / (6.0 * np.pi)
+    d = x[:, :, None] - x[:, None, :]
+    D = depth_kernel(np.sqrt((d * d).sum(0)), app)
+    np.fill_diagonal(D, 0.0)
+    shell_of = np.concatenate([np.zeros(nc, int), np.repeat(np.arange(1, n_shells + 1), 6)])
+    speed = np.zeros(n)
+    for k in range(1, n_shells + 1):
+        idx = np.nonzero(shell_of == k)[0]
+        for i in idx:
+            phi_with = -G * D[i, shell_of <= k + 1].sum()
+            phi_without = -G * D[i, shell_of <= k].sum()
+            if k == n_shells:
+                ke = -phi_without * 1.5  # E = +phi/2 ... well above zero
+            else:
+                ke = -0.5 * (phi_with + phi_without)
+            speed[i] = np.sqrt(2.0 * ke)
+    v = np.zeros((3, n))
+    v[:, nc:] = sd * speed[nc:]
+    x = x + 256.0
+    retained = nc + 6 * (n_shells - 20)
+    return np.asfortranarray(x), np.asfortranarray(v), retained
