@@ -437,8 +437,11 @@ def main():
 
     if args.only == "bound":
         ctx = _native.Context(0)
-        print(json.dumps({"metric": METRIC, "n_gpus": 1,
-                          "unbinding": bench_bound(args, torch, _native, ctx)}), flush=True)
+        try:
+            res = bench_bound(args, torch, _native, ctx)
+        finally:
+            ctx.close()
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "unbinding": res}), flush=True)
         return
     if args.only == "copy":
         # calibration for HBM counters: a device copy of 1 GiB (1 GiB read,
@@ -548,6 +551,7 @@ def main():
         del x_d, in_h, hid
         torch.cuda.empty_cache()
         out["unbinding"] = bench_bound(args, torch, _native, ctx)
+    ctx.close()
     if not args.no_serve and args.only == "all":
         out["serving"] = bench_serve(args)
     print(json.dumps(out), flush=True)

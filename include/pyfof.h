@@ -45,6 +45,13 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
                   int64_t ng_val, double ng_global_val, int32_t* in_halo, int32_t* halo_id,
                   double* ang_mom, int32_t mem_kind);
 
+/* Frees the per-device default contexts pyfof_fof uses (streams and HBM
+ * workspaces), as bound_release (bound_fortran.f90:1816-1840) frees the
+ * reference's module buffers and FFT plans. A later pyfof_fof creates them
+ * again. Must not overlap a pyfof_fof call. Call it before the process
+ * exits, while the HIP runtime is still up. */
+void pyfof_release(void);
+
 /* Context API: explicit device, own stream and reusable HBM workspace. */
 pyfof_ctx* pyfof_ctx_create(int32_t device);
 void pyfof_ctx_destroy(pyfof_ctx* ctx);

@@ -23,4 +23,5 @@ from .oracle import (  # noqa: F401
     oracle_c2_check,
     oracle_tl_pp_check,
     oracle_direct_check,
+    oracle_set_link_threads,
 )

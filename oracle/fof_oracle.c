@@ -402,6 +402,13 @@ static void add_ang_mom(const double *xp, const double *vp, const double *mp,
  * per particle, -1 for particles of skipped regions.
  * Returns ierr (0, 3 or 10) with the reference's meanings.
  */
+/* Checker switch (not a reference feature): k > 0 links every region with k
+ * threads instead of the reference's rule. The union always hooks the larger
+ * root under the smaller, so the partition and its labels do not depend on
+ * the thread count; only the time to build a large fixture does. */
+static int32_t g_link_threads = 0;
+void oracle_set_link_threads(int32_t k) { g_link_threads = k > 0 ? k : 0; }
+
 int32_t oracle_fof(const double *xp, const double *vp, const double *mp,
                    const int32_t *np_halo, int32_t nhalo, double L, double b,
                    int32_t np_min, int32_t n_threads, int32_t total_np,
@@ -423,6 +430,7 @@ int32_t oracle_fof(const double *xp, const double *vp, const double *mp,
     for (int32_t h = 0; h < nhalo; ++h) nmax = imax(nmax, np_halo[h]);
     int32_t team, nest;
     oracle_parallel_config(T, nhalo, nmax, &team, &nest);
+    if (g_link_threads > 0) nest = g_link_threads;
     double Lh = 0.5 * L;
     region_cands_t *rc = (region_cands_t *)calloc((size_t)nhalo, sizeof(region_cands_t));
     int32_t *start = (int32_t *)malloc(sizeof(int32_t) * (size_t)nhalo);

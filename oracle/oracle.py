@@ -283,3 +283,12 @@ def oracle_direct_check(q, w, keep0, removed, app, a=1.0):
     f64 (bound_fortran.f90:2613-2647)."""
     extra = (ctypes.c_double(app), ctypes.c_double(a))
     return _incremental_check("oracle_direct_check", q, w, keep0, removed, extra, np.float64)
+
+
+def oracle_set_link_threads(k):
+    """Checker switch: link every region with k threads (0 restores the
+    reference's rule). The partition does not depend on it."""
+    lib = load_oracle()
+    lib.oracle_set_link_threads.restype = None
+    lib.oracle_set_link_threads.argtypes = [ctypes.c_int32]
+    lib.oracle_set_link_threads(int(k))

@@ -3,10 +3,12 @@
 There is no fallback: if the library is missing or cannot be loaded, every
 call raises. Build it with ``python -m py_fof_amd.build``.
 """
+import atexit
 import ctypes
 import os
 import sys
 import threading
+import weakref
 
 import numpy as np
 
@@ -26,7 +28,7 @@ EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",
     "pyfof_catalog_size", "pyfof_catalog_copy", "pyfof_ctx_timings", "pyfof_ctx_stats",
     "pyfof_ctx_stream", "pyfof_ctx_set_option", "pyfof_ctx_potential",
-    "pyfof_last_error", "pyfof_version",
+    "pyfof_last_error", "pyfof_version", "pyfof_release",
     # request server (py_fof_amd.serve)
     "pyfof_server_create", "pyfof_server_destroy", "pyfof_job_footprint", "pyfof_submit",
     "pyfof_poll", "pyfof_wait", "pyfof_server_status", "pyfof_server_last_error",
@@ -102,7 +104,36 @@ def _load_from(path):
     lib.pyfof_last_error.argtypes = []
     lib.pyfof_version.restype = ctypes.c_char_p
     lib.pyfof_version.argtypes = []
+    lib.pyfof_release.restype = None
+    lib.pyfof_release.argtypes = []
+    atexit.register(_release_all)
     return lib
+
+
+_live = weakref.WeakSet()  # contexts still open
+_servers = weakref.WeakSet()  # request servers still running
+
+
+def register_for_exit(obj):
+    """Closes obj (a request server) at interpreter exit, before the
+    contexts and the default contexts are released."""
+    _servers.add(obj)
+
+
+def _release_all():
+    """Runs at interpreter exit, before the HIP runtime and any profiler
+    attached to it tear down: destroys every context still open (streams,
+    HBM workspaces) and the library's per-device default contexts, so that
+    nothing is freed after the runtime is gone."""
+    for group in (_servers, _live):
+        for c in list(group):
+            try:
+                c.close()
+            except Exception:
+                pass
+    _default_ctx.clear()
+    if _lib is not None:
+        _lib.pyfof_release()
 
 
 def last_error():
@@ -126,6 +157,7 @@ class Context:
         if not self.h:
             raise NativeLibraryError("pyfof_ctx_create failed: " + last_error())
         self.device = device
+        _live.add(self)
 
     def close(self):
         if self.h:

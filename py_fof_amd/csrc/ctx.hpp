@@ -168,5 +168,8 @@ struct CallArgs {
 // message of a failure is last_error_message() on the calling thread.
 int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads);
 const std::string& last_error_message();
+void set_last_error(const std::string& m);
+// Zeroes in_halo, halo_id and ang_mom of a call that ends without a result.
+void clear_outputs(Ctx* C, const CallArgs& A);
 
 }  // namespace pyfof

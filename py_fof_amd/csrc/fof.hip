@@ -1518,6 +1518,8 @@ static thread_local std::string g_last_error;
 
 static void set_error(const std::string& m) { g_last_error = m; }
 
+void set_last_error(const std::string& m) { g_last_error = m; }
+
 const std::string& last_error_message() { return g_last_error; }
 
 static inline uint32_t nblk(uint64_t n) { return div_up(n == 0 ? 1 : n, kBlk); }
@@ -1928,14 +1930,28 @@ static Ctx* default_ctx() {
     return g_default_ctx[dev];
 }
 
-int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
-    set_error("");
-    // fof_fortran.f90:110-123 -- outputs cleared, thread bound, empty input
-    const bool dev_mem = A.mem_kind == PYFOF_MEM_DEVICE;
-    if (A.mem_kind != PYFOF_MEM_HOST && !dev_mem) {
-        set_error("mem_kind must be PYFOF_MEM_HOST or PYFOF_MEM_DEVICE");
-        return IERR_INVALID;
+// fof_fortran.f90:110-113: in_halo, halo_id and ang_mom are zero whenever a
+// call ends without a result (validation errors, ierr 10, empty input, a
+// failed unbinding). Host arrays are cleared here; device arrays on the
+// context's stream. Lengths are the caller's: total_np and 3 * nhalo.
+void clear_outputs(Ctx* C, const CallArgs& A) {
+    const size_t n = A.total_np > 0 ? (size_t)A.total_np : 0;
+    const size_t h = A.nhalo > 0 ? (size_t)A.nhalo : 0;
+    if (A.mem_kind == PYFOF_MEM_HOST) {
+        if (A.in_halo && n) memset(A.in_halo, 0, 4 * n);
+        if (A.halo_id && n) memset(A.halo_id, 0, 4 * n);
+        if (A.ang_mom && h) memset(A.ang_mom, 0, 24 * h);
+        return;
     }
+    if (A.mem_kind != PYFOF_MEM_DEVICE) return;
+    (void)hipSetDevice(C->device);
+    if (A.in_halo && n) (void)hipMemsetAsync(A.in_halo, 0, 4 * n, C->stream);
+    if (A.halo_id && n) (void)hipMemsetAsync(A.halo_id, 0, 4 * n, C->stream);
+    if (A.ang_mom && h) (void)hipMemsetAsync(A.ang_mom, 0, 24 * h, C->stream);
+    (void)hipStreamSynchronize(C->stream);
+}
+
+static int32_t validate(const CallArgs& A, int32_t n_threads) {
     if (n_threads > 256) {
         set_error("n_threads exceeds 256");
         return IERR_THREADS;
@@ -1962,21 +1978,43 @@ int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
         set_error("softening must be positive for bound_check");
         return IERR_INVALID;
     }
+    return IERR_OK;
+}
+
+int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
+    set_error("");
+    if (A.mem_kind != PYFOF_MEM_HOST && A.mem_kind != PYFOF_MEM_DEVICE) {
+        set_error("mem_kind must be PYFOF_MEM_HOST or PYFOF_MEM_DEVICE");
+        return IERR_INVALID;
+    }
     std::lock_guard<std::mutex> lk(C->mu);
+    int32_t err = validate(A, n_threads);
+    if (err != IERR_OK || A.nhalo <= 0 || A.total_np <= 0) {
+        const std::string msg = g_last_error;
+        clear_outputs(C, A);
+        set_error(msg);
+        return err;
+    }
     try {
         PYFOF_HIP(hipSetDevice(C->device));
-        return run_fof(*C, A);
+        err = run_fof(*C, A);
     } catch (const HipError& e) {
         set_error(e.what());
         (void)hipStreamSynchronize(C->stream);
-        return e.code == hipErrorOutOfMemory ? IERR_ALLOC : IERR_HIP;
+        err = e.code == hipErrorOutOfMemory ? IERR_ALLOC : IERR_HIP;
     } catch (const std::bad_alloc&) {
         set_error("host allocation failed");
-        return IERR_ALLOC;
+        err = IERR_ALLOC;
     } catch (const std::exception& e) {
         set_error(e.what());
-        return IERR_HIP;
+        err = IERR_HIP;
     }
+    if (err != IERR_OK) {
+        const std::string msg = g_last_error;
+        clear_outputs(C, A);
+        set_error(msg);
+    }
+    return err;
 }
 
 }  // namespace pyfof
@@ -2044,6 +2082,14 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
                ang_mom, nullptr, mem_kind};
     A.ng_global = ng_global_val;
     return fof_entry(C, A, n_threads);
+}
+
+void pyfof_release(void) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (Ctx*& c : g_default_ctx) {
+        delete c;
+        c = nullptr;
+    }
 }
 
 int32_t pyfof_catalog_size(pyfof_ctx* ctx, int32_t* nhalo_found, int64_t* n_members) {

@@ -136,6 +136,7 @@ class Server:
             raise ServerError(self._lib.pyfof_server_last_error().decode(errors="replace"))
         self.device = int(device)
         self._lock = threading.Lock()
+        _native.register_for_exit(self)
 
     def close(self):
         """Waits for every queued and running job, then frees the workers."""

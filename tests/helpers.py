@@ -168,3 +168,26 @@ def max_iter_halo(n_shells=24, n_core_pairs=60, step=0.1, app=0.1, seed=7):
     x = x + 256.0
     retained = nc + 6 * (n_shells - 20)
     return np.asfortranarray(x), np.asfortranarray(v), retained
+
+
+def wrapped_clumps(L, per_clump, centres, n_bg, seed, sigma=0.3):
+    """Gaussian clumps (sigma) at the given centres plus n_bg uniform
+    particles, wrapped into [0, L) and shuffled; (3, N) f64 Fortran order.
+    Clumps centred near a face straddle the periodic boundary."""
+    rng = np.random.default_rng(seed)
+    parts = [np.asarray(c, float)[:, None] + sigma * rng.standard_normal((3, per_clump))
+             for c in centres]
+    parts.append(rng.uniform(0, L, (3, n_bg)))
+    x = np.mod(np.hstack(parts), L)
+    return np.asfortranarray(x[:, rng.permutation(x.shape[1])])
+
+
+# periodic regions with 1 or 2 reference cells per axis, where the -1 and +1
+# neighbours of a cell are the same cell (fof_fortran.f90:729-755):
+# (L, b, members per clump, clump centres, background, sigma, cells per axis)
+TINY_GRID_CASES = [
+    (10.0, 3.4, 50, [(0.1, 0.1, 0.1), (5.1, 0.1, 0.1), (0.1, 5.1, 5.1), (5.1, 5.1, 9.9)], 6, 0.3, 2),
+    (10.0, 5.5, 60, [(0.1, 0.1, 9.95), (5.1, 5.1, 5.1)], 0, 0.3, 1),
+    (10.0, 3.4, 600, [(0.1, 0.1, 0.1), (5.1, 0.1, 0.1), (0.1, 5.1, 5.1), (5.1, 5.1, 9.9)], 0, 0.15, 2),
+    (10.0, 5.5, 900, [(0.1, 0.1, 9.95), (5.1, 5.1, 5.1)], 0, 0.15, 1),
+]

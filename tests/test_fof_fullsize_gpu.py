@@ -1,7 +1,14 @@
-"""FoF at the benchmark's full size (134,217,728 particles, L = 512, b = 0.2),
-checked through properties that do not depend on size. The oracle cannot run
-here in seconds; the same kernels are compared with it bit for bit at 2M
-particles and below (test_fof_gpu.py).
+"""FoF at the benchmark's full size (134,217,728 particles, L = 512, b = 0.2).
+
+Bit-exact against the oracle through the golden fixture
+tests/golden/c2_nfw134M.json (written by tests/golden/make_c2_fixture.py,
+which ran the oracle on this input for about 5 minutes on 8 cores): the
+SHA-256 digests of the input, of the partition roots, of halo_id and of
+in_halo, and the halo and member counts. The input digest is checked first,
+so a generator that does not reproduce the same float64 positions on this
+host fails loudly instead of comparing other data.
+
+Size-independent properties are checked as well:
 
 * the output is deterministic: a second call gives the same halo_id;
 * the partition roots (per particle, the input index of its component's
@@ -19,6 +26,9 @@ distances here therefore use b (1 -+ 1e-3) as the two bounds.
 The partners come from a numpy cell grid of side 1 (> b), searched over the
 27 neighbouring cells with periodic minimum images in float64.
 """
+import hashlib
+import json
+import os
 import sys
 
 import numpy as np
@@ -37,14 +47,40 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def test_fof_134m_properties():
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_nfw134M.json")
+
+
+def _digest_input(xp):
+    h = hashlib.sha256()
+    flat = xp.reshape(-1, order="F")
+    step = 1 << 24
+    for s in range(0, flat.size, step):
+        h.update(np.ascontiguousarray(flat[s:s + step]).tobytes())
+    return h.hexdigest()
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
+
+
+def test_fof_134m_bit_exact_and_properties():
     import py_fof_amd as pf
     from py_fof_amd.synth import clumpy_nfw
+    with open(FIXTURE) as f:
+        gold = json.load(f)
+    assert gold["n"] == N and gold["L_Box"] == L and gold["seed"] == 2
     _say("case: 134,217,728-particle NFW-clump set, seed 2")
     xp = clumpy_nfw(n=N, L=L, seed=2)
+    assert _digest_input(xp) == gold["input_sha256"], \
+        "the generator gave other positions on this host: the fixture does not apply"
     in_h, hid, _, ex = pf._fof_core(xp, L_Box=L, linking_length=B, np_halo_min=NP_MIN,
                                     _want_roots=True)
     roots = ex["roots"]
+    assert int(hid.max()) == gold["nhalo"] and int(in_h.sum()) == gold["members"]
+    assert _digest(roots) == gold["roots_sha256"], "partition differs from the oracle"
+    assert _digest(hid) == gold["halo_id_sha256"], "halo_id differs from the oracle"
+    assert _digest(in_h) == gold["in_halo_sha256"], "in_halo differs from the oracle"
+    _say(f"bit-exact against the oracle: {gold['nhalo']} halos, {gold['members']} members")
     _, hid2, _ = pf._fof_core(xp, L_Box=L, linking_length=B, np_halo_min=NP_MIN)
     assert np.array_equal(hid, hid2), "second call differs"
     del hid2

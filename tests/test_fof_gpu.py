@@ -14,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import chain_halo
+from helpers import TINY_GRID_CASES, chain_halo, wrapped_clumps
 from oracle import oracle_fof
 
 import py_fof_amd
@@ -176,6 +176,22 @@ def test_slab_periodic_one_axis_and_tiny_grid():
     # a periodic box with only 3 cells per axis (no pruning path)
     xp = rng.uniform(0, 3.0, (3, 4000))
     g, r = _run_both([np.asfortranarray(xp)], 3.0, 0.9, 3)
+    _check(g, r)
+
+
+@pytest.mark.parametrize("case", range(len(TINY_GRID_CASES)))
+def test_periodic_one_and_two_cells_per_axis(case):
+    # the -1 and +1 neighbour of a cell are one cell (fof_fortran.f90:729-755);
+    # the larger clumps put 150-450 members in a cell (mid and dense cells)
+    L, b, pc, cs, nbg, sig, nd = TINY_GRID_CASES[case]
+    xp = wrapped_clumps(L, pc, cs, nbg, seed=case + 1, sigma=sig)
+    g, r = _run_both([xp], L, b, 5)
+    _check(g, r)
+    assert len(np.unique(r[3])) >= 2
+    # the same clumps inside a larger region: one cell per axis only in the
+    # sub-box of two regions
+    xq = wrapped_clumps(L, pc, cs, nbg, seed=case + 11, sigma=sig)
+    g, r = _run_both([xp, xq], L, b, 5)
     _check(g, r)
 
 

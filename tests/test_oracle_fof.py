@@ -10,7 +10,8 @@ closed form (fof_fortran.f90:224-248).
 import numpy as np
 import pytest
 
-from helpers import brute_force_labels, chain_halo, labels_to_canonical, modulo
+from helpers import (TINY_GRID_CASES, brute_force_labels, chain_halo, labels_to_canonical, modulo,
+                     wrapped_clumps)
 from oracle import oracle_fof, oracle_geometry
 
 B = 2.0
@@ -72,6 +73,21 @@ def test_partition_periodic_one_dim():
     assert g["periodic"] == [True, False, False]
     _, _, _, lab = oracle_fof(xp, [n], 30.0, 0.3, 1, with_labels=True)
     assert _partition_equal(lab, brute_force_labels(xp, 30.0, 0.3))
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_partition_tiny_periodic_grids_vs_brute_force(case):
+    # 1 or 2 cells per axis: each neighbour cell is visited once
+    # (fof_fortran.f90:729-755); every pair lies in adjacent cells, so the
+    # all-pairs brute force is the reference partition
+    L, b, pc, cs, nbg, sig, nd = TINY_GRID_CASES[case]
+    xp = wrapped_clumps(L, pc, cs, nbg, seed=case + 1, sigma=sig)
+    n = xp.shape[1]
+    g = oracle_geometry(xp, L, b)
+    assert g["n"] == [nd] * 3 and g["periodic"] == [True] * 3
+    _, _, _, lab = oracle_fof(xp, [n], L, b, 1, with_labels=True)
+    assert _partition_equal(lab, brute_force_labels(xp, L, b))
+    assert len(np.unique(lab)) >= 2
 
 
 def test_partition_vs_ckdtree():
