@@ -107,27 +107,11 @@ def bound_workload(n_halos, seed=3):
 
 
 def physical_workload(n_halos, seed=4):
-    """The isolated-halo generator in physical units: positions of the cube
-    generator (grid of n_sim^(1/3) = 1024 cells over box 300 Mpc/h) times
-    dx, velocities from the virial sigma with the cube-unit G at a = 2/3
-    (z = 0.5) converted to km/s, masses log-normal (0.3 dex) around
-    2e9 Msun/h."""
-    from py_fof_amd.synth import isolated_halos
-    box, ng, z, om = 300.0, 1024.0, 0.5, 0.3
-    a = 1.0 / (1.0 + z)
-    xs, vs = isolated_halos(n_halos=n_halos, seed=seed, L=ng, a_val=a)
-    dx = box / ng
-    v_scale = (a * ng) / (150.0 * np.sqrt(om) * box)
-    xp = np.asfortranarray(np.hstack(xs) * dx)
-    vp = np.asfortranarray(np.hstack(vs) / v_scale)
-    rng = np.random.default_rng(seed + 100)
-    mp = 2e9 * 10.0 ** rng.normal(0.0, 0.3, xp.shape[1])
-    np_halo = np.array([x.shape[1] for x in xs], np.int32)
-    return [xp[:, o:o + k] for o, k in zip(np.concatenate([[0], np.cumsum(np_halo)[:-1]]), np_halo)], \
-        [vp[:, o:o + k] for o, k in zip(np.concatenate([[0], np.cumsum(np_halo)[:-1]]), np_halo)], \
-        mp, dict(L_Box=box, linking_length=0.2 * dx, np_halo_min=10, bound_check=True,
-                 units="physical", box=box, n_sim=int(ng ** 3), z=z, omega_m=om,
-                 include_hubble_flow=True, softening=0.01)
+    """The isolated-halo generator in physical units: box 300 Mpc/h,
+    n_sim 1024^3, z 0.5, Hubble flow, masses log-normal (0.3 dex) around
+    2e9 Msun/h, softening 0.01 Mpc/h (py_fof_amd.synth.physical_halos)."""
+    from py_fof_amd.synth import physical_halos
+    return physical_halos(n_halos=n_halos, seed=seed)
 
 
 def bench_physical(args, _native, dmax):

@@ -24,6 +24,7 @@ __all__ = [
     "clumpy_nfw",
     "halo_sizes",
     "isolated_halos",
+    "physical_halos",
 ]
 
 
@@ -205,3 +206,31 @@ def isolated_halos(n_halos=20_000, seed=3, alpha=1.9, nmin=20, nmax=1_000_000, L
         xs.append(np.asfortranarray(x[:, perm] + centre[:, None]))
         vs.append(np.asfortranarray(v[:, perm]))
     return xs, vs
+
+
+def physical_halos(n_halos=20_000, seed=4, sizes=None, box=300.0, n_side=1024, z=0.5,
+                   omega_m=0.3, log_mass=(2e9, 0.3)):
+    """The isolated-halo set in physical units (Mpc/h, km/s, Msun/h).
+
+    Positions of ``isolated_halos`` on a grid of ``n_side`` cells over ``box``
+    (times dx = box / n_side), velocities from the virial sigma with the
+    cube-unit G at a = 1 / (1 + z) converted to km/s, per-particle masses
+    log-normal (``log_mass[1]`` dex) around ``log_mass[0]``. Returns
+    (xp_list, vp_list, mp, kwargs) where kwargs are the fof() arguments of
+    the set: b 0.2 cells, np_halo_min 10, bound_check, n_sim n_side^3, the
+    Hubble flow, softening 0.01 Mpc/h.
+    """
+    a = 1.0 / (1.0 + z)
+    ng = float(n_side)
+    xs, vs = isolated_halos(n_halos=n_halos, seed=seed, L=ng, a_val=a, sizes=sizes)
+    dx = box / ng
+    v_scale = (a * ng) / (150.0 * np.sqrt(omega_m) * box)
+    counts = [x.shape[1] for x in xs]
+    rng = np.random.default_rng(seed + 100)
+    mp = log_mass[0] * 10.0 ** rng.normal(0.0, log_mass[1], int(sum(counts)))
+    xs = [np.asfortranarray(x * dx) for x in xs]
+    vs = [np.asfortranarray(v / v_scale) for v in vs]
+    kw = dict(L_Box=box, linking_length=0.2 * dx, np_halo_min=10, bound_check=True,
+              units="physical", box=box, n_sim=int(n_side) ** 3, z=z, omega_m=omega_m,
+              include_hubble_flow=True, softening=0.01)
+    return xs, vs, mp, kw

@@ -191,3 +191,33 @@ TINY_GRID_CASES = [
     (10.0, 3.4, 600, [(0.1, 0.1, 0.1), (5.1, 0.1, 0.1), (0.1, 5.1, 5.1), (5.1, 5.1, 9.9)], 0, 0.15, 2),
     (10.0, 5.5, 900, [(0.1, 0.1, 9.95), (5.1, 5.1, 5.1)], 0, 0.15, 1),
 ]
+
+
+def physical_to_code(xs, vs, mp, L_Box, linking_length, softening, box=300.0, n_sim=None,
+                     z=0.0, h0=0.67, omega_m=0.3, omega_l=0.7, include_hubble_flow=False,
+                     ng_global=None):
+    """The reference's physical -> code unit conversion (py_fof/__init__.py:
+    266-326) with its own expressions and operation order, for feeding the
+    oracle the inputs the reference's native code would receive. Returns
+    (xp, vp, mp, kw) with kw the oracle scalars L, b, app, a_val, b_val, s15."""
+    a_val = 1.0 / (1.0 + float(z))
+    if include_hubble_flow:
+        t0 = 1.0 / (150.0 * float(h0) * np.sqrt(float(omega_m)))
+        hubble = 100.0 * float(h0) * np.sqrt(float(omega_m) / (a_val ** 3) + float(omega_l))
+        b_val = hubble * (a_val ** 2) * t0
+    else:
+        b_val = 0.0
+    if n_sim is not None and float(n_sim) > 0:
+        ng_eff = float(round(float(n_sim) ** (1.0 / 3.0)))
+    else:
+        ng_eff = float(ng_global) if ng_global is not None else 512.0
+    box_mpc_h = float(L_Box)
+    dx = box_mpc_h / ng_eff
+    xp = np.asfortranarray(np.hstack(xs) / dx, dtype=np.float64)
+    v_scale = (a_val * ng_eff) / (150.0 * np.sqrt(float(omega_m)) * box_mpc_h)
+    vp = np.asfortranarray(np.hstack(vs) * v_scale, dtype=np.float64)
+    m_grid = float(omega_m) * 2.77536627e11 * (box_mpc_h ** 3) / (ng_eff ** 3)
+    mpc = np.ascontiguousarray(np.asarray(mp) / m_grid, dtype=np.float64)
+    kw = dict(L=ng_eff, b=float(linking_length) / dx, app=float(softening) / dx, a_val=a_val,
+              b_val=b_val, s15=15.0 * float(h0) * ng_eff / box_mpc_h)
+    return xp, vp, mpc, kw

@@ -1,9 +1,11 @@
 """fof() / fof_with_info() / the server on torch tensors in HBM.
 
-The host-array path is checked against the oracle elsewhere; here every
-device-tensor call must give exactly what the same call on numpy arrays
-gives: the library computes on the same float64 values, so in_halo, halo_id,
-the catalog and ang_mom are equal bit for bit. Inputs cover the layouts a
+Every device-tensor call is checked against the oracle (FoF bit-exact,
+unbinding on direct-summation halos bit-exact, the physical-unit conversion
+restated from the reference in tests/helpers.physical_to_code) and must give
+exactly what the same call on numpy arrays gives: the library computes on
+the same float64 values, so in_halo, halo_id, the catalog and ang_mom are
+equal bit for bit. Inputs cover the layouts a
 caller can hold: a (3, N) row-major tensor (packed by one device copy), the
 transposed view of an (N, 3) tensor (read in place), a list of regions,
 float32 positions, and physical units with device masses.
@@ -15,6 +17,9 @@ import pytest
 # imported before any test loads the library: the library then lets torch's
 # own HIP runtime initialise first (see py_fof_amd._native._torch_first)
 import torch  # noqa: F401
+
+from helpers import physical_to_code
+from oracle import oracle_fof, oracle_fof_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +70,10 @@ def test_fof_device_tensor_equals_host(layout):
     assert np.array_equal(_host(d_id), h_id)
     assert np.array_equal(_host(d_am), h_am)
     assert h_in.any() and h_id.max() > 10
+    o_in, o_id, o_am = oracle_fof(xp, [xp.shape[1]], L, 0.2, 10, vp=vp, n_threads=4)
+    assert np.array_equal(_host(d_in), o_in.astype(bool))
+    assert np.array_equal(_host(d_id), o_id)
+    assert np.allclose(_host(d_am), o_am, rtol=1e-9, atol=1e-12 * np.abs(o_am).max())
     # fof() returns the same in_halo, and no input tensor was modified
     out = pf.fof(xt, L_Box=L, linking_length=0.2, np_halo_min=10)
     assert np.array_equal(_host(out), h_in)
@@ -85,6 +94,10 @@ def test_regions_float32_and_catalog():
     dev = pf.fof_with_info([torch.as_tensor(p, device="cuda:0") for p in p32], L_Box=L,
                            linking_length=0.2, np_halo_min=8)
     assert np.array_equal(_host(dev["in_halo"]), host["in_halo"])
+    x64 = np.asfortranarray(np.hstack([p.astype(np.float64) for p in p32]))
+    o_in, o_id, _ = oracle_fof(x64, [p.shape[1] for p in p32], L, 0.2, 8, n_threads=4)
+    assert np.array_equal(_host(dev["in_halo"]), o_in.astype(bool))
+    assert dev["nhalo_found"] == int(o_id.max())
     assert dev["nhalo_found"] == host["nhalo_found"] > 0
     assert np.array_equal(dev["np_per_halo"], host["np_per_halo"])
     assert len(dev["halo_indices"]) == len(host["halo_indices"])
@@ -113,6 +126,17 @@ def test_unbinding_physical_units_device_masses():
     assert np.array_equal(_host(d_id), h_id)
     assert np.array_equal(_host(d_am), h_am)
     assert h_id.max() > 0
+    # the oracle on the code-unit inputs of the reference's conversion: every
+    # region here is summed directly, so the bound sets are exact
+    xp_c, vp_c, mp_c, ok = physical_to_code(xs_p, vs, np.concatenate(mps), box, 0.2 * dx,
+                                            0.1 * dx, box=box, ng_global=ng)
+    r = oracle_fof_bound(xp_c, [x.shape[1] for x in xs], ok["L"], ok["b"], 10, vp_c, mp=mp_c,
+                         app=ok["app"], a_val=ok["a_val"], b_val=ok["b_val"], s15=ok["s15"])
+    assert r["ierr"] == 0
+    ref = r["in_halo"].astype(bool)
+    assert int(np.sum(_host(d_in) != ref)) <= 1e-4 * len(ref)
+    if np.all(r["gid"] == 0):
+        assert np.array_equal(_host(d_in), ref) and np.array_equal(_host(d_id), r["halo_id"])
 
 
 def test_errors_and_empty_input():

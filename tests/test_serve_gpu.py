@@ -98,3 +98,45 @@ def test_budget_admission():
         big = _box_job(200_000, 22)
         with pytest.raises(ServerError, match="exceeds the server's HBM budget"):
             srv.submit(big[0], big[1], **big[2])
+
+
+def test_served_jobs_match_the_oracle():
+    """Each job of a concurrent mix against the oracle: plain FoF jobs (one of
+    12,000,000 particles) bit-exact in in_halo and halo_id, angular momentum
+    to rtol 1e-9; unbinding jobs exact on their direct-summation halos and
+    within 1e-4 of the particles overall."""
+    import os
+    import sys
+    from oracle import oracle_fof, oracle_fof_bound, oracle_set_link_threads
+    from py_fof_amd.serve import Server
+    jobs = [_box_job(12_000_000, 31), _box_job(200_000, 32, bound=True), _halo_job(150, 33),
+            _box_job(400_000, 34), _box_job(60_000, 35, bound=True), _box_job(30_000, 36)]
+    print("case: served mix of 6 jobs, largest 12,000,000 particles", flush=True)
+    sys.stdout.flush()
+    with Server(slots=4) as srv:
+        handles = [srv.submit(xp, vp, **kw) for xp, vp, kw in jobs]
+        results = [h.result(with_halo_id=True) for h in handles]
+    oracle_set_link_threads(min(16, os.cpu_count() or 1))
+    try:
+        for (xp, vp, kw), (inh, hid, am) in zip(jobs, results):
+            xs = xp if isinstance(xp, list) else [xp]
+            vs = vp if isinstance(vp, list) else [vp]
+            x = np.asfortranarray(np.hstack(xs))
+            v = np.asfortranarray(np.hstack(vs))
+            nps = [a.shape[1] for a in xs]
+            if not kw.get("bound_check"):
+                o_in, o_hid, o_am = oracle_fof(x, nps, kw["L_Box"], 0.2, 10, vp=v)
+                assert np.array_equal(inh, o_in.astype(bool)) and np.array_equal(hid, o_hid)
+                assert np.allclose(am, o_am, rtol=1e-9, atol=1e-12 * (np.abs(o_am).max() + 1e-300))
+                continue
+            s15 = 15.0 * 0.67 * kw["ng_global"] / 300.0
+            r = oracle_fof_bound(x, nps, kw["L_Box"], 0.2, 10, v, s15=s15)
+            assert r["ierr"] == 0
+            ref = r["in_halo"].astype(bool)
+            mism = int(np.sum(inh != ref))
+            print(f"bound job of {x.shape[1]} particles: {mism} differ from the oracle", flush=True)
+            assert mism <= 1e-4 * x.shape[1]
+            if np.all(r["gid"] == 0):
+                assert np.array_equal(inh, ref) and np.array_equal(hid, r["halo_id"])
+    finally:
+        oracle_set_link_threads(0)
