@@ -52,6 +52,33 @@ int32_t pyfof_fof(const double* xp, const double* vp, const double* mp, const in
  * exits, while the HIP runtime is still up. */
 void pyfof_release(void);
 
+/*
+ * Allocation-free variant of pyfof_fof, for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It has no counterpart in the reference. Same
+ * arguments and ierr codes as pyfof_fof, with every array (xp, vp, mp,
+ * in_halo, halo_id, ang_mom) in HBM of the calling thread's current device;
+ * np_halo stays in host memory. Plain FoF only: bound_check != 0 returns
+ * ierr 1. At most PYFOF_NOALLOC_MAX_N particles (ierr 21 beyond); every
+ * buffer of the call comes from one static scratch array of
+ * PYFOF_NOALLOC_SCRATCH_MIB MiB per device, held until the process ends, and
+ * an input whose cell grid does not fit it returns ierr 3. The call makes no
+ * hipMalloc or hipFree (the first call on a device looks up the scratch
+ * address and creates the stream it runs on). Calls on one device share that
+ * scratch, which keeps the last call's contents: the library serialises them
+ * (a call returns with its device work finished), so overlapping calls wait
+ * instead of racing on it. Other entry points never use the scratch.
+ */
+#define PYFOF_NOALLOC_MAX_N 524288
+#define PYFOF_NOALLOC_SCRATCH_MIB 128
+int32_t pyfof_fof_noalloc(const double* xp, const double* vp, const double* mp,
+                          const int32_t* np_halo, int32_t nhalo, double L_Box,
+                          double linking_length, int32_t np_halo_min, int32_t bound_check,
+                          int32_t verbose, double softening, double a_val, double b_val,
+                          double s_15_val, int32_t n_threads, int32_t total_np, int64_t ng_val,
+                          double ng_global_val, int32_t* in_halo, int32_t* halo_id,
+                          double* ang_mom);
+
 /* Context API: explicit device, own stream and reusable HBM workspace. */
 pyfof_ctx* pyfof_ctx_create(int32_t device);
 void pyfof_ctx_destroy(pyfof_ctx* ctx);

@@ -28,7 +28,7 @@ EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",
     "pyfof_catalog_size", "pyfof_catalog_copy", "pyfof_ctx_timings", "pyfof_ctx_stats",
     "pyfof_ctx_stream", "pyfof_ctx_set_option", "pyfof_ctx_potential",
-    "pyfof_last_error", "pyfof_version", "pyfof_release",
+    "pyfof_last_error", "pyfof_version", "pyfof_release", "pyfof_fof_noalloc",
     # request server (py_fof_amd.serve)
     "pyfof_server_create", "pyfof_server_destroy", "pyfof_job_footprint", "pyfof_submit",
     "pyfof_poll", "pyfof_wait", "pyfof_server_status", "pyfof_server_last_error",

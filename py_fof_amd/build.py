@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpyfof_hip.so")
-SOURCES = ["prims.hip", "fof.hip", "fft.hip", "bound.hip", "serve.hip"]
+SOURCES = ["prims.hip", "fof.hip", "fft.hip", "bound.hip", "serve.hip", "noalloc.hip"]
 ARCH = os.environ.get("PYFOF_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: the FoF predicate and the bookkeeping sums must round

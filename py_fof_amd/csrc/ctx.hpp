@@ -19,11 +19,46 @@ void fof_init_device_tables();
 // PYFOF_OPT_POISON_ALLOC: 0, or 1 + the byte every new allocation is filled with
 inline std::atomic<int> g_poison_alloc{0};
 
+// A bump allocator over caller-provided device memory. While a thread has
+// one installed (ArenaScope), every DevBuf::get on that thread carves from it
+// instead of calling hipMalloc; a buffer asked for again within the same
+// epoch (one call) keeps its memory. Carved buffers are never freed.
+struct Arena {
+    unsigned char* base = nullptr;
+    size_t size = 0, off = 0;
+    uint64_t epoch = 0;
+    void reset() { off = 0; ++epoch; }
+};
+inline thread_local Arena* tl_arena = nullptr;
+
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { a->reset(); tl_arena = a; }
+    ~ArenaScope() { tl_arena = nullptr; }
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    uint64_t epoch = 0;    // arena epoch of a carved buffer
+    bool carved = false;   // p points into an arena, not a hipMalloc block
+    void* carve(Arena& a, size_t bytes) {
+        if (carved && epoch == a.epoch && cap >= bytes) return p;
+        const size_t at = (a.off + 255) & ~(size_t)255;
+        if (at > a.size || bytes > a.size - at)
+            throw HipError(hipErrorOutOfMemory,
+                           ("the input needs more than the " + std::to_string(a.size >> 20) +
+                            " MiB of static scratch of the allocation-free entry point").c_str());
+        a.off = at + bytes;
+        p = a.base + at;
+        cap = bytes;
+        epoch = a.epoch;
+        carved = true;
+        return p;
+    }
     void* get(size_t bytes) {
         if (bytes == 0) bytes = 16;
+        if (tl_arena) return carve(*tl_arena, bytes);
+        if (carved) { p = nullptr; cap = 0; carved = false; }
         if (bytes > cap) {
             if (p) PYFOF_HIP(hipFree(p));
             p = nullptr;
@@ -48,9 +83,10 @@ struct DevBuf {
     template <class T>
     T* as(size_t count) { return static_cast<T*>(get(count * sizeof(T))); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !carved) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        carved = false;
     }
 };
 
