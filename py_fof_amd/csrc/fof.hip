@@ -89,32 +89,45 @@ __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
 }
 
 // reference cell id per particle (fof_fortran.f90:580-585, 0-based global id)
+// and, in input order, its x_crs = real(x_rel, 4) (fof_fortran.f90:608) with
+// the region in w: the one coalesced read of xp the binning makes after the
+// bounding box; k_gather then moves 16 B per particle instead of re-reading
+// the 24 B of xp at random.
 __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
                                                const Chunk* __restrict__ chunks,
                                                const RegionDev* __restrict__ reg,
                                                uint32_t* __restrict__ keys,
-                                               uint32_t* __restrict__ vals, double L, double Lh) {
+                                               uint32_t* __restrict__ vals,
+                                               float4* __restrict__ xr, double L, double Lh) {
     const Chunk ch = chunks[blockIdx.x];
     const RegionDev& g = reg[ch.region];
     const uint32_t off = g.cell_off;
+    const size_t a = (size_t)g.start;
+    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
+    const float wr = __int_as_float(ch.region);
     if (g.skip) {
         for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
             keys[i] = off;
             vals[i] = (uint32_t)i;
+            xr[i] = make_float4((float)rel_coord(xp[3 * (size_t)i], xa0, L, Lh),
+                                (float)rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh),
+                                (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), wr);
         }
         return;
     }
-    const size_t a = (size_t)g.start;
-    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
     const double o0 = g.origin[0], o1 = g.origin[1], o2 = g.origin[2];
     const double c0 = g.cell[0], c1 = g.cell[1], c2 = g.cell[2];
     const int32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
-        const int32_t ix = ref_cell_coord(rel_coord(xp[3 * (size_t)i], xa0, L, Lh), o0, c0, n0);
-        const int32_t iy = ref_cell_coord(rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh), o1, c1, n1);
-        const int32_t iz = ref_cell_coord(rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), o2, c2, n2);
+        const double v0 = rel_coord(xp[3 * (size_t)i], xa0, L, Lh);
+        const double v1 = rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh);
+        const double v2 = rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh);
+        const int32_t ix = ref_cell_coord(v0, o0, c0, n0);
+        const int32_t iy = ref_cell_coord(v1, o1, c1, n1);
+        const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
         keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
         vals[i] = (uint32_t)i;
+        xr[i] = make_float4((float)v0, (float)v1, (float)v2, wr);
     }
 }
 
@@ -126,25 +139,14 @@ __global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint3
     if (p + 1 == n || skeys[p + 1] != k) ends[k] = p + 1;
 }
 
-// x_crs = real(x_rel, 4) in CRS order (fof_fortran.f90:608); w holds the region.
-__global__ void __launch_bounds__(kBlk) k_gather(const double* __restrict__ xp,
+// x_crs in CRS order (fof_fortran.f90:608): the input-order copy permuted.
+__global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
                                                  const uint32_t* __restrict__ perm,
-                                                 const uint32_t* __restrict__ skeys,
-                                                 const uint32_t* __restrict__ cell_off,
-                                                 const RegionDev* __restrict__ reg,
                                                  float4* __restrict__ xc, int32_t* __restrict__ parent,
-                                                 uint32_t n, int32_t nreg, double L, double Lh) {
+                                                 uint32_t n) {
     const uint32_t p = blockIdx.x * kBlk + threadIdx.x;
     if (p >= n) return;
-    const uint32_t r = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, skeys[p]);
-    const size_t i = perm[p];
-    const size_t a = (size_t)reg[r].start;
-    float4 v;
-    v.x = (float)rel_coord(xp[3 * i], xp[3 * a], L, Lh);
-    v.y = (float)rel_coord(xp[3 * i + 1], xp[3 * a + 1], L, Lh);
-    v.z = (float)rel_coord(xp[3 * i + 2], xp[3 * a + 2], L, Lh);
-    v.w = __int_as_float((int)r);
-    xc[p] = v;
+    xc[p] = xr[perm[p]];
     parent[p] = (int32_t)p;
 }
 
@@ -1687,7 +1689,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* k1 = C.keys1.as<uint32_t>(N);
         uint32_t* v0 = C.vals0.as<uint32_t>(N);
         uint32_t* v1 = C.vals1.as<uint32_t>(N);
-        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, A.L, P.Lh);
+        float4* xr = C.buf("xrel").as<float4>(N);
+        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, xr, A.L, P.Lh);
         PYFOF_LAUNCH_CHECK();
         const size_t tmp_bytes = std::max(radix_temp_bytes(N), scan_temp_bytes(ncell + 1));
         void* tmp = C.temp.get(std::max(tmp_bytes, scan_temp_bytes(N) + 64));
@@ -1702,7 +1705,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         inclusive_max_scan_u32(cell_ptr + 1, cell_ptr + 1, ncell, tmp, s);
         float4* xc = C.xc.as<float4>(N);
         int32_t* parent = C.parent.as<int32_t>(N);
-        k_gather<<<nblk(N), kBlk, 0, s>>>(xp, perm, skeys, d_coff, d_reg, xc, parent, N, H, A.L, P.Lh);
+        k_gather<<<nblk(N), kBlk, 0, s>>>(xr, perm, xc, parent, N);
         PYFOF_LAUNCH_CHECK();
         // dense cells and their sub-cell lists
         int32_t* aux = C.aux.as<int32_t>(ncell);

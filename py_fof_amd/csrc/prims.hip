@@ -140,7 +140,9 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
 // the wave's earlier items (the rows need no barrier: a wave's lanes run in
 // lockstep). One barrier then turns the four rows into prefixes in wave
 // order, so the rank of an item follows (wave, round, lane) = item order:
-// the sort is stable.
+// the sort is stable. The ranked tile is then laid out in LDS in digit order
+// and written from there, so that neighbouring lanes store neighbouring
+// addresses (runs of one digit) instead of scattering lane by lane.
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ vals,
                                                           uint32_t* __restrict__ okeys,
@@ -148,14 +150,20 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
                                                           int shift,
                                                           const uint32_t* __restrict__ hist,
                                                           uint32_t ntiles) {
+    static_assert(kBlock == kRadix, "one thread per digit in the digit scans");
     constexpr int kWaves = kBlock / kWave;
     constexpr int kRounds = kTile / kBlock;  // rounds of 64 items per wave
-    __shared__ uint32_t wc[kWaves][kRadix];
+    __shared__ uint32_t wc[kWaves][kRadix];  // per-wave counts, then per-wave offsets in a digit
+    __shared__ uint32_t lstart[kRadix];      // first tile-local position of each digit
+    __shared__ uint32_t gstart[kRadix];      // first output position of each digit of this tile
+    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t sk[kTile], sv[kTile];
     const int t = threadIdx.x, w = t / kWave;
     uint32_t* wf = &wc[0][0];
     for (int i = t; i < kWaves * kRadix; i += kBlock) wf[i] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kRounds * kWave);
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
+    const uint64_t base = tile0 + (uint64_t)w * (kRounds * kWave);
     const uint64_t lt = lanemask_lt();
     uint32_t kk[kRounds], vv[kRounds], rk[kRounds];
 #pragma unroll
@@ -179,23 +187,44 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
         rk[j] = before + r;
     }
     __syncthreads();
-    for (int dd = t; dd < kRadix; dd += kBlock) {
-        uint32_t acc = hist[(uint64_t)dd * ntiles + blockIdx.x];
-#pragma unroll
-        for (int w2 = 0; w2 < kWaves; ++w2) {
-            const uint32_t c = wc[w2][dd];
-            wc[w2][dd] = acc;
-            acc += c;
-        }
+    {
+        const int d = t;
+        const uint32_t c0 = wc[0][d], c1 = wc[1][d], c2 = wc[2][d], c3 = wc[3][d];
+        wc[0][d] = 0;
+        wc[1][d] = c0;
+        wc[2][d] = c0 + c1;
+        wc[3][d] = c0 + c1 + c2;
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        const uint32_t inc = wave_inclusive_scan<OpSum>(tot);
+        if (__lane_id() == kWave - 1) wsum[w] = inc;
+        gstart[d] = hist[(uint64_t)d * ntiles + blockIdx.x];
+        __syncthreads();
+        uint32_t off = 0;
+        for (int k = 0; k < w; ++k) off += wsum[k];
+        lstart[d] = off + inc - tot;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRounds; ++j) {
         const uint64_t idx = base + (uint64_t)j * kWave + __lane_id();
         if (idx >= n) continue;
-        const uint32_t dst = wc[w][(kk[j] >> shift) & (kRadix - 1)] + rk[j];
-        okeys[dst] = kk[j];
-        ovals[dst] = vv[j];
+        const uint32_t d = (kk[j] >> shift) & (kRadix - 1);
+        const uint32_t lp = lstart[d] + wc[w][d] + rk[j];
+        sk[lp] = kk[j];
+        sv[lp] = vv[j];
+    }
+    __syncthreads();
+    const uint64_t rem = n - tile0;
+    const uint32_t cnt = rem < (uint64_t)kTile ? (uint32_t)rem : (uint32_t)kTile;
+#pragma unroll
+    for (int j = 0; j < kRounds; ++j) {
+        const uint32_t lp = (uint32_t)(j * kBlock + t);
+        if (lp >= cnt) continue;
+        const uint32_t k = sk[lp];
+        const uint32_t d = (k >> shift) & (kRadix - 1);
+        const uint32_t dst = gstart[d] + (lp - lstart[d]);
+        okeys[dst] = k;
+        ovals[dst] = sv[lp];
     }
 }
 
