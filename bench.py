@@ -33,10 +33,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FoF Mparticles/s; bound halos/s batched vs plain at stated err; p99 latency"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per call of the bbox + binning + linking kernels on the
-# 134,217,728-particle workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# scaled by a 1 GiB copy (profiles/fof_nfw134M_hbm_traffic.md)
-TRAFFIC_BYTES_134M = 75.01e9
+# measured HBM bytes per call of the bbox + binning + linking kernels on the
+# 134,217,728-particle workload: the total line of this rocprofv3 summary
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fof_nfw134M_hbm_traffic.md")
+# Oracle FoF on the full 134,217,728-particle set against its bounded
+# 4,194,304-particle sample, both on the same 8 threads of the build host
+# (tests/golden/make_c2_fixture.py, tests/golden/c2_nfw134M.json): 306.4 s
+# and 4.6 s, so the full set links at 0.48 times the sample's per-particle
+# rate (its clumps reach 2e6 members and denser cores).
+CPU_FULL_OVER_SAMPLE = (134_217_728 / 306.4) / (4_194_304 / 4.6)
 PHASES = ["bbox", "binning", "linking", "relabel", "unbinding", "catalog"]
 
 
@@ -77,20 +82,72 @@ def box_for(n):
     return 512.0 * (n / 134_217_728) ** (1.0 / 3.0)
 
 
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def traffic_from_profile(path=TRAFFIC_PROFILE):
+    """Bytes per call from the summary's '**X GB per call**' line, or None."""
+    import re
+    try:
+        with open(path) as f:
+            m = re.search(r"\*\*([0-9.]+) GB per call\*\*", f.read())
+        return float(m.group(1)) * 1e9 if m else None
+    except OSError:
+        return None
+
+
 def cpu_baseline(args):
-    from oracle import oracle_fof
+    """The oracle (C restatement of the reference) on a bounded sample of the
+    same generator, with the reference's thread rule for one region (2
+    linking threads whatever n_threads is, fof_fortran.f90:369-379) at
+    n_threads = the host's cores, and once more linking on every core."""
+    from oracle import oracle_fof, oracle_set_link_threads
     from py_fof_amd.synth import clumpy_nfw
+    model, cores = host_cpu()
     n = args.cpu_n
     L = box_for(n)
     xp = clumpy_nfw(n=n, L=L, seed=args.seed)
     t0 = time.perf_counter()
-    oracle_fof(xp, [n], L, 0.2, 10, n_threads=2)
+    oracle_fof(xp, [n], L, 0.2, 10, n_threads=cores)
     dt = time.perf_counter() - t0
+    oracle_set_link_threads(cores)
+    try:
+        t0 = time.perf_counter()
+        oracle_fof(xp, [n], L, 0.2, 10, n_threads=cores)
+        dt_all = time.perf_counter() - t0
+    finally:
+        oracle_set_link_threads(0)
+    rate = n / dt / 1e6
     return {
-        "value": round(n / dt / 1e6, 4), "unit": "Mparticles/s", "cores": 2, "kind": "port",
-        "sample": f"oracle FoF on the same generator at {n} particles, L={L:g} "
-                  f"(same density), seed {args.seed}, 2 linking threads as the reference's "
-                  f"rule for one region; {dt:.2f} s",
+        "value": round(rate, 4), "unit": "Mparticles/s", "cores": 2, "kind": "port",
+        "sample": f"oracle FoF on the same generator at {n} particles, L={L:g} (same density), "
+                  f"seed {args.seed}, n_threads={cores} (the host's cores), which the "
+                  f"reference's rule for one region turns into 2 linking threads; {dt:.2f} s",
+        "host": {"cpu": model, "cores_available": cores},
+        "all_cores": {"value": round(n / dt_all / 1e6, 4), "unit": "Mparticles/s",
+                      "cores": cores, "seconds": round(dt_all, 2),
+                      "note": "same sample linked on every core (not the reference's rule)"},
+        "full_size_estimate": {
+            "value": round(rate * CPU_FULL_OVER_SAMPLE, 4), "unit": "Mparticles/s",
+            "factor": round(CPU_FULL_OVER_SAMPLE, 3),
+            "note": "sample rate x the full-set / sample per-particle rate ratio of the oracle, "
+                    "both measured on 8 threads of the build host (306.4 s for 134,217,728 "
+                    "particles, 4.6 s for the 4,194,304-particle sample)"},
     }
 
 
@@ -189,7 +246,10 @@ def bench_bound(args, torch, _native, ctx):
 
     same = []
 
-    def timed(steps):
+    fft = {"ms": 0.0, "bytes": 0.0, "slots": 0, "calls": 0}
+
+    def timed(steps, time_fft=False):
+        ctx.set_option(_native.OPT_TIME_FFT, int(time_fft))
         step()  # warm-up of this form
         first = in_h.clone()
         torch.cuda.synchronize()
@@ -197,15 +257,23 @@ def bench_bound(args, torch, _native, ctx):
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
-            ph += np.array(ctx.timings()[:6])
+            tm = ctx.timings()
+            ph += np.array(tm[:6])
+            if time_fft:
+                sf = ctx.stats()
+                fft["ms"] += tm[6]
+                fft["bytes"] += sf["fft_bytes"]
+                fft["slots"] += sf["fft_slots"]
+                fft["calls"] += 1
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         same.append(bool(torch.equal(first, in_h)))
+        ctx.set_option(_native.OPT_TIME_FFT, 0)
         return dt, ph / steps, ctx.stats()
 
     # the reference's form: crossover 1000, FP64 direct sums, batched
     form(1000, False, True)
-    dt, phases, st = timed(args.bound_steps)
+    dt, phases, st = timed(args.bound_steps, time_fft=True)
     ref_in = in_h.clone()
     log(args, f"unbinding (reference form): {dt * 1e3:.1f} ms/call")
     ncand = st["n_ok"] + st["n_too_few"] + st["n_max_iter"]
@@ -226,6 +294,19 @@ def bench_bound(args, torch, _native, ctx):
         "pm_passes": st["passes"],
         "form": "reference: direct sum (<= 1000 members) FP64, PM f32, batched",
     }
+    if fft["calls"]:
+        ach = fft["bytes"] / (fft["ms"] * 1e-3) / 1e9
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_fft_conv: the FFT convolutions of the mesh potentials, the largest "
+                      "share of a call; algorithmic bytes 48 n^3 per slot convolution (forward "
+                      "and inverse transforms, three axis passes reading and writing the n^3 "
+                      "complex grid), timed with HIP events around each convolution launch",
+            "ms_per_call": round(fft["ms"] / fft["calls"], 3),
+            "slot_convolutions_per_call": fft["slots"] // fft["calls"],
+            "GB_per_call": round(fft["bytes"] / fft["calls"] / 1e9, 3),
+        }
     if not args.no_forms:
         from py_fof_amd.tune import tune_direct_max
         # plain per-halo FP64 form (one candidate at a time)
@@ -287,14 +368,43 @@ def bench_bound(args, torch, _native, ctx):
                              np.asfortranarray(vp[:, cols]), app=app, s15=s15)
         dtc = time.perf_counter() - t0
         nvalid = int(len(np.unique(r["halo_id"][r["halo_id"] > 0])))
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        model, cores = host_cpu()
+        # the GPU on exactly the same sample (reference form), device-resident
+        xs_d = torch.from_numpy(np.ascontiguousarray(xp[:, cols].T)).to(dev)
+        vs_d = torch.from_numpy(np.ascontiguousarray(vp[:, cols].T)).to(dev)
+        in_s = torch.empty(nk, dtype=torch.int32, device=dev)
+        id_s = torch.empty(nk, dtype=torch.int32, device=dev)
+        nps = np.ascontiguousarray(np_halo[pick], dtype=np.int32)
+        form(1000, False, True)
+
+        def sample_step():
+            err = ctx.fof(xs_d, vs_d, None, nps, L, b, 10, 1, 0, app, 1.0, 0.0, s15, 0, nk,
+                          256, 512.0, in_s, id_s, None, None, _native.MEM_DEVICE)
+            if err:
+                raise RuntimeError(f"pyfof ierr={err}: {_native.last_error()}")
+
+        sample_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sample_step()
+        torch.cuda.synchronize()
+        dtg = (time.perf_counter() - t0) / 3
+        gpu_in = in_s.cpu().numpy()
+        ndiff = int(np.sum(gpu_in != r["in_halo"]))
         out["cpu_baseline"] = {
-            "value": round(nvalid / dtc, 2), "unit": "bound halos/s", "cores": threads,
+            "value": round(nvalid / dtc, 2), "unit": "bound halos/s", "cores": cores,
             "kind": "port",
             "sample": f"oracle FoF + unbinding on the first {k} halos of the same set with "
-                      f"fewer than 20,000 members ({nk} particles, {nvalid} bound halos); FoF "
-                      f"linking and direct sums on 1 thread, FFTs and Green functions on "
-                      f"{threads}; {dtc:.2f} s",
+                      f"fewer than 20,000 members ({nk} particles, {nvalid} bound halos); "
+                      f"FoF linking, direct sums and the removal loops on 1 thread, the FFTs "
+                      f"and Green functions on up to {cores}; {dtc:.2f} s",
+            "host": {"cpu": model, "cores_available": cores},
+            "gpu_same_sample": {"bound_halos_per_s": round(nvalid / dtg, 1),
+                                "ms_per_call": round(dtg * 1e3, 3),
+                                "speedup_vs_cpu": round(dtc / dtg, 1)},
+            "bound_status_vs_oracle": {"particles": ndiff, "of": nk,
+                                       "fraction": float(f"{ndiff / max(nk, 1):.3e}")},
         }
     return out
 
@@ -516,7 +626,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": TRAFFIC_BYTES_134M if n == 134_217_728 else None,
+            "traffic": traffic_from_profile() if n == 134_217_728 else None,
             "kernel": "binning + linking (k_bbox..k_dense_cross), algorithmic bytes N*64 + cells*12; "
                       "traffic: measured HBM bytes of the same kernels per call",
         },

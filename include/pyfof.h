@@ -110,6 +110,10 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
  *     one launch per step; 0 unbinds one candidate at a time (plain form).
  *   PYFOF_OPT_RECORD_PHI (default 0): keep the first-pass potential of every
  *     candidate member for pyfof_ctx_potential.
+ *   PYFOF_OPT_TIME_FFT (default 0): HIP events around every FFT
+ *     convolution launch of the unbinding; pyfof_ctx_timings [6] and
+ *     pyfof_ctx_stats [16..17] report their device time, slot count and
+ *     algorithmic bytes.
  *   PYFOF_OPT_POISON_ALLOC (default 0; process-wide, not per context): a
  *     debugging aid. 0 = off; 1 + b fills every device allocation made from
  *     then on with the byte b (0..255), so that a read of memory no kernel
@@ -121,6 +125,7 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
 #define PYFOF_OPT_BATCHED 4
 #define PYFOF_OPT_RECORD_PHI 5
 #define PYFOF_OPT_POISON_ALLOC 6
+#define PYFOF_OPT_TIME_FFT 7
 int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value);
 
 /* First-pass potentials recorded by the last bound_check call on ctx (with
@@ -142,8 +147,9 @@ int32_t pyfof_catalog_copy(pyfof_ctx* ctx, int64_t* offsets, int32_t* indices, i
 
 /* Device time in milliseconds of the phases of the last call on ctx
  * (0: bbox+geometry, 1: binning, 2: linking, 3: relabel/outputs,
- * 4: unbinding, 5: catalog+angular momentum), measured with HIP events on
- * the context's stream. Returns the number of phases written. */
+ * 4: unbinding, 5: catalog+angular momentum; 6: the unbinding's FFT
+ * convolutions, with PYFOF_OPT_TIME_FFT), measured with HIP events on the
+ * context's stream. Returns the number of values written. */
 int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
 
 /* Sizes of the last call on ctx: [0] particles, [1] reference cells,
@@ -151,7 +157,9 @@ int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n);
  * those halos; with bound_check: [5] valid bound halos, [6..8] candidates
  * ending OK / TOO_FEW / MAX_ITER, [9] direct-summation candidates,
  * [10..13] candidates on the 48/64/128/256 grids, [14] potential passes,
- * [15] direct-summation candidates above 1000 members.
+ * [15] direct-summation candidates above 1000 members; with
+ * PYFOF_OPT_TIME_FFT [16] FFT convolutions (slots) and [17] their
+ * algorithmic bytes (48 n^3 each).
  * Returns the number of values written. */
 int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n);
 
@@ -207,7 +215,10 @@ int32_t pyfof_poll(pyfof_server* server, int64_t id, int32_t* ierr, char* msg, i
 int32_t pyfof_wait(pyfof_server* server, int64_t id, int32_t* ierr, char* msg, int32_t msg_cap,
                    double* times);
 
-/* [0] queued jobs, [1] running jobs, [2] admitted footprint, [3] budget. */
+/* [0] queued jobs, [1] running jobs, [2] footprint of the running jobs,
+ * [3] budget, [4] bytes the workers' contexts hold between jobs, [5] bytes
+ * claimed in all (per worker the larger of [4]'s share and its job's
+ * footprint; admission keeps it within the budget). */
 int32_t pyfof_server_status(pyfof_server* server, int64_t* out, int32_t n);
 const char* pyfof_server_last_error(void);
 

@@ -23,6 +23,7 @@ OPT_DIRECT_FP32 = 3
 OPT_BATCHED = 4
 OPT_RECORD_PHI = 5
 OPT_POISON_ALLOC = 6  # process-wide: 0 off, 1 + b fills new device allocations with byte b
+OPT_TIME_FFT = 7
 
 EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",
@@ -216,11 +217,11 @@ class Context:
 
     def stats(self):
         """dict of the last call's sizes (particles, cells, dense cells, halos, members)."""
-        buf = (ctypes.c_int64 * 16)()
-        n = self.lib.pyfof_ctx_stats(self.h, buf, 16)
+        buf = (ctypes.c_int64 * 18)()
+        n = self.lib.pyfof_ctx_stats(self.h, buf, 18)
         keys = ["n", "ncell", "ndense", "nhalo", "nmem", "nvalid", "n_ok", "n_too_few",
                 "n_max_iter", "n_direct", "n_pm48", "n_pm64", "n_pm128", "n_pm256", "passes",
-                "n_direct_large"]
+                "n_direct_large", "fft_slots", "fft_bytes"]
         return {keys[i]: int(buf[i]) for i in range(n)}
 
     def potential(self):

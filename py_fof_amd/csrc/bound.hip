@@ -2664,8 +2664,10 @@ static int32_t run_bound_impl(Ctx& C, const BoundArgs& A, uint8_t* keep_out, Bou
                                                                   active, fscale, g, clist + cls_first[g],
                                                                   cls_count[g], acc[g]);
                     PYFOF_LAUNCH_CHECK();
+                    C.fft_begin();
                     fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
                                Gfull[g], s);
+                    C.fft_end(n, (int)sl.size());
                     k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
                                                       mkeep, grid[g], g, M, A.a, phi_c1);
                     PYFOF_LAUNCH_CHECK();
@@ -2704,8 +2706,10 @@ static int32_t run_bound_impl(Ctx& C, const BoundArgs& A, uint8_t* keep_out, Bou
                                                                        acc_in[c]);
                             PYFOF_LAUNCH_CHECK();
                         }
+                        C.fft_begin();
                         fft_conv3d(*plan_in[c], grid_in[c], acc_in[c], d_tl_ls, (int32_t)ls.size(), d_tl_sh,
                                    fscale, Gin[c], s);
+                        C.fft_end(n_in, (int)ls.size());
                         k_tl_interp<<<blocks(M), kB, 0, s>>>(tgt_slot, tgt_first, tgt_wt, self_in, d_tlslots, ratio,
                                                              cs, cs + tl_chunk[c], keep, active, qw, grid_in[c], M,
                                                              A.a, phi_in);

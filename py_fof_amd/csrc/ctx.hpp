@@ -133,6 +133,41 @@ struct Ctx {
     // contexts that share the GPU with others (the request server's workers),
     // where the co-residency those launches need is not assured
     bool solo = true;
+    // PYFOF_OPT_TIME_FFT: HIP events around every FFT convolution launch of
+    // the unbinding (pairs, kept between calls), their summed device time and
+    // the convolutions' algorithmic bytes (48 n^3 per slot: forward and
+    // inverse transforms, three axis passes, each reading and writing 8 B per
+    // node of the n^3 complex grid)
+    bool time_fft = false;
+    std::vector<hipEvent_t> fft_ev;
+    size_t fft_ev_used = 0;
+    double fft_ms = 0.0, fft_bytes = 0.0;
+    int64_t fft_slots = 0;
+    void fft_begin() {
+        if (!time_fft) return;
+        while (fft_ev.size() < fft_ev_used + 2) {
+            hipEvent_t e = nullptr;
+            PYFOF_HIP(hipEventCreate(&e));
+            fft_ev.push_back(e);
+        }
+        PYFOF_HIP(hipEventRecord(fft_ev[fft_ev_used], stream));
+    }
+    void fft_end(int n, int nslots) {
+        if (!time_fft) return;
+        PYFOF_HIP(hipEventRecord(fft_ev[fft_ev_used + 1], stream));
+        fft_ev_used += 2;
+        fft_bytes += 48.0 * (double)n * (double)n * (double)n * (double)nslots;
+        fft_slots += nslots;
+    }
+    // after the stream has drained
+    void fft_collect() {
+        fft_ms = 0.0;
+        for (size_t k = 0; k + 1 < fft_ev_used; k += 2) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, fft_ev[k], fft_ev[k + 1]) == hipSuccess) fft_ms += t;
+        }
+    }
+    void fft_reset() { fft_ev_used = 0; fft_ms = fft_bytes = 0.0; fft_slots = 0; }
     uint64_t rec_n = 0;  // recorded first-pass potentials of the last call
     DevBuf rec_phi, rec_idx;
     BoundStats last_bound;  // build the fof_with_info catalog on every call
@@ -154,22 +189,39 @@ struct Ctx {
     ~Ctx() {
         (void)hipSetDevice(device);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto& e : fft_ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+        release_workspace();
+    }
+    template <class F>
+    void for_each_buf(F f) {
         for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
                           &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
                           &sub_ptr, &sub_list, &subs, &ncomp, &croot, &needs_sub, &temp, &out_in, &out_id, &out_root, &anchor,
                           &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
-                          &am_h, &am, &pin})
-            b->release();
-        for (auto& kv : named) kv.second.release();
-        rec_phi.release();
-        rec_idx.release();
+                          &am_h, &am, &pin, &rec_phi, &rec_idx})
+            f(*b);
+        for (auto& kv : named) f(kv.second);
         for (auto& kv : greens) {
-            kv.second->G.release();
-            kv.second->tw.release();
-            kv.second->self.release();
-            delete kv.second;
+            f(kv.second->G);
+            f(kv.second->tw);
+            f(kv.second->self);
         }
+    }
+    // Device bytes this context holds between calls (its workspace and the
+    // cached Green functions).
+    size_t resident_bytes() {
+        size_t t = 0;
+        for_each_buf([&](DevBuf& b) { if (b.p && !b.carved) t += b.cap; });
+        return t;
+    }
+    // Frees the workspace and the caches; the next call allocates again. The
+    // context's stream must be idle.
+    void release_workspace() {
+        for_each_buf([](DevBuf& b) { b.release(); });
+        for (auto& kv : greens) delete kv.second;
+        greens.clear();
+        named.clear();
     }
     void mark(int e) {
         PYFOF_HIP(hipEventRecord(ev[e], stream));

@@ -1538,6 +1538,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
     C.cat_hoff = nullptr;
     C.cat_members = nullptr;
     C.last_bound = BoundStats();
+    C.fft_reset();
     C.mark(kEvStart);
 
     // inputs in HBM
@@ -1914,6 +1915,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                 PYFOF_HIP(hipMemcpyAsync(A.root_out, o_root, 4 * (size_t)N, hipMemcpyDeviceToHost, s));
         }
         PYFOF_HIP(hipStreamSynchronize(s));
+        if (C.time_fft) C.fft_collect();
         if (A.verbose >= 2)
             printf("[pyfof] N=%u regions=%d cells=%llu dense=%u halos=%u\n", N, H,
                    (unsigned long long)ncell, ndense, nh);
@@ -2150,6 +2152,7 @@ int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
         case PYFOF_OPT_DIRECT_FP32: C.direct_f32 = value != 0; return IERR_OK;
         case PYFOF_OPT_BATCHED: C.batched = value != 0; return IERR_OK;
         case PYFOF_OPT_RECORD_PHI: C.record_phi = value != 0; return IERR_OK;
+        case PYFOF_OPT_TIME_FFT: C.time_fft = value != 0; return IERR_OK;
         case PYFOF_OPT_POISON_ALLOC:
             if (value < 0 || value > 256) {
                 set_error("PYFOF_OPT_POISON_ALLOC takes 0 or 1 + a byte value (at most 256)");
@@ -2190,14 +2193,15 @@ int32_t pyfof_ctx_stats(pyfof_ctx* ctx, int64_t* out, int32_t n) {
     if (!ctx) return 0;
     const Ctx& C = *ctx->c;
     const BoundStats& b = C.last_bound;
-    const int64_t v[16] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
+    const int64_t v[18] = {(int64_t)C.last_n, (int64_t)C.last_ncell, (int64_t)C.last_ndense,
                            (int64_t)C.cat_nh, (int64_t)C.cat_nmem, (int64_t)b.nvalid,
                            (int64_t)b.n_ok, (int64_t)b.n_too_few, (int64_t)b.n_max_iter,
                            (int64_t)b.n_direct, (int64_t)b.n_pm[0], (int64_t)b.n_pm[1],
                            (int64_t)b.n_pm[2], (int64_t)b.n_pm[3], (int64_t)b.passes,
-                           (int64_t)b.n_direct_large};
+                           (int64_t)b.n_direct_large, (int64_t)C.fft_slots,
+                           (int64_t)C.fft_bytes};
     int32_t w = 0;
-    for (; w < n && w < 16; ++w) out[w] = v[w];
+    for (; w < n && w < 18; ++w) out[w] = v[w];
     return w;
 }
 
@@ -2211,6 +2215,7 @@ int32_t pyfof_ctx_timings(pyfof_ctx* ctx, double* ms, int32_t n) {
         float t = 0.f;
         if (hipEventElapsedTime(&t, C.ev[e - 1], C.ev[e]) == hipSuccess) ms[w] = t;
     }
+    if (w < n) ms[w++] = C.fft_ms;
     return w;
 }
 

@@ -11,6 +11,7 @@
 // reentrant (bound_fortran.f90:1524, 1564-1568).
 #include <chrono>
 #include <condition_variable>
+#include <algorithm>
 #include <deque>
 #include <memory>
 #include <string.h>
@@ -40,30 +41,42 @@ struct Job {
 
 }  // namespace
 
-// Estimated device bytes of one job: the FoF workspace per particle
-// (sort keys and values with their ping-pong copies 16 B, CRS float4 16 B,
-// DSU and labels 20 B, outputs and catalog 24 B, cells about 8 B), the
-// staging of host arrays, and for the unbinding the per-member state
-// (about 120 B) plus one 256^3 grid pair (2 x 128 MiB) for the largest halo.
+// Estimated device bytes of one job: the FoF workspace per particle (sort
+// keys and values with their ping-pong copies 16 B, the input-order and the
+// CRS float4 copies 32 B, DSU, labels and numbering 20 B, dense-cell tables
+// up to 20 B, outputs, anchors and catalog 40 B, cells about 8 B, and the
+// 1/8 growth slack of the workspace buffers), the staging of host arrays,
+// and for the unbinding the per-member state (about 120 B) plus one 256^3
+// grid pair (2 x 128 MiB) for the largest halo.
 int64_t job_footprint(const CallArgs& a) {
     const int64_t n = a.total_np;
-    int64_t b = n * 88 + (16ll << 20);
+    int64_t b = n * 160 + (16ll << 20);
     if (a.mem_kind == PYFOF_MEM_HOST) b += n * (24 + (a.vp ? 24 : 0) + (a.mp ? 8 : 0) + 12);
     if (a.bound_check) b += n * 120 + (256ll << 20);
     return b;
 }
 
+// Admission counts what every worker holds, not only the jobs in flight: a
+// worker's claim on HBM is the larger of the workspace its context keeps
+// between jobs and the footprint of the job it runs. The head of the queue
+// starts on an idle worker k when the other workers' claims plus the larger
+// of k's resident workspace and the job's footprint fit the budget. When
+// only resident workspaces stand in the way, the idle workers holding them
+// free them (k its own at once; the others when asked), and a worker whose
+// workspace grew past what the budget allows after a job frees it as well.
 struct Server {
     int device = 0;
     int64_t budget = 0;
-    int64_t inflight = 0;
     int64_t next_id = 1;
     bool closing = false;
+    bool trim = false;  // idle workers free their workspaces
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Job>> queue;
     std::unordered_map<int64_t, std::shared_ptr<Job>> jobs;
     std::vector<Ctx*> ctxs;
+    std::vector<int64_t> resident;  // bytes each worker's context holds
+    std::vector<int64_t> running;   // footprint of each worker's job, 0 when idle
     std::vector<std::thread> workers;
 
     Server(int dev, int64_t hbm_budget, int slots) : device(dev), budget(hbm_budget) {
@@ -72,6 +85,8 @@ struct Server {
             ctxs.push_back(new Ctx(dev));
             ctxs.back()->solo = slots == 1;
         }
+        resident.assign(slots, 0);
+        running.assign(slots, 0);
         for (int k = 0; k < slots; ++k) workers.emplace_back([this, k] { run(k); });
     }
 
@@ -85,31 +100,73 @@ struct Server {
         for (Ctx* c : ctxs) delete c;
     }
 
-    void run(int k) {
+    int64_t claim(size_t w) const { return std::max(resident[w], running[w]); }
+    // claims of the workers other than k; busy_only leaves out idle ones
+    int64_t others(size_t k, bool busy_only) const {
+        int64_t t = 0;
+        for (size_t w = 0; w < resident.size(); ++w)
+            if (w != k && (!busy_only || running[w] > 0)) t += claim(w);
+        return t;
+    }
+    // bytes held or claimed on the device by all workers (under mu)
+    int64_t claimed() const { return others(resident.size(), false); }
+
+    enum Step { kWait, kTake, kFreeOwn, kExit };
+    Step next_step(size_t k) {
+        if (queue.empty()) return closing ? kExit : (trim && resident[k] > 0 ? kFreeOwn : kWait);
+        const int64_t f = queue.front()->footprint;
+        if (others(k, false) + std::max(resident[k], f) <= budget) return kTake;
+        if (resident[k] > 0 && others(k, false) + f <= budget) return kFreeOwn;
+        if (others(k, true) + f <= budget) {
+            if (!trim) {
+                trim = true;  // idle workers' workspaces are in the way: wake them
+                cv_work.notify_all();
+            }
+            if (resident[k] > 0) return kFreeOwn;
+        }
+        return kWait;
+    }
+
+    void free_workspace(size_t k, std::unique_lock<std::mutex>& lk) {
+        lk.unlock();
+        ctxs[k]->release_workspace();  // the worker's stream is idle
+        lk.lock();
+        resident[k] = 0;
+        cv_work.notify_all();
+    }
+
+    void run(size_t k) {
         (void)hipSetDevice(device);
         for (;;) {
             std::shared_ptr<Job> j;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] {
-                    if (queue.empty()) return closing;
-                    return inflight + queue.front()->footprint <= budget;
-                });
-                if (queue.empty()) return;  // closing and drained
+                Step st;
+                while ((st = next_step(k)) == kWait) cv_work.wait(lk);
+                if (st == kExit) return;  // closing and drained
+                if (st == kFreeOwn) {
+                    free_workspace(k, lk);
+                    continue;
+                }
                 j = queue.front();
                 queue.pop_front();
-                inflight += j->footprint;
+                trim = false;
+                running[k] = j->footprint;
                 j->t_start = Clock::now();
             }
             cv_work.notify_all();  // the next head may fit as well
             const int32_t e = fof_entry(ctxs[k], j->a, j->n_threads);
+            const int64_t held = (int64_t)ctxs[k]->resident_bytes();
             {
-                std::lock_guard<std::mutex> lk(mu);
+                std::unique_lock<std::mutex> lk(mu);
                 j->ierr = e;
                 j->msg = e ? last_error_message() : std::string();
                 j->t_end = Clock::now();
                 j->done = true;
-                inflight -= j->footprint;
+                running[k] = 0;
+                resident[k] = held;
+                // a workspace larger than the budget leaves room for
+                if (claimed() > budget) free_workspace(k, lk);
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -262,12 +319,16 @@ int32_t pyfof_server_status(pyfof_server* server, int64_t* out, int32_t n) {
     if (!server) return 0;
     Server* s = server->s;
     std::lock_guard<std::mutex> lk(s->mu);
-    int64_t running = 0;
-    for (auto& kv : s->jobs)
-        if (!kv.second->done && kv.second->t_start != Clock::time_point()) ++running;
-    const int64_t v[4] = {(int64_t)s->queue.size(), running, s->inflight, s->budget};
+    int64_t running = 0, inflight = 0, resident = 0;
+    for (size_t w = 0; w < s->running.size(); ++w) {
+        running += s->running[w] > 0;
+        inflight += s->running[w];
+        resident += s->resident[w];
+    }
+    const int64_t v[6] = {(int64_t)s->queue.size(), running, inflight, s->budget, resident,
+                          s->claimed()};
     int32_t w = 0;
-    for (; w < n && w < 4; ++w) out[w] = v[w];
+    for (; w < n && w < 6; ++w) out[w] = v[w];
     return w;
 }
 

@@ -157,9 +157,10 @@ class Server:
             pass
 
     def status(self):
-        buf = (ctypes.c_int64 * 4)()
-        n = self._lib.pyfof_server_status(self._h, buf, 4)
-        keys = ["queued", "running", "admitted_bytes", "budget_bytes"]
+        buf = (ctypes.c_int64 * 6)()
+        n = self._lib.pyfof_server_status(self._h, buf, 6)
+        keys = ["queued", "running", "admitted_bytes", "budget_bytes", "resident_bytes",
+                "claimed_bytes"]
         return {keys[i]: int(buf[i]) for i in range(n)}
 
     def submit(self, xp_list, vp_list=None, **kwargs):

@@ -87,7 +87,7 @@ def test_budget_admission():
     probe = Server(slots=1, hbm_budget=1 << 40)
     fp = probe.footprint(xp, vp, **kw)
     probe.close()
-    assert fp > 60_000 * 88
+    assert fp > 60_000 * 160
     # a budget that admits one such job at a time still completes them all
     with Server(slots=4, hbm_budget=fp + 1) as srv:
         hs = [srv.submit(xp, vp, **kw) for _ in range(5)]
@@ -140,3 +140,43 @@ def test_served_jobs_match_the_oracle():
                 assert np.array_equal(inh, ref) and np.array_equal(hid, r["halo_id"])
     finally:
         oracle_set_link_threads(0)
+
+
+def test_resident_workspaces_stay_within_budget():
+    """Admission counts the workspace each worker keeps between jobs: after a
+    large job, small concurrent jobs and the large one again, the bytes the
+    server claims never exceed its budget, and neither does the device
+    memory it holds (hipMemGetInfo, through torch)."""
+    import torch
+    from py_fof_amd.serve import Server
+    big = _box_job(3_000_000, 41)
+    small = [_box_job(150_000, 42 + k) for k in range(6)]
+    probe = Server(slots=1, hbm_budget=1 << 40)
+    fp_big = probe.footprint(big[0], big[1], **big[2])
+    fp_small = probe.footprint(small[0][0], small[0][1], **small[0][2])
+    probe.close()
+    budget = fp_big + 2 * fp_small
+    ref_big = _alone(*big)
+    ref_small = [_alone(*j) for j in small]
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    peak_used = 0
+    with Server(slots=3, hbm_budget=budget) as srv:
+        def check():
+            nonlocal peak_used
+            st = srv.status()
+            assert st["claimed_bytes"] <= budget, st
+            peak_used = max(peak_used, free0 - torch.cuda.mem_get_info()[0])
+        _same(srv.submit(*big[:2], **big[2]).result(with_halo_id=True), ref_big)
+        check()
+        hs = [srv.submit(xp, vp, **kw) for xp, vp, kw in small]
+        for h, ref in zip(hs, ref_small):
+            _same(h.result(with_halo_id=True), ref)
+            check()
+        _same(srv.submit(*big[:2], **big[2]).result(with_halo_id=True), ref_big)
+        check()
+        assert srv.status()["resident_bytes"] > 0
+    print(f"budget {budget / 2**20:.0f} MiB, peak device memory held {peak_used / 2**20:.0f} MiB",
+          flush=True)
+    # streams, events and kernel arguments take a few MiB beyond the workspaces
+    assert peak_used <= budget + (64 << 20)
