@@ -308,16 +308,23 @@ def bench_bound(args, torch, _native, ctx):
             "GB_per_call": round(fft["bytes"] / fft["calls"] / 1e9, 3),
         }
     if not args.no_forms:
-        from py_fof_amd.tune import tune_direct_max
+        from py_fof_amd.tune import tune_direct_max, tune_fft_lines
         # plain per-halo FP64 form (one candidate at a time)
         form(1000, False, False)
         dt_plain, _, st_p = timed(1)
         log(args, f"unbinding (plain FP64 per halo): {dt_plain * 1e3:.1f} ms/call")
-        # autotuned crossover for the f32 form, then the batched f32 form
+        # autotuned crossover and FFT launch shape for the f32 form, then the
+        # batched f32 form
         dmax, table = tune_direct_max(f32=True)
         log(args, f"unbinding: tuned direct-summation threshold {dmax}")
+        fft_lines, fft_table = tune_fft_lines()
+        log(args, f"unbinding: tuned FFT lines per workgroup {fft_lines}")
         form(dmax, True, True)
+        for nsz, nl in fft_lines.items():
+            ctx.set_option(_native.OPT_FFT_LINES, 1000 * nsz + nl)
         dt_mix, _, st_m = timed(args.bound_steps)
+        for nsz in fft_lines:
+            ctx.set_option(_native.OPT_FFT_LINES, 1000 * nsz + 8)
         mix_in = in_h.clone()
         log(args, f"unbinding (batched f32, crossover {dmax}): {dt_mix * 1e3:.1f} ms/call")
         # potential error of the f32 form against FP64 at the same crossover
@@ -339,6 +346,7 @@ def bench_bound(args, torch, _native, ctx):
                              "ms_per_call": out["ms_per_call"], "crossover": 1000},
             "batched_f32_tuned": {"bound_halos_per_s": round(st_m["nvalid"] / dt_mix, 1),
                                   "ms_per_call": round(dt_mix * 1e3, 3), "crossover": dmax,
+                                  "fft_lines": {str(k): v for k, v in fft_lines.items()},
                                   "direct_sum_halos": st_m["n_direct"],
                                   "direct_sum_above_1000": st_m["n_direct_large"]},
             "speedup_batched_f32_vs_plain_fp64": round(dt_plain / dt_mix, 2),
@@ -351,6 +359,8 @@ def bench_bound(args, torch, _native, ctx):
             "status_differs_vs_reference_form": {
                 "particles": diff, "fraction": float(f"{diff / n:.3e}")},
             "tuning": table,
+            "fft_lines_tuned": {str(k): v for k, v in fft_lines.items()},
+            "fft_lines_tuning": fft_table,
         }
         out["physical_units"] = bench_physical(args, _native, dmax)
     out["deterministic"] = all(same)  # repeated calls of each form gave identical in_halo

@@ -114,6 +114,10 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
  *     convolution launch of the unbinding; pyfof_ctx_timings [6] and
  *     pyfof_ctx_stats [16..17] report their device time, slot count and
  *     algorithmic bytes.
+ *   PYFOF_OPT_FFT_LINES (value 1000 * n + lines; default 8 lines for every
+ *     n): grid lines per workgroup (4, 8 or 16) of the FFT convolutions on
+ *     n^3 grids, n in {48, 64, 128, 216, 256}. A launch shape: results do
+ *     not depend on it. py_fof_amd.tune.tune_fft_lines measures the best.
  *   PYFOF_OPT_POISON_ALLOC (default 0; process-wide, not per context): a
  *     debugging aid. 0 = off; 1 + b fills every device allocation made from
  *     then on with the byte b (0..255), so that a read of memory no kernel
@@ -126,6 +130,7 @@ int32_t pyfof_fof_ctx(pyfof_ctx* ctx, const double* xp, const double* vp, const 
 #define PYFOF_OPT_RECORD_PHI 5
 #define PYFOF_OPT_POISON_ALLOC 6
 #define PYFOF_OPT_TIME_FFT 7
+#define PYFOF_OPT_FFT_LINES 8
 int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value);
 
 /* First-pass potentials recorded by the last bound_check call on ctx (with

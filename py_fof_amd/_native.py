@@ -24,6 +24,7 @@ OPT_BATCHED = 4
 OPT_RECORD_PHI = 5
 OPT_POISON_ALLOC = 6  # process-wide: 0 off, 1 + b fills new device allocations with byte b
 OPT_TIME_FFT = 7
+OPT_FFT_LINES = 8  # value 1000 * n + lines
 
 EXPORTS = [
     "pyfof_fof", "pyfof_fof_ctx", "pyfof_ctx_create", "pyfof_ctx_destroy",

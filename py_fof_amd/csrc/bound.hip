@@ -2665,8 +2665,8 @@ static int32_t run_bound_impl(Ctx& C, const BoundArgs& A, uint8_t* keep_out, Bou
                                                                   cls_count[g], acc[g]);
                     PYFOF_LAUNCH_CHECK();
                     C.fft_begin();
-                    fft_conv3d(*plans[g], grid[g], acc[g], d_slots, (int32_t)sl.size(), d_slot_halo[g], fscale,
-                               Gfull[g], s);
+                    fft_conv3d(*plans[g], C.fft_lines_for(n), grid[g], acc[g], d_slots, (int32_t)sl.size(),
+                               d_slot_halo[g], fscale, Gfull[g], s);
                     C.fft_end(n, (int)sl.size());
                     k_interp<<<blocks(M), kB, 0, s>>>(qw, A.mhalo, d_pmof, d_pm, keep, first, wt, selfv, active,
                                                       mkeep, grid[g], g, M, A.a, phi_c1);
@@ -2707,8 +2707,8 @@ static int32_t run_bound_impl(Ctx& C, const BoundArgs& A, uint8_t* keep_out, Bou
                             PYFOF_LAUNCH_CHECK();
                         }
                         C.fft_begin();
-                        fft_conv3d(*plan_in[c], grid_in[c], acc_in[c], d_tl_ls, (int32_t)ls.size(), d_tl_sh,
-                                   fscale, Gin[c], s);
+                        fft_conv3d(*plan_in[c], C.fft_lines_for(n_in), grid_in[c], acc_in[c], d_tl_ls,
+                                   (int32_t)ls.size(), d_tl_sh, fscale, Gin[c], s);
                         C.fft_end(n_in, (int)ls.size());
                         k_tl_interp<<<blocks(M), kB, 0, s>>>(tgt_slot, tgt_first, tgt_wt, self_in, d_tlslots, ratio,
                                                              cs, cs + tl_chunk[c], keep, active, qw, grid_in[c], M,

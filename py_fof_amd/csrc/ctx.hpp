@@ -129,6 +129,23 @@ struct Ctx {
     int32_t direct_f32 = 0;
     int32_t batched = 1;
     bool record_phi = false;
+    // FFT convolution lines per workgroup for the grids 48, 64, 128, 216 and
+    // 256 (PYFOF_OPT_FFT_LINES; a launch shape, the results do not change)
+    int fft_lines[5] = {8, 8, 8, 8, 8};
+    static int fft_size_index(int n) {
+        switch (n) {
+            case 48: return 0;
+            case 64: return 1;
+            case 128: return 2;
+            case 216: return 3;
+            case 256: return 4;
+            default: return -1;
+        }
+    }
+    int fft_lines_for(int n) const {
+        const int i = fft_size_index(n);
+        return i < 0 ? 8 : fft_lines[i];
+    }
     // large candidates may use multi-workgroup cooperative launches; off for
     // contexts that share the GPU with others (the request server's workers),
     // where the co-residency those launches need is not assured

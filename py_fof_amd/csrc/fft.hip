@@ -332,23 +332,24 @@ __global__ void __launch_bounds__(kFftBlock) k_fft_conv(float2* __restrict__ dat
     }
 }
 
-template <int N>
+// NL lines per workgroup in every pass (4, 8 or 16; the transform of a line
+// does not depend on it, so neither does the result). More lines read longer
+// row segments per load and fewer workgroups stay resident (LDS 16 NL N B);
+// the best count depends on N and is measured (py_fof_amd.tune).
+template <int N, int NL>
 void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
                  int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G, hipStream_t s) {
-    // 8 lines per workgroup in every pass: 16 (whole 128 B row segments)
-    // doubles the LDS per workgroup and halves the resident workgroups, which
-    // measured slower (C3 unbinding 151 vs 131 ms per call)
-    constexpr int NLY = kLines;
-    constexpr int NLZ = kLines;
+    constexpr int NLY = NL;
+    constexpr int NLZ = NL;
     constexpr int m = N / 2 + 1, mc = m + 1;
     constexpr int64_t n2 = (int64_t)N * N;
-    constexpr int ygroups = (mc + kLines - 1) / kLines;
+    constexpr int ygroups = (mc + NL - 1) / NL;
     // the grids are real and G is real and even, so every spectrum is
     // Hermitian along x: the y and z passes keep kx <= N/2 only and the x
     // inverse rebuilds the other half
     constexpr int hx = N / 2;
     constexpr int xgroups_y = (hx + 1 + NLY - 1) / NLY, xgroups_z = (hx + 1 + NLZ - 1) / NLZ;
-    const AxisDesc ax_x{(int64_t)kLines * N, n2, ygroups, mc, N, 1, 1};
+    const AxisDesc ax_x{(int64_t)NL * N, n2, ygroups, mc, N, 1, 1};
     const AxisDesc ax_y{NLY, n2, xgroups_y, mc, 1, N, 0};
     const AxisDesc ax_z{NLZ, N, xgroups_z, N, 1, n2, 0};
     // {axis, line_lim, in_lim, out_lim}
@@ -359,7 +360,7 @@ void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, 
     const uint32_t gzz = (uint32_t)(xgroups_z * N);
     // about 2048 workgroups in the z pass: slot groups when the columns are few
     const uint32_t zgroups = (uint32_t)std::max(1, std::min(nslots, (int32_t)((2048 + gzz - 1) / gzz)));
-    k_fft_conv<kConvXFwd, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
+    k_fft_conv<kConvXFwd, NL, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxf);
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvYFwd, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyf);
     PYFOF_LAUNCH_CHECK();
@@ -367,7 +368,7 @@ void conv_launch(const FftPlan& p, float2* grid, const unsigned long long* acc, 
     PYFOF_LAUNCH_CHECK();
     k_fft_conv<kConvYInv, NLY, N><<<gy, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dyi);
     PYFOF_LAUNCH_CHECK();
-    k_fft_conv<kConvXInv, kLines, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxi);
+    k_fft_conv<kConvXInv, NL, N><<<gx, kFftBlock, 0, s>>>(grid, acc, slots, nslots, slot_halo, fscale, G, p.tw, dxi);
     PYFOF_LAUNCH_CHECK();
 }
 
@@ -417,16 +418,32 @@ void fft3d(const FftPlan& p, float2* data, const int32_t* slots, int32_t nslots,
     }
 }
 
-void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
-                int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
-                hipStream_t s) {
+namespace {
+template <int N>
+void conv_lines(const FftPlan& p, int lines, float2* grid, const unsigned long long* acc,
+                const int32_t* slots, int32_t nslots, const int32_t* slot_halo, const double* fscale,
+                const float* G, hipStream_t s) {
+    switch (lines) {
+        case 4: conv_launch<N, 4>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 8: conv_launch<N, 8>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 16: conv_launch<N, 16>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        default: throw HipError(hipErrorInvalidValue, "FFT lines per workgroup must be 4, 8 or 16");
+    }
+}
+}  // namespace
+
+bool fft_conv_size_ok(int n) { return n == 48 || n == 64 || n == 128 || n == 216 || n == 256; }
+
+void fft_conv3d(const FftPlan& p, int lines, float2* grid, const unsigned long long* acc,
+                const int32_t* slots, int32_t nslots, const int32_t* slot_halo, const double* fscale,
+                const float* G, hipStream_t s) {
     if (nslots <= 0) return;
     switch (p.n) {
-        case 48: conv_launch<48>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
-        case 64: conv_launch<64>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
-        case 128: conv_launch<128>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
-        case 216: conv_launch<216>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
-        case 256: conv_launch<256>(p, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 48: conv_lines<48>(p, lines, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 64: conv_lines<64>(p, lines, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 128: conv_lines<128>(p, lines, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 216: conv_lines<216>(p, lines, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
+        case 256: conv_lines<256>(p, lines, grid, acc, slots, nslots, slot_halo, fscale, G, s); break;
         default: throw HipError(hipErrorInvalidValue, "no convolution kernel for this grid size");
     }
 }

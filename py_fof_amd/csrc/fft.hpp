@@ -30,9 +30,13 @@ void fft3d(const FftPlan& p, float2* data, const int32_t* slots, int32_t nslots,
 // nodes [0, n/2 + 1]^3, unnormalised (divide by n^3). acc holds each slot's
 // int64 fixed-point deposit, non-zero only on [0, n/2 + 1]^3; slot_halo maps a
 // slot to the halo whose scale fscale[] it uses. Other nodes of grid are
-// left with intermediate values.
-void fft_conv3d(const FftPlan& p, float2* grid, const unsigned long long* acc, const int32_t* slots,
-                int32_t nslots, const int32_t* slot_halo, const double* fscale, const float* G,
-                hipStream_t s);
+// left with intermediate values. lines: grid lines per workgroup (4, 8 or
+// 16), a launch shape only: the result does not depend on it.
+void fft_conv3d(const FftPlan& p, int lines, float2* grid, const unsigned long long* acc,
+                const int32_t* slots, int32_t nslots, const int32_t* slot_halo, const double* fscale,
+                const float* G, hipStream_t s);
+
+// True for the grid sizes fft_conv3d has kernels for (48, 64, 128, 216, 256).
+bool fft_conv_size_ok(int n);
 
 }  // namespace pyfof

@@ -2153,6 +2153,17 @@ int32_t pyfof_ctx_set_option(pyfof_ctx* ctx, int32_t option, int64_t value) {
         case PYFOF_OPT_BATCHED: C.batched = value != 0; return IERR_OK;
         case PYFOF_OPT_RECORD_PHI: C.record_phi = value != 0; return IERR_OK;
         case PYFOF_OPT_TIME_FFT: C.time_fft = value != 0; return IERR_OK;
+        case PYFOF_OPT_FFT_LINES: {
+            const int n = (int)(value / 1000), lines = (int)(value % 1000);
+            const int i = Ctx::fft_size_index(n);
+            if (value < 0 || i < 0 || (lines != 4 && lines != 8 && lines != 16)) {
+                set_error("PYFOF_OPT_FFT_LINES takes 1000 * n + lines with n in {48, 64, 128, 216, 256} "
+                          "and lines in {4, 8, 16}");
+                return IERR_INVALID;
+            }
+            C.fft_lines[i] = lines;
+            return IERR_OK;
+        }
         case PYFOF_OPT_POISON_ALLOC:
             if (value < 0 || value > 256) {
                 set_error("PYFOF_OPT_POISON_ALLOC takes 0 or 1 + a byte value (at most 256)");

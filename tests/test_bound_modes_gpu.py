@@ -8,7 +8,10 @@
   partners in f32 added in f64) stays within the stated potential error of
   the FP64 form: relative error of every member's first-pass potential
   <= 1e-5, RMS <= 1e-6; bound status differs for at most 0.1 % of the
-  members.
+  members;
+* the FFT convolutions' lines per workgroup (4, 8, 16: a launch shape) give
+  bit-identical potentials and members on every mesh size, two-level inner
+  grids included.
 """
 import numpy as np
 import pytest
@@ -19,10 +22,13 @@ L = 512.0
 
 
 def _run(xs, vs, b=0.25, np_min=10, softening=0.1, direct_max=1000, f32=False, batched=True,
-         record=False):
+         record=False, fft_lines=None):
     import py_fof_amd as pf
     from py_fof_amd import _native
     ctx = _native.Context(0)
+    for n in (48, 64, 128, 216, 256):
+        if fft_lines:
+            ctx.set_option(_native.OPT_FFT_LINES, 1000 * n + fft_lines)
     ctx.set_option(_native.OPT_DIRECT_MAX, direct_max)
     ctx.set_option(_native.OPT_DIRECT_FP32, int(f32))
     ctx.set_option(_native.OPT_BATCHED, int(batched))
@@ -150,3 +156,21 @@ def test_no_reads_of_unwritten_device_memory():
         _native.Context(0).set_option(_native.OPT_POISON_ALLOC, 0)
     for a, b in zip(zero, ones):
         assert np.array_equal(a, b)
+
+
+def test_fft_lines_per_workgroup_do_not_change_results():
+    sizes = [1500, 3000, 6000, 12000, 30000, 90000]
+    xs, vs = _halos(sizes, 43)
+    runs = {nl: _run(xs, vs, softening=0.05, record=True, fft_lines=nl) for nl in (4, 8, 16)}
+    st = runs[8]["stats"]
+    assert st["n_pm256"] >= 1 and st["n_pm48"] + st["n_pm64"] + st["n_pm128"] >= 1
+    for nl in (4, 16):
+        _same(runs[nl], runs[8])
+        assert np.array_equal(runs[nl]["phi"], runs[8]["phi"])
+    with pytest.raises(RuntimeError):
+        from py_fof_amd import _native
+        c = _native.Context(0)
+        try:
+            c.set_option(_native.OPT_FFT_LINES, 100 * 1000 + 8)
+        finally:
+            c.close()
