@@ -62,7 +62,7 @@ def parse():
                     help="skip the plain / f32 / tuned unbinding forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=2000)
-    ap.add_argument("--serve-pool", type=int, default=32, help="distinct job inputs, reused")
+    ap.add_argument("--serve-pool", type=int, default=40, help="distinct job inputs, reused")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=0.0,
                     help="Poisson arrival rate, jobs/s (0: 0.9 x the sequential rate)")
@@ -423,11 +423,12 @@ def serve_pool(n_pool, seed=5):
     """Job inputs of the serving stream: periodic sub-boxes of N particles,
     N log-uniform in [1e4, 2e7], the NFW-clump / uniform 50/50 mixture at one
     particle per cube cell (L = N^(1/3)), Gaussian velocities (sigma 0.5),
-    30 % with bound_check (s_15 from ng_global 2048)."""
+    30 % of them (rounded) with bound_check (s_15 from ng_global 2048)."""
     from py_fof_amd.synth import clumpy_nfw
     rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(1e4), np.log(2e7), n_pool)).astype(np.int64)
-    bound = rng.uniform(size=n_pool) < 0.3
+    bound = np.zeros(n_pool, bool)
+    bound[rng.choice(n_pool, int(round(0.3 * n_pool)), replace=False)] = True
     jobs = []
     for k, (n, b) in enumerate(zip(ns, bound)):
         L = float(n) ** (1.0 / 3.0)
@@ -475,6 +476,7 @@ def bench_serve(args):
     rate = args.serve_rate if args.serve_rate > 0 else 0.9 * seq_rate
     rng = np.random.default_rng(55)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, args.serve_jobs))
+    pick = rng.integers(0, len(pool), args.serve_jobs)  # which pool input each job carries
     lat = [None] * args.serve_jobs
     handles = [None] * args.serve_jobs
     with Server(slots=args.serve_slots, hbm_budget=200 << 30) as srv:
@@ -499,7 +501,7 @@ def bench_serve(args):
             wait = t0 + arrivals[i] - time.perf_counter()
             if wait > 0:
                 time.sleep(wait)
-            xp, vp, kw = pool[i % len(pool)]
+            xp, vp, kw = pool[pick[i]]
             t_arr = t0 + arrivals[i]
             handles[i] = (srv.submit(xp, vp, **kw), t_arr)
             if time.perf_counter() - last_log > 20.0:
@@ -509,11 +511,13 @@ def bench_serve(args):
         wall = time.perf_counter() - t0
     lat_ms = [x * 1e3 for x in lat]
     nb = sum(1 for j in pool if j[2]["bound_check"])
+    nbj = int(sum(1 for k in pick if pool[k][2]["bound_check"]))
     return {
         "workload": f"{args.serve_jobs} jobs, Poisson arrivals at {rate:.2f} jobs/s, inputs "
                     f"from a pool of {len(pool)} periodic sub-boxes (N log-uniform in [1e4, 2e7], "
                     f"NFW clumps 50% + uniform, 1 particle per cell, sigma_v 0.5; {nb} of "
-                    f"{len(pool)} with bound_check), host inputs, {args.serve_slots} slots",
+                    f"{len(pool)} with bound_check), each job a uniformly drawn pool input "
+                    f"({nbj} jobs with bound_check), host inputs, {args.serve_slots} slots",
         "arrival_rate_jobs_per_s": round(rate, 3),
         "jobs_per_s": round(args.serve_jobs / wall, 3),
         "p50_ms": _pct(lat_ms, 50),
