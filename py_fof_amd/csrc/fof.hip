@@ -1759,14 +1759,26 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
-            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
-            PYFOF_LAUNCH_CHECK();
         }
-        k_link<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
+        // Every parent written by a plain store (the clique pre-unions,
+        // k_dense_intra, k_cell_intra) is in place; from here on every kernel
+        // joins components only by atomic unions, whose order does not change
+        // the partition. The sparse-particle kernels run on the second
+        // stream beside the mid- and dense-cell kernels, filling the GPU with
+        // two kinds of latency-bound waves (LDS-heavy and register-heavy).
+        hipStream_t s2 = C.stream2;
+        PYFOF_HIP(hipEventRecord(C.ev_fork, s));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
+        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         if (ndense > 0) {
-            k_link_dense<<<nblk(N), kBlk, 0, s>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
-                                                  d_reg, parent, dmask, P);
+            k_link_dense<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
+                                                   d_reg, parent, dmask, P);
+            PYFOF_LAUNCH_CHECK();
+        }
+        PYFOF_HIP(hipEventRecord(C.ev_join, s2));
+        if (nmid > 0) {
+            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         if (nsubs > 0) {
@@ -1779,6 +1791,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                        parent, P);
             PYFOF_LAUNCH_CHECK();
         }
+        PYFOF_HIP(hipStreamWaitEvent(s, C.ev_join, 0));
         C.mark(kEvLink);
 
         // ---- roots, sizes, candidate numbering, outputs
@@ -2005,6 +2018,7 @@ int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
         err = run_fof(*C, A);
     } catch (const HipError& e) {
         set_error(e.what());
+        (void)hipStreamSynchronize(C->stream2);
         (void)hipStreamSynchronize(C->stream);
         err = e.code == hipErrorOutOfMemory ? IERR_ALLOC : IERR_HIP;
     } catch (const std::bad_alloc&) {
