@@ -1356,9 +1356,10 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     const int32_t r = label[p];
     const bool cand = (int64_t)size[r] >= (int64_t)np_min;
     const uint32_t i = perm[p];
-    // the one scattered write per particle; in_halo follows from halo_id in
-    // a coalesced pass (k_in_halo)
-    halo_id[i] = cand ? (int32_t)cand_idx[r] + 1 : 0;
+    // the one scattered write per halo member (halo_id was zero-filled,
+    // coalesced, before); in_halo follows from halo_id in a coalesced pass
+    // (k_in_halo)
+    if (cand) halo_id[i] = (int32_t)cand_idx[r] + 1;
     if (root_out) root_out[i] = reg[__float_as_int(xc[p].w)].skip ? -1 : (int32_t)perm[r];
     if (flag[p]) {
         anchor[cand_idx[p]] = i;
@@ -1812,6 +1813,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
+        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(reinterpret_cast<uint32_t*>(o_id), N, 0u);
+        PYFOF_LAUNCH_CHECK();
         k_write<<<nblk(N), kBlk, 0, s>>>(label, size, flag, cidx, perm, xc, d_reg, N, A.np_min, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
