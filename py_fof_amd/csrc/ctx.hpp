@@ -116,9 +116,11 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // second stream of the linking phase: the sparse-particle kernels run on
-    // it beside the mid- and dense-cell kernels (all join by atomic unions)
+    // it beside the mid- and dense-cell kernels (fork after binning, intra:
+    // the local components of mid and dense cells are written, join before
+    // the relabel)
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_intra = nullptr, ev_join = nullptr;
     hipEvent_t ev[kNumEv] = {};
     bool ev_recorded[kNumEv] = {};
     DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
@@ -208,6 +210,7 @@ struct Ctx {
         PYFOF_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
         for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
         PYFOF_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        PYFOF_HIP(hipEventCreateWithFlags(&ev_intra, hipEventDisableTiming));
         PYFOF_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     }
     ~Ctx() {
@@ -215,6 +218,7 @@ struct Ctx {
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         for (auto& e : fft_ev) if (e) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_intra) (void)hipEventDestroy(ev_intra);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);

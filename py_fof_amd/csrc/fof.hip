@@ -1740,8 +1740,20 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         C.last_ncell = ncell;
         C.last_ndense = ndense;
 
-        // ---- linking. k_dense_intra writes the parents of dense-cell members
-        // directly (local components), so it runs before any atomic union.
+        // ---- linking. k_dense_intra and k_cell_intra write the parents of
+        // dense-clique and mid-cell members directly (local components), so
+        // they run before any atomic union on those members. k_link touches
+        // neither kind (it joins sparse particles, and the dense-cell members
+        // of regions without cliques, which no plain store writes), so it
+        // starts at once on the second stream, beside them.
+        hipStream_t s2 = C.stream2;
+        // per-particle masks of dense cells to scan, in the buffer that holds
+        // the labels after linking
+        uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
+        PYFOF_HIP(hipEventRecord(C.ev_fork, s));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
+        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
+        PYFOF_LAUNCH_CHECK();
         int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* croot = C.croot.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* needs_sub = C.needs_sub.as<int32_t>(std::max<uint32_t>(ndense, 1));
@@ -1754,9 +1766,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                     sub_list, xc, parent, ncomp, croot, needs_sub, P);
             PYFOF_LAUNCH_CHECK();
         }
-        // per-particle masks of dense cells to scan, in the buffer that holds
-        // the labels after linking
-        uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
@@ -1764,14 +1773,11 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // Every parent written by a plain store (the clique pre-unions,
         // k_dense_intra, k_cell_intra) is in place; from here on every kernel
         // joins components only by atomic unions, whose order does not change
-        // the partition. The sparse-particle kernels run on the second
-        // stream beside the mid- and dense-cell kernels, filling the GPU with
-        // two kinds of latency-bound waves (LDS-heavy and register-heavy).
-        hipStream_t s2 = C.stream2;
-        PYFOF_HIP(hipEventRecord(C.ev_fork, s));
-        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
-        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
-        PYFOF_LAUNCH_CHECK();
+        // the partition. The sparse-particle kernels on the second stream
+        // share the GPU with the mid- and dense-cell kernels: two kinds of
+        // latency-bound waves (register-heavy and LDS-heavy).
+        PYFOF_HIP(hipEventRecord(C.ev_intra, s));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_intra, 0));
         if (ndense > 0) {
             k_link_dense<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
                                                    d_reg, parent, dmask, P);
