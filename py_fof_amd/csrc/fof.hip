@@ -916,6 +916,8 @@ __device__ __forceinline__ bool cliques_linked(uint32_t a0, uint32_t a1, uint32_
 // are processed nearest first in classes separated by barriers, so that once
 // the close sub-cells have merged the far offsets are skipped on the root
 // check alone. Cells with at most kIntraStage members are staged in LDS.
+// Each sub-cell's local root (or -1 for an empty one) is kept in sub_root,
+// the row of the cell's sub_ptr layout, for k_dense_cross.
 constexpr int kIntraStage = 768;
 constexpr int kFwdClassEnd[5] = {3, 9, 13, 16, 62};
 
@@ -925,7 +927,7 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
                             const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
                             const float4* __restrict__ xc, int32_t* parent, int32_t* __restrict__ ncomp,
                             int32_t* __restrict__ croot, int32_t* __restrict__ needs_sub,
-                            unsigned char* lds) {
+                            int32_t* __restrict__ sub_root, unsigned char* lds) {
     const int32_t k0 = g.k[0], k1 = g.k[1], k2 = g.k[2];
     const int32_t K = k0 * k1 * k2;
     uint32_t* rp = reinterpret_cast<uint32_t*>(lds);
@@ -988,9 +990,13 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
         if (r == A) atomicAdd(&nl[1], 1u);
     }
     __syncthreads();
+    int32_t* srow = sub_root + (size_t)d * P.sub_stride;
+    for (int i = threadIdx.x; i < K; i += kBlk)
+        if (rp[i + 1] == rp[i]) srow[i] = -1;
     for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
         const int32_t A = (int32_t)list[i];
         const int32_t root = (int32_t)mh[lds_find(lp, A)];
+        srow[A] = root;
         for (uint32_t t = rp[A]; t < rp[A + 1]; ++t) parent[sub_list[cs + t]] = root;
     }
     if (threadIdx.x == 0) {
@@ -1009,7 +1015,8 @@ __global__ void __launch_bounds__(kBlk) k_dense_intra(const uint32_t* __restrict
                                                       const float4* __restrict__ xc, int32_t* parent,
                                                       int32_t* __restrict__ ncomp,
                                                       int32_t* __restrict__ croot,
-                                                      int32_t* __restrict__ needs_sub, LinkParams P) {
+                                                      int32_t* __restrict__ needs_sub,
+                                                      int32_t* __restrict__ sub_root, LinkParams P) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t d = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t c = dense_cells[d];
@@ -1020,10 +1027,10 @@ __global__ void __launch_bounds__(kBlk) k_dense_intra(const uint32_t* __restrict
     }
     if (g.is_periodic)
         dense_intra<true>(d, g, P, dense_cells, cell_ptr, sub_ptr, sub_list, xc, parent, ncomp, croot,
-                          needs_sub, lds);
+                          needs_sub, sub_root, lds);
     else
         dense_intra<false>(d, g, P, dense_cells, cell_ptr, sub_ptr, sub_list, xc, parent, ncomp, croot,
-                           needs_sub, lds);
+                           needs_sub, sub_root, lds);
 }
 
 // One wave per dense cell of a single local component: for each forward
@@ -1172,6 +1179,8 @@ __global__ void __launch_bounds__(kWave) k_dense_cell_cross(const uint32_t* __re
         cell_cross<false>(d, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
 }
 
+constexpr int kJoinedCache = 4;  // local roots remembered as joined in dense_cross
+
 // One thread per non-empty clique sub-cell A of a cell with several local
 // components, or next to such a cell: offsets into forward dense coarse cells,
 // sub-cell against sub-cell. (Pairs with sparse cells are taken by k_link.)
@@ -1185,6 +1194,7 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
                                             const int32_t* __restrict__ needs_sub,
                                             const uint32_t* __restrict__ sub_ptr,
                                             const uint32_t* __restrict__ sub_list,
+                                            const int32_t* __restrict__ sub_root,
                                             const float4* __restrict__ xc, int32_t* parent) {
     const uint32_t d = ent.x, si = ent.y;
     const uint32_t c = dense_cells[d];
@@ -1222,9 +1232,21 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
     const uint32_t a0 = cs + rowA[si], a1 = cs + rowA[si + 1];
     int32_t rA = dsu_find(parent, (int32_t)ent.z);
     // pairs of single-component dense cells are decided per cell pair by
-    // k_dense_cell_cross; the last neighbour found joined is remembered
+    // k_dense_cell_cross. B's sub-cell is known by its local root (k_dense_
+    // intra's component of that cell); the last kJoinedCache local roots found
+    // joined to A are remembered, so the sub-cells of a local component
+    // already joined are skipped without a find.
     const bool single = ncomp[d] == 1;
-    int32_t same_cell = -1;
+    int32_t joined[kJoinedCache];
+#pragma unroll
+    for (int i = 0; i < kJoinedCache; ++i) joined[i] = -1;
+    int next = 0;
+    auto remember = [&](int32_t lb) {
+#pragma unroll
+        for (int i = 0; i < kJoinedCache; ++i)
+            if (i == next) joined[i] = lb;
+        next = next + 1 == kJoinedCache ? 0 : next + 1;
+    };
     for (int oz = 0; oz < 5; ++oz) {
         const int32_t jz = jt[2][oz];
         if (jz < 0) continue;
@@ -1241,18 +1263,26 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
                 const int32_t dB = aux[cB];
                 if (dB < 0) continue;  // sparse cell: below
                 if (single && ncomp[dB] == 1) continue;  // decided by k_dense_cell_cross
-                if (dB == same_cell) continue;
                 const uint32_t sb = (uint32_t)st[0][ox] + (uint32_t)kk[0] * ((uint32_t)st[1][oy] + (uint32_t)kk[1] * (uint32_t)st[2][oz]);
+                const int32_t lb = sub_root[(size_t)dB * P.sub_stride + sb];
+                if (lb < 0) continue;  // empty sub-cell
+                bool known = false;
+#pragma unroll
+                for (int i = 0; i < kJoinedCache; ++i) known = known || joined[i] == lb;
+                if (known) continue;
+                const int32_t rB = dsu_find(parent, lb);
+                if (rB != rA) rA = dsu_find(parent, rA);
+                if (rB == rA) {
+                    remember(lb);
+                    continue;
+                }
                 const uint32_t* rowB = sub_ptr + (size_t)dB * P.sub_stride;
                 const uint32_t bs = cell_ptr[cB];
                 const uint32_t b0 = bs + rowB[sb], b1 = bs + rowB[sb + 1];
-                if (b0 >= b1) continue;
-                const int32_t rB = dsu_find(parent, (int32_t)sub_list[b0]);
-                if (rB == rA) continue;
-                rA = dsu_find(parent, rA);
-                if (rB == rA) continue;
-                if (cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P)) rA = dsu_unite(parent, rA, rB);
-                if (ncomp[dB] == 1 && dsu_find(parent, croot[dB]) == rA) same_cell = dB;
+                if (cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P)) {
+                    rA = dsu_unite(parent, rA, rB);
+                    remember(lb);
+                }
             }
         }
     }
@@ -1267,6 +1297,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
                                                       const int32_t* __restrict__ needs_sub,
                                                       const uint32_t* __restrict__ sub_ptr,
                                                       const uint32_t* __restrict__ sub_list,
+                                                      const int32_t* __restrict__ sub_root,
                                                       const RegionDev* __restrict__ reg,
                                                       const float4* __restrict__ xc, int32_t* parent,
                                                       LinkParams P) {
@@ -1275,9 +1306,11 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
     const uint4 ent = subs[t];
     const RegionDev& g = reg[ent.w];
     if (g.is_periodic)
-        dense_cross<true>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
+        dense_cross<true>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list,
+                          sub_root, xc, parent);
     else
-        dense_cross<false>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
+        dense_cross<false>(ent, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list,
+                           sub_root, xc, parent);
 }
 
 // labels, and each root's own count: size[p] = 1 for a root, 0 otherwise
@@ -1757,13 +1790,15 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* croot = C.croot.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* needs_sub = C.needs_sub.as<int32_t>(std::max<uint32_t>(ndense, 1));
+        int32_t* sub_root = C.sub_root.as<int32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         if (nsubs > 0) {
             // LDS: sub-cell tables, plus staged positions within a 64 KiB budget
             const size_t base = (size_t)16 * ((4 * P.sub_stride + 2 + 3) / 4);
             P.intra_stage = (int32_t)std::min<size_t>(kIntraStage, base < 65536 ? (65536 - base) / 16 : 0);
             const size_t lds = base + 16 * (size_t)P.intra_stage;
             k_dense_intra<<<ndense, kBlk, lds, s>>>(dense_cells, cell_ptr, d_coff, d_reg, H, sub_ptr,
-                                                    sub_list, xc, parent, ncomp, croot, needs_sub, P);
+                                                    sub_list, xc, parent, ncomp, croot, needs_sub, sub_root,
+                                                    P);
             PYFOF_LAUNCH_CHECK();
         }
         if (nmid > 0) {
@@ -1794,8 +1829,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                         xc, parent, P);
             PYFOF_LAUNCH_CHECK();
             k_dense_cross<<<nblk(nsubs), kBlk, 0, s>>>(subs, nsubs, dense_cells, cell_ptr, aux, ncomp,
-                                                       croot, needs_sub, sub_ptr, sub_list, d_reg, xc,
-                                                       parent, P);
+                                                       croot, needs_sub, sub_ptr, sub_list, sub_root, d_reg,
+                                                       xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipStreamWaitEvent(s, C.ev_join, 0));
