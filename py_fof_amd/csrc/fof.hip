@@ -139,6 +139,76 @@ __global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint3
     if (p + 1 == n || skeys[p + 1] != k) ends[k] = p + 1;
 }
 
+// cell_ptr[c] = the first CRS position whose cell is c or later, for
+// c = 0..ncell (cell_ptr[ncell] = n), straight from the sorted keys:
+// position p (0 <= p <= n) owns the entries key[p-1] + 1 .. key[p] (from 0
+// for p = 0, up to ncell for p = n), which are p. A wave's 64 positions own
+// one contiguous range of entries; up to kWaveSpan of them are written by
+// all lanes in turn, each entry's owner found by a binary search over the
+// lanes' last entries. In a wave with a longer range each lane writes its own
+// run of up to kGapInline entries, and a longer run (empty space) is cut into
+// records of kGapChunk entries that k_fill_gaps writes with whole
+// workgroups. Records needed: at most (ncell + 1) / kGapInline.
+constexpr uint32_t kGapInline = 64, kGapChunk = 4096, kWaveSpan = 1024;
+
+__host__ __device__ inline uint64_t gap_records(uint64_t ncell) { return (ncell + 1) / kGapInline + 2; }
+
+__global__ void k_cell_starts(const uint32_t* __restrict__ skeys, uint32_t n, uint64_t ncell,
+                              uint32_t* __restrict__ cell_ptr, uint4* __restrict__ gaps,
+                              uint32_t* __restrict__ ngaps) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = __lane_id();
+    // last entry owned by p; positions past n own nothing (ncell again);
+    // ncell < 2^32 - 1, so every entry index fits 32 bits
+    const uint32_t hi32 = p < n ? skeys[p] : (uint32_t)ncell;
+    const uint64_t p0 = p - (uint64_t)lane;
+    if (p0 > n) return;  // a wave past position n owns nothing
+    const uint64_t first = p0 == 0 ? 0 : (uint64_t)skeys[p0 - 1] + 1;  // wave-uniform
+    const uint64_t last = (uint64_t)__shfl(hi32, kWave - 1, kWave);
+    if (first > last) return;  // the whole wave is in one cell
+    if (last - first + 1 <= kWaveSpan) {
+        // a wave-uniform trip count: the shuffles read every lane's hi32,
+        // which a lane that had left the loop would no longer provide
+        for (uint64_t base = first; base <= last; base += kWave) {
+            const uint64_t e = base + (uint64_t)lane;
+            // owner: the first lane whose last entry is e or later
+            int j = 0;
+            for (int step = kWave / 2; step > 0; step >>= 1) {
+                const uint32_t h = (uint32_t)__shfl(hi32, j + step - 1, kWave);
+                if ((uint64_t)h < e) j += step;
+            }
+            if (e <= last) cell_ptr[e] = (uint32_t)(p0 + (uint64_t)j);
+        }
+        return;
+    }
+    if (p > n) return;
+    const uint64_t lo = p == 0 ? 0 : (uint64_t)skeys[p - 1] + 1;
+    const uint64_t hi = hi32;
+    if (lo > hi) return;  // same cell as position p - 1
+    const uint64_t len = hi - lo + 1;
+    if (len <= kGapInline) {
+        for (uint64_t c = lo; c <= hi; ++c) cell_ptr[c] = (uint32_t)p;
+        return;
+    }
+    const uint32_t nch = (uint32_t)((len + kGapChunk - 1) / kGapChunk);
+    const uint32_t at = atomicAdd(ngaps, nch);
+    for (uint32_t k = 0; k < nch; ++k) {
+        const uint64_t a = lo + (uint64_t)k * kGapChunk;
+        const uint64_t b = a + kGapChunk < hi + 1 ? a + kGapChunk : hi + 1;
+        gaps[at + k] = make_uint4((uint32_t)a, (uint32_t)b, (uint32_t)p, 0u);
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_fill_gaps(const uint4* __restrict__ gaps,
+                                                    const uint32_t* __restrict__ ngaps,
+                                                    uint32_t* __restrict__ cell_ptr) {
+    const uint32_t m = *ngaps;
+    for (uint32_t g = blockIdx.x; g < m; g += gridDim.x) {
+        const uint4 r = gaps[g];
+        for (uint32_t c = r.x + threadIdx.x; c < r.y; c += kBlk) cell_ptr[c] = r.z;
+    }
+}
+
 // x_crs in CRS order (fof_fortran.f90:608): the input-order copy permuted.
 __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
                                                  const uint32_t* __restrict__ perm,
@@ -1732,12 +1802,15 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s);
         uint32_t* skeys = alt ? k1 : k0;
         uint32_t* perm = alt ? v1 : v0;
+        uint32_t* scal = C.scal.as<uint32_t>(64);
+        k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
+        PYFOF_LAUNCH_CHECK();
         uint32_t* cell_ptr = C.ends.as<uint32_t>(ncell + 1);
-        k_fill_u32<<<nblk(ncell + 1), kBlk, 0, s>>>(cell_ptr, ncell + 1, 0u);
+        uint4* gaps = C.buf("gaps").as<uint4>(gap_records(ncell));
+        k_cell_starts<<<nblk((uint64_t)N + 1), kBlk, 0, s>>>(skeys, N, ncell, cell_ptr, gaps, scal + 16);
         PYFOF_LAUNCH_CHECK();
-        k_run_ends<<<nblk(N), kBlk, 0, s>>>(skeys, N, cell_ptr + 1);
+        k_fill_gaps<<<1024, kBlk, 0, s>>>(gaps, scal + 16, cell_ptr);
         PYFOF_LAUNCH_CHECK();
-        inclusive_max_scan_u32(cell_ptr + 1, cell_ptr + 1, ncell, tmp, s);
         float4* xc = C.xc.as<float4>(N);
         int32_t* parent = C.parent.as<int32_t>(N);
         k_gather<<<nblk(N), kBlk, 0, s>>>(xr, perm, xc, parent, N);
@@ -1745,9 +1818,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // dense cells and their sub-cell lists
         int32_t* aux = C.aux.as<int32_t>(ncell);
         uint32_t* dense_cells = C.dense.as<uint32_t>(N / kDenseMin + 1);
-        uint32_t* scal = C.scal.as<uint32_t>(64);
-        k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
-        PYFOF_LAUNCH_CHECK();
         uint32_t* mid_cells = C.mid.as<uint32_t>(N / kMidMin + 1);
         const uint64_t mark_cells = (uint64_t)kMarkBlk * kMarkPer;
         k_dense_mark<<<(uint32_t)((ncell + mark_cells - 1) / mark_cells), kMarkBlk, 0, s>>>(
