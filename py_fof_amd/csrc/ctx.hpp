@@ -115,12 +115,12 @@ enum { kEvStart = 0, kEvBbox, kEvBin, kEvLink, kEvOut, kEvBound, kEvCat, kNumEv 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // second stream of the linking phase: the sparse-particle kernels run on
-    // it beside the mid- and dense-cell kernels (fork after binning, intra:
-    // the local components of mid and dense cells are written, join before
-    // the relabel)
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_intra = nullptr, ev_join = nullptr;
+    // further streams of the linking phase: the sparse-particle kernels run
+    // on stream2 and the mid-cell kernels on stream3, beside the dense-cell
+    // kernels on stream (fork after binning; ev_dense / ev_mid: the local
+    // components of dense / mid cells are written; joins before the relabel)
+    hipStream_t stream2 = nullptr, stream3 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_dense = nullptr, ev_mid = nullptr, ev_join = nullptr, ev_pair = nullptr;
     hipEvent_t ev[kNumEv] = {};
     bool ev_recorded[kNumEv] = {};
     DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
@@ -208,18 +208,18 @@ struct Ctx {
         fof_init_device_tables();
         PYFOF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         PYFOF_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+        PYFOF_HIP(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
         for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
-        PYFOF_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        PYFOF_HIP(hipEventCreateWithFlags(&ev_intra, hipEventDisableTiming));
-        PYFOF_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        for (hipEvent_t* e : {&ev_fork, &ev_dense, &ev_mid, &ev_join, &ev_pair})
+            PYFOF_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     ~Ctx() {
         (void)hipSetDevice(device);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         for (auto& e : fft_ev) if (e) (void)hipEventDestroy(e);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_intra) (void)hipEventDestroy(ev_intra);
-        if (ev_join) (void)hipEventDestroy(ev_join);
+        for (hipEvent_t e : {ev_fork, ev_dense, ev_mid, ev_join, ev_pair})
+            if (e) (void)hipEventDestroy(e);
+        if (stream3) (void)hipStreamDestroy(stream3);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
         release_workspace();

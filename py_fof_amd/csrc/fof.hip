@@ -503,14 +503,14 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
                                                   const uint32_t* __restrict__ cell_ptr,
                                                   const int32_t* __restrict__ aux,
                                                   const float4* __restrict__ xc, int32_t* parent) {
+    // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
+    if (aux[c] == kAuxMid) return 0u;
     LinkCtx L{parent, (int32_t)p, 0u};  // rp becomes a root at the first unite
     const uint32_t lc = c - g.cell_off;
     int32_t ijk[3];
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
-    if (aux[c] == kAuxMid) return 0u;
     visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, false, g, P, cell_ptr, aux, xc, L);
     auto neighbour = [&](int o) {
         int32_t jijk[3];
@@ -748,10 +748,18 @@ __device__ __forceinline__ void lds_unite(int32_t* lp, int32_t a, int32_t b) {
 // smallest position of its local component, a valid state for the global
 // union-find (parents point to smaller positions). k_link then skips these
 // cells' own pairs.
+//
+// A 64-pair round links about 2 pairs in the clump cells of the benchmark,
+// so joining them where they are found would run the union-find loops with
+// a lane or two of 64. The linked pairs are queued in LDS instead and joined
+// 64 at a time, one per lane.
+constexpr uint32_t kIntraQueue = 2 * kWave;
+
 template <bool PERIODIC>
 __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const float4* __restrict__ xc,
-                           int32_t* parent, float4* pos, int32_t* lp) {
+                           int32_t* parent, float4* pos, int32_t* lp, uint16_t* hq) {
     const int lane = __lane_id();
+    const uint64_t below = lanemask_lt();
     for (uint32_t i = lane; i < n; i += kWave) {
         pos[i] = xc[s + i];
         lp[i] = (int32_t)i;
@@ -763,12 +771,37 @@ __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const fl
     const uint32_t total = n * ((n - 1) / 2) + ((n & 1u) ? 0u : n / 2);
     uint32_t k = (uint32_t)lane, m = 1;
     while (k >= n) { k -= n; ++m; }
-    for (uint32_t t = lane; t < total; t += kWave) {
-        uint32_t j = k + m;
-        if (j >= n) j -= n;
-        if (pair_linked_screened<PERIODIC>(pos[k], pos[j], P)) lds_unite(lp, (int32_t)k, (int32_t)j);
+    uint32_t nq = 0;  // queued pairs (wave-uniform)
+    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+        bool hit = false;
+        uint32_t j = 0;
+        if (t0 + (uint32_t)lane < total) {
+            j = k + m;
+            if (j >= n) j -= n;
+            hit = pair_linked_screened<PERIODIC>(pos[k], pos[j], P);
+        }
+        const uint64_t hits = __ballot(hit);
+        if (hit) hq[nq + (uint32_t)__popcll(hits & below)] = (uint16_t)(k | (j << 8));
+        nq += (uint32_t)__popcll(hits);
+        if (nq >= (uint32_t)kWave) {
+            // one queued pair per lane; the rest (fewer than kWave) moves to the front
+            __syncthreads();
+            const uint32_t e = hq[lane];
+            const uint32_t rest = nq - (uint32_t)kWave;
+            const uint32_t moved = (uint32_t)lane < rest ? hq[kWave + lane] : 0u;
+            lds_unite(lp, (int32_t)(e & 0xffu), (int32_t)(e >> 8));
+            __syncthreads();
+            if ((uint32_t)lane < rest) hq[lane] = (uint16_t)moved;
+            nq = rest;
+            __syncthreads();
+        }
         k += kWave;
         while (k >= n) { k -= n; ++m; }
+    }
+    __syncthreads();
+    if ((uint32_t)lane < nq) {
+        const uint32_t e = hq[lane];
+        lds_unite(lp, (int32_t)(e & 0xffu), (int32_t)(e >> 8));
     }
     __syncthreads();
     for (uint32_t i = lane; i < n; i += kWave) parent[s + i] = (int32_t)s + lds_find(lp, (int32_t)i);
@@ -782,11 +815,12 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
                                                       LinkParams P) {
     __shared__ float4 pos[kMidMax];
     __shared__ int32_t lp[kMidMax];
+    __shared__ uint16_t hq[kIntraQueue];
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     const uint32_t s = cell_ptr[c], n = cell_ptr[c + 1] - s;
-    if (g.is_periodic) cell_intra<true>(s, n, P, xc, parent, pos, lp);
-    else cell_intra<false>(s, n, P, xc, parent, pos, lp);
+    if (g.is_periodic) cell_intra<true>(s, n, P, xc, parent, pos, lp, hq);
+    else cell_intra<false>(s, n, P, xc, parent, pos, lp, hq);
 }
 
 // One wave per mid cell A (kAuxMid): every pair of A's members with a
@@ -805,9 +839,24 @@ template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
-                          uint16_t* ia, uint32_t* ib, uint8_t* fa) {
+                          uint16_t* ia, uint32_t* ib, uint8_t* fa, uint2* uq) {
     const int lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
+    // linked pairs, queued and joined one per lane once kWave are waiting (a
+    // round of pair tests links a few pairs of 64)
+    uint32_t nq = 0;
+    auto drain = [&](bool all) {
+        __syncthreads();
+        const uint32_t take = all ? nq : (uint32_t)kWave;
+        const uint32_t rest = nq - take;
+        const uint2 e = (uint32_t)lane < take ? uq[lane] : make_uint2(0u, 0u);
+        const uint2 moved = (uint32_t)lane < rest ? uq[take + lane] : make_uint2(0u, 0u);
+        if ((uint32_t)lane < take) dsu_unite(parent, (int32_t)e.x, (int32_t)e.y);
+        __syncthreads();
+        if ((uint32_t)lane < rest) uq[lane] = moved;
+        nq = rest;
+        __syncthreads();
+    };
     const uint32_t lc = c - g.cell_off;
     int32_t ijk[3];
     double lo[3];
@@ -885,12 +934,19 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         auto flush = [&](uint32_t nb) {
             __syncthreads();
             if (!denseB) {
-                for (uint32_t t = lane; t < na * nb; t += kWave) {
-                    const uint32_t i = t / nb, j = t - i * nb;
-                    if (pair_linked_screened<PERIODIC>(pa[ia[i]], pb[j], P)) {
-                        LinkCtx L{parent, (int32_t)(sA + ia[i]), 0u};
-                        L.link((int32_t)(sB + ib[j]));
+                for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
+                    const uint32_t t = t0 + (uint32_t)lane;
+                    bool hit = false;
+                    uint32_t i = 0, j = 0;
+                    if (t < na * nb) {
+                        i = t / nb;
+                        j = t - i * nb;
+                        hit = pair_linked_screened<PERIODIC>(pa[ia[i]], pb[j], P);
                     }
+                    const uint64_t hits = __ballot(hit);
+                    if (hit) uq[nq + (uint32_t)__popcll(hits & below)] = make_uint2(sA + ia[i], sB + ib[j]);
+                    nq += (uint32_t)__popcll(hits);
+                    if (nq >= (uint32_t)kWave) drain(false);
                 }
             } else {
                 for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
@@ -904,16 +960,19 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                         xb = __float_as_int(q.w);
                         if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked_screened<PERIODIC>(a, q, P);
                     }
+                    // each distinct parent pair of the round is queued once
                     uint64_t m = __ballot(hit);
                     while (m) {
                         const int l = __ffsll((long long)m) - 1;
                         const int32_t la = __shfl(xa, l, kWave), lb = __shfl(xb, l, kWave);
-                        if (lane == l) dsu_unite(parent, la, lb);
+                        if (lane == 0) uq[nq] = make_uint2((uint32_t)la, (uint32_t)lb);
+                        ++nq;
                         ja = la;
                         jb = lb;
                         hit = hit && !(xa == la && xb == lb);
                         m = __ballot(hit);
                     }
+                    if (nq >= (uint32_t)kWave) drain(false);
                 }
             }
             __syncthreads();
@@ -947,6 +1006,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         }
         if (nb) flush(nb);
     }
+    if (nq) drain(true);
 }
 
 __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict__ mid_cells,
@@ -960,12 +1020,13 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
     __shared__ uint16_t ia[kMidMax];
     __shared__ uint32_t ib[kWave];
     __shared__ uint8_t fa[kMidMax];
+    __shared__ uint2 uq[2 * kWave];
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa);
+        cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa, uq);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa);
+        cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa, uq);
 }
 
 template <bool PERIODIC>
@@ -1015,22 +1076,29 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
     if (staged)
         for (uint32_t t = threadIdx.x; t < n_c; t += kBlk) pos[t] = xc[sub_list[cs + t]];
     __syncthreads();
+    // list entries: the sub-cell index (12 bits, K <= 4096) and its three
+    // coordinates (4 bits each, k <= 16), found once per sub-cell here
     for (int i = threadIdx.x; i < K; i += kBlk) {
         lp[i] = i;
         mh[i] = 0xffffffffu;
-        if (rp[i + 1] > rp[i]) list[atomicAdd(&nl[0], 1u)] = (uint32_t)i;
+        if (rp[i + 1] > rp[i]) {
+            const uint32_t sx = (uint32_t)(i % k0), sy = (uint32_t)((i / k0) % k1), sz = (uint32_t)(i / (k0 * k1));
+            list[atomicAdd(&nl[0], 1u)] = (uint32_t)i | (sx << 12) | (sy << 16) | (sz << 20);
+        }
     }
     __syncthreads();
     const uint32_t na = nl[0];
-    int cls_begin = 0;
-    for (int cls = 0; cls < 5; ++cls) {
-        const uint32_t noff = (uint32_t)(kFwdClassEnd[cls] - cls_begin);
+    // one class of offsets [b0, b1): the class sizes are constants once the
+    // calls below are inlined, so the (entry, offset) split costs no division
+    auto offset_class = [&](const int b0, const int b1) {
+        const uint32_t noff = (uint32_t)(b1 - b0);
         for (uint32_t it = threadIdx.x; it < na * noff; it += kBlk) {
-            const int32_t A = (int32_t)list[it / noff];
-            const int oi = cls_begin + (int)(it % noff);
-            const int32_t ax = A % k0 + c_fwd_off[oi][0];
-            const int32_t ay = (A / k0) % k1 + c_fwd_off[oi][1];
-            const int32_t az = A / (k0 * k1) + c_fwd_off[oi][2];
+            const uint32_t ent = list[it / noff];
+            const int oi = b0 + (int)(it % noff);
+            const int32_t A = (int32_t)(ent & 0xfffu);
+            const int32_t ax = (int32_t)((ent >> 12) & 15u) + c_fwd_off[oi][0];
+            const int32_t ay = (int32_t)((ent >> 16) & 15u) + c_fwd_off[oi][1];
+            const int32_t az = (int32_t)((ent >> 20) & 15u) + c_fwd_off[oi][2];
             if (ax < 0 || ax >= k0 || ay < 0 || ay >= k1 || az < 0 || az >= k2) continue;
             const int32_t B = ax + k0 * (ay + k1 * az);
             if (rp[B + 1] == rp[B]) continue;
@@ -1048,11 +1116,18 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
             }
             if (hit) lds_unite(lp, A, B);
         }
-        cls_begin = kFwdClassEnd[cls];
         __syncthreads();
-    }
+    };
+    static_assert(kFwdClassEnd[0] == 3 && kFwdClassEnd[1] == 9 && kFwdClassEnd[2] == 13 &&
+                      kFwdClassEnd[3] == 16 && kFwdClassEnd[4] == 62,
+                  "offset classes");
+    offset_class(0, 3);
+    offset_class(3, 9);
+    offset_class(9, 13);
+    offset_class(13, 16);
+    offset_class(16, 62);
     for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
-        const int32_t A = (int32_t)list[i];
+        const int32_t A = (int32_t)(list[i] & 0xfffu);
         // head of A: its smallest position (set as parent of every member)
         const uint32_t h = (uint32_t)parent[sub_list[cs + rp[A]]];
         const int32_t r = lds_find(lp, A);
@@ -1064,14 +1139,14 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
     for (int i = threadIdx.x; i < K; i += kBlk)
         if (rp[i + 1] == rp[i]) srow[i] = -1;
     for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
-        const int32_t A = (int32_t)list[i];
+        const int32_t A = (int32_t)(list[i] & 0xfffu);
         const int32_t root = (int32_t)mh[lds_find(lp, A)];
         srow[A] = root;
         for (uint32_t t = rp[A]; t < rp[A + 1]; ++t) parent[sub_list[cs + t]] = root;
     }
     if (threadIdx.x == 0) {
         ncomp[d] = (int32_t)nl[1];
-        croot[d] = nl[1] == 1 ? (int32_t)mh[lds_find(lp, (int32_t)list[0])] : -1;
+        croot[d] = nl[1] == 1 ? (int32_t)mh[lds_find(lp, (int32_t)(list[0] & 0xfffu))] : -1;
         needs_sub[d] = nl[1] != 1 ? 1 : 0;
     }
 }
@@ -1845,16 +1920,24 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
 
         // ---- linking. k_dense_intra and k_cell_intra write the parents of
         // dense-clique and mid-cell members directly (local components), so
-        // they run before any atomic union on those members. k_link touches
-        // neither kind (it joins sparse particles, and the dense-cell members
-        // of regions without cliques, which no plain store writes), so it
-        // starts at once on the second stream, beside them.
-        hipStream_t s2 = C.stream2;
+        // every atomic union that may reach those members waits for them.
+        // Three streams: the sparse-particle kernels on s2, the mid-cell
+        // kernels on s3, the dense-cell kernels on s. k_link touches neither
+        // kind of member (it joins sparse particles, and the dense-cell
+        // members of regions without cliques, which no plain store writes),
+        // so it starts at once. The dense-cell pair kernels join dense clique
+        // members only, whose union-find paths stay among dense members until
+        // k_cell_pair or k_link_dense joins another kind: they wait for
+        // k_dense_intra alone, beside k_cell_intra. k_cell_pair and
+        // k_link_dense join mid or sparse members to dense ones and wait for
+        // both intra-cell kernels.
+        hipStream_t s2 = C.stream2, s3 = C.stream3;
         // per-particle masks of dense cells to scan, in the buffer that holds
         // the labels after linking
         uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
         PYFOF_HIP(hipEventRecord(C.ev_fork, s));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
+        PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_fork, 0));
         k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
@@ -1871,28 +1954,26 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                     P);
             PYFOF_LAUNCH_CHECK();
         }
+        PYFOF_HIP(hipEventRecord(C.ev_dense, s));
         if (nmid > 0) {
-            k_cell_intra<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
+            k_cell_intra<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
-        // Every parent written by a plain store (the clique pre-unions,
-        // k_dense_intra, k_cell_intra) is in place; from here on every kernel
-        // joins components only by atomic unions, whose order does not change
-        // the partition. The sparse-particle kernels on the second stream
-        // share the GPU with the mid- and dense-cell kernels: two kinds of
-        // latency-bound waves (register-heavy and LDS-heavy).
-        PYFOF_HIP(hipEventRecord(C.ev_intra, s));
-        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_intra, 0));
+        PYFOF_HIP(hipEventRecord(C.ev_mid, s3));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_dense, 0));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_mid, 0));
         if (ndense > 0) {
             k_link_dense<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
                                                    d_reg, parent, dmask, P);
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipEventRecord(C.ev_join, s2));
+        PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_dense, 0));
         if (nmid > 0) {
-            k_cell_pair<<<nmid, kWave, 0, s>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
+            k_cell_pair<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
+        PYFOF_HIP(hipEventRecord(C.ev_pair, s3));
         if (nsubs > 0) {
             k_dense_cell_cross<<<ndense, kWave, 0, s>>>(dense_cells, ndense, cell_ptr, d_coff, d_reg, H,
                                                         aux, ncomp, croot, needs_sub, sub_ptr, sub_list,
@@ -1903,6 +1984,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                        xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
+        PYFOF_HIP(hipStreamWaitEvent(s, C.ev_pair, 0));
         PYFOF_HIP(hipStreamWaitEvent(s, C.ev_join, 0));
         C.mark(kEvLink);
 
@@ -2133,6 +2215,7 @@ int32_t fof_entry(Ctx* C, const CallArgs& A, int32_t n_threads) {
     } catch (const HipError& e) {
         set_error(e.what());
         (void)hipStreamSynchronize(C->stream2);
+        (void)hipStreamSynchronize(C->stream3);
         (void)hipStreamSynchronize(C->stream);
         err = e.code == hipErrorOutOfMemory ? IERR_ALLOC : IERR_HIP;
     } catch (const std::bad_alloc&) {
