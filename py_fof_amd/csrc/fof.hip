@@ -1510,20 +1510,10 @@ __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t*
     }
 }
 
-// Candidates, fof_fortran.f90:866-872: every CRS position whose component
-// size (0 for non-roots) reaches np_min, in position order. With
-// np_halo_min <= 0 this counts non-root positions too (empty candidates).
-__global__ void k_cand_flags(const uint32_t* __restrict__ size, uint32_t n, int32_t np_min,
-                             uint32_t* __restrict__ flag) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    flag[p] = ((int64_t)size[p] >= (int64_t)np_min) ? 1u : 0u;
-}
-
 // in_halo / halo_id per particle (fof_fortran.f90:209-223); per candidate the
 // anchor particle (its first CRS member) and its region
 __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
-                        const uint32_t* __restrict__ flag, const uint32_t* __restrict__ cand_idx,
+                        const uint32_t* __restrict__ cand_idx,
                         const uint32_t* __restrict__ perm, const float4* __restrict__ xc,
                         const RegionDev* __restrict__ reg, uint32_t n, int32_t np_min,
                         int32_t* __restrict__ halo_id,
@@ -1539,7 +1529,9 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     // (k_in_halo)
     if (cand) halo_id[i] = (int32_t)cand_idx[r] + 1;
     if (root_out) root_out[i] = reg[__float_as_int(xc[p].w)].skip ? -1 : (int32_t)perm[r];
-    if (flag[p]) {
+    // candidates are the positions whose size (0 for non-roots) reaches
+    // np_min (fof_fortran.f90:866-872), numbered by cand_idx
+    if ((int64_t)size[p] >= (int64_t)np_min) {
         anchor[cand_idx[p]] = i;
         hregion[cand_idx[p]] = __float_as_int(xc[p].w);
     }
@@ -1991,15 +1983,14 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // ---- roots, sizes, candidate numbering, outputs
         int32_t* label = C.label.as<int32_t>(N);
         uint32_t* size = C.size.as<uint32_t>(N);
-        uint32_t* flag = C.flag.as<uint32_t>(N);
         uint32_t* cidx = C.cidx.as<uint32_t>(N);
         k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, size, N);
         PYFOF_LAUNCH_CHECK();
         k_count<<<nblk((N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(label, N, size);
         PYFOF_LAUNCH_CHECK();
-        k_cand_flags<<<nblk(N), kBlk, 0, s>>>(size, N, A.np_min, flag);
-        PYFOF_LAUNCH_CHECK();
-        exclusive_scan_u32(flag, cidx, N, scal + 1, tmp, s);
+        // candidates (fof_fortran.f90:866-872): positions whose component
+        // size (0 for non-roots) reaches np_min, numbered in position order
+        exclusive_count_at_least_u32(size, A.np_min, cidx, N, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
         int32_t* o_id = dev_mem ? A.halo_id : C.out_id.as<int32_t>(N);
         int32_t* o_root = nullptr;
@@ -2008,7 +1999,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         int32_t* hreg = C.hregion.as<int32_t>(N);
         k_fill_u32<<<nblk(N), kBlk, 0, s>>>(reinterpret_cast<uint32_t*>(o_id), N, 0u);
         PYFOF_LAUNCH_CHECK();
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, flag, cidx, perm, xc, d_reg, N, A.np_min, o_id,
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, xc, d_reg, N, A.np_min, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
         k_in_halo<<<nblk(N), kBlk, 0, s>>>(o_id, N, o_in);

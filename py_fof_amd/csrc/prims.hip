@@ -27,6 +27,17 @@ struct OpMax {
     static __device__ uint32_t f(uint32_t a, uint32_t b) { return a > b ? a : b; }
 };
 
+// Scan inputs: the array itself, or a 0/1 predicate on it.
+struct InPlain {
+    const uint32_t* p;
+    __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
+};
+struct InAtLeast {  // 1 where (int64) p[i] >= thr
+    const uint32_t* p;
+    int64_t thr;
+    __device__ uint32_t operator()(uint64_t i) const { return (int64_t)p[i] >= thr ? 1u : 0u; }
+};
+
 template <class Op>
 __device__ inline uint32_t wave_reduce(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = Op::f(v, __shfl_down(v, o, kWave));
@@ -43,8 +54,8 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+template <class Op, class In>
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const In in, uint64_t n,
                                                         uint32_t* __restrict__ partial) {
     __shared__ uint32_t wsum[kBlock / kWave];
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
@@ -52,7 +63,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const uint32_t* __restri
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
         uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        if (i < n) s = Op::f(s, in[i]);
+        if (i < n) s = Op::f(s, in(i));
     }
     s = wave_reduce<Op>(s);
     if (__lane_id() == 0) wsum[threadIdx.x / kWave] = s;
@@ -69,8 +80,8 @@ __device__ inline int pad_idx(int e) { return e + (e >> 4); }
 
 // Scans one tile per block. offsets[block] (exclusive prefix of the tiles
 // before it) is folded in; inclusive selects the inclusive form.
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n,
+template <class Op, class In>
+__global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out, uint64_t n,
                                                       const uint32_t* __restrict__ offsets,
                                                       uint32_t* __restrict__ total, int inclusive) {
     __shared__ uint32_t tile[kTile + kTile / 16];
@@ -81,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const uint32_t* in, uint32
     for (int j = 0; j < kIpt; ++j) {
         int e = j * kBlock + t;
         uint64_t i = base + e;
-        tile[pad_idx(e)] = i < n ? in[i] : Op::id();
+        tile[pad_idx(e)] = i < n ? in(i) : Op::id();
     }
     __syncthreads();
     uint32_t loc[kIpt];
@@ -237,22 +248,22 @@ size_t scan_levels_bytes(uint64_t n) {
     return b + 256;
 }
 
-template <class Op>
-void scan_rec(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, uint32_t* temp,
+template <class Op, class In>
+void scan_rec(const In in, uint32_t* out, uint64_t n, uint32_t* total, uint32_t* temp,
               int inclusive, hipStream_t s) {
     if (n == 0) return;
     const uint32_t nb = div_up(n, kTile);
     if (nb == 1) {
-        k_scan_down<Op><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total, inclusive);
+        k_scan_down<Op, In><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total, inclusive);
         PYFOF_LAUNCH_CHECK();
         return;
     }
     uint32_t* partial = temp;
     uint32_t* next = temp + ((nb + 63) / 64) * 64;
-    k_scan_reduce<Op><<<nb, kBlock, 0, s>>>(in, n, partial);
+    k_scan_reduce<Op, In><<<nb, kBlock, 0, s>>>(in, n, partial);
     PYFOF_LAUNCH_CHECK();
-    scan_rec<Op>(partial, partial, nb, nullptr, next, 0, s);
-    k_scan_down<Op><<<nb, kBlock, 0, s>>>(in, out, n, partial, total, inclusive);
+    scan_rec<Op>(InPlain{partial}, partial, nb, nullptr, next, 0, s);
+    k_scan_down<Op, In><<<nb, kBlock, 0, s>>>(in, out, n, partial, total, inclusive);
     PYFOF_LAUNCH_CHECK();
 }
 
@@ -274,12 +285,17 @@ size_t radix_temp_bytes(uint64_t n) {
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* temp,
                         hipStream_t s) {
-    scan_rec<OpSum>(in, out, n, total, static_cast<uint32_t*>(temp), 0, s);
+    scan_rec<OpSum>(InPlain{in}, out, n, total, static_cast<uint32_t*>(temp), 0, s);
+}
+
+void exclusive_count_at_least_u32(const uint32_t* in, int64_t thr, uint32_t* out, uint64_t n,
+                                  uint32_t* total, void* temp, hipStream_t s) {
+    scan_rec<OpSum>(InAtLeast{in, thr}, out, n, total, static_cast<uint32_t*>(temp), 0, s);
 }
 
 void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp,
                             hipStream_t s) {
-    scan_rec<OpMax>(in, out, n, nullptr, static_cast<uint32_t*>(temp), 1, s);
+    scan_rec<OpMax>(InPlain{in}, out, n, nullptr, static_cast<uint32_t*>(temp), 1, s);
 }
 
 bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
@@ -294,7 +310,7 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32
     for (int shift = 0; shift < bits; shift += kRadixBits) {
         k_radix_hist<<<ntiles, kBlock, 0, s>>>(ik, n, shift, hist, ntiles);
         PYFOF_LAUNCH_CHECK();
-        scan_rec<OpSum>(hist, hist, m, nullptr, stemp, 0, s);
+        scan_rec<OpSum>(InPlain{hist}, hist, m, nullptr, stemp, 0, s);
         k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);
         PYFOF_LAUNCH_CHECK();
         uint32_t* tk = ik; ik = ok; ok = tk;
