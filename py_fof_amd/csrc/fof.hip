@@ -222,9 +222,11 @@ __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
 
 // marks cells with at least kDenseMin particles in regions with sub-cells
 // aux[c]: the dense index of a dense cell (>= 0); kAuxMid for a cell whose
-// own pairs k_cell_intra resolves; -1 otherwise. Negative values read as
-// "not dense" everywhere.
+// own pairs k_cell_intra resolves; kAuxNearDense for any other cell with a
+// dense neighbour in a clique region (k_mark_near_dense); -1 otherwise.
+// Negative values read as "not dense" everywhere.
 constexpr int32_t kAuxMid = -2;
+constexpr int32_t kAuxNearDense = -3;
 constexpr uint32_t kMidMin = 8, kMidMax = 255;  // particles of a k_cell_intra cell
 constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
 
@@ -393,7 +395,7 @@ struct LinkCtx {
     int32_t rp;  // last known root of the thread's particle
     uint32_t dmask;  // neighbour offsets (bit o) whose dense cells are left to k_link_dense
     __device__ __forceinline__ void link(int32_t q) {
-        const int32_t rq = dsu_find(parent, q);
+        const int32_t rq = dsu_find_near(parent, q);
         if (rq != rp) rp = dsu_unite(parent, rp, rq);
     }
 };
@@ -424,26 +426,19 @@ __device__ __forceinline__ bool pair_linked_screened(const float4& a, const floa
 
 // Tests the particle `me` (position p) against cell jc, reached by neighbour
 // offset o (13: its own cell). For the own cell only partners q > p are
-// tested. Pruning is exact: a pair passing the predicate is within b (+ float
-// rounding < m) per axis, so its partner lies inside the query box
-// [x - R, x + R] with R = b + m; the cell and sub-cell coordinates used to
-// prune are monotone in x. A cell with sub-cells (dense) is not scanned here:
-// bit o of the thread's mask is set and k_link_dense scans it, which keeps the
-// register footprint of this kernel, and so its occupancy, low. A neighbouring
-// mid cell is skipped: k_cell_pair takes its pairs with this cell.
+// tested. The caller visits only the neighbours the query box can reach
+// (link_particle); every pair is decided by the predicate, so a cell visited
+// needlessly costs time only. A cell with sub-cells (dense) is not scanned
+// here: bit o of the thread's mask is set and k_link_dense scans it, which
+// keeps the register footprint of this kernel, and so its occupancy, low. A
+// neighbouring mid cell is skipped: k_cell_pair takes its pairs with this
+// cell.
 template <bool PERIODIC>
-__device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o,
-                                           const int32_t jijk[3], const double q[3], bool own, bool back,
-                                           const RegionDev& g, const LinkParams& P,
+__device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_t jc, int o, bool own,
+                                           bool back, const LinkParams& P,
                                            const uint32_t* __restrict__ cell_ptr,
                                            const int32_t* __restrict__ aux,
                                            const float4* __restrict__ xc, LinkCtx& L) {
-    if (g.prune && !own) {
-        for (int d = 0; d < 3; ++d) {
-            const double lo = g.origin[d] + (double)jijk[d] * g.cell[d];
-            if (q[d] + P.R < lo - P.m || q[d] - P.R > lo + g.cell[d] + P.m) return;
-        }
-    }
     // class and member range loaded together, before any branch on them
     const int32_t cls = aux[jc];
     const uint32_t s = cell_ptr[jc], e = cell_ptr[jc + 1];
@@ -464,8 +459,8 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
         const bool h0 = pair_linked<PERIODIC>(me, y0, P), h1 = pair_linked<PERIODIC>(me, y1, P);
         if (!(h0 || h1)) continue;
         const int32_t q0 = (int32_t)qq;
-        const int32_t a0 = h0 ? atomic_load_relaxed(&L.parent[q0]) : 0;
-        const int32_t a1 = h1 ? atomic_load_relaxed(&L.parent[q0 + 1]) : 0;
+        const int32_t a0 = h0 ? load_near(&L.parent[q0]) : 0;
+        const int32_t a1 = h1 ? load_near(&L.parent[q0 + 1]) : 0;
         if (h0 && a0 != L.rp) L.link(q0);
         if (h1 && a1 != L.rp) L.link(q0 + 1);
     }
@@ -497,6 +492,32 @@ __device__ __forceinline__ uint32_t local_cell(const RegionDev& g, const int32_t
     return (uint32_t)jijk[0] + (uint32_t)g.n[0] * ((uint32_t)jijk[1] + (uint32_t)g.n[1] * (uint32_t)jijk[2]);
 }
 
+// Marks the plain cells (aux -1) next to a dense cell of a clique region as
+// kAuxNearDense: only their particles need k_link's visits of backward
+// neighbour cells, which look for dense cells alone (k_link_dense takes
+// those pairs from the sparse side). Every writer stores the same value.
+__global__ void __launch_bounds__(kBlk) k_mark_near_dense(const uint32_t* __restrict__ dense_cells, uint32_t ndense,
+                                                          const uint32_t* __restrict__ cell_off,
+                                                          const RegionDev* __restrict__ reg, int32_t nreg,
+                                                          int32_t* __restrict__ aux, double L) {
+    const uint32_t d = blockIdx.x * kBlk + threadIdx.x;
+    if (d >= ndense) return;
+    const uint32_t c = dense_cells[d];
+    const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
+    if (!g.clique) return;
+    double lo[3];
+    int32_t ijk[3];
+    cell_lower_corner(g, c - g.cell_off, lo, ijk);
+    for (int o = 0; o < 27; ++o) {
+        if (o == 13) continue;
+        int32_t jijk[3];
+        double sh[3];
+        if (!neighbour_cell(g, ijk, o, jijk, sh, L)) continue;
+        const uint32_t jc = g.cell_off + local_cell(g, jijk);
+        if (aux[jc] == -1) aux[jc] = kAuxNearDense;
+    }
+}
+
 template <bool PERIODIC>
 __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, const RegionDev& g,
                                                   uint32_t c, const LinkParams& P,
@@ -504,15 +525,24 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
                                                   const int32_t* __restrict__ aux,
                                                   const float4* __restrict__ xc, int32_t* parent) {
     // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
-    if (aux[c] == kAuxMid) return 0u;
+    const int32_t own_cls = aux[c];
+    if (own_cls == kAuxMid) return 0u;
+    // backward neighbours are visited only for their dense cells, which a
+    // cell not marked kAuxNearDense does not have (clique regions only)
+    const bool near_dense = g.clique && own_cls == kAuxNearDense;
     LinkCtx L{parent, (int32_t)p, 0u};  // rp becomes a root at the first unite
     const uint32_t lc = c - g.cell_off;
     int32_t ijk[3];
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
     const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
-    visit_cell<PERIODIC>(me, p, c, 13, ijk, x0, true, false, g, P, cell_ptr, aux, xc, L);
+    visit_cell<PERIODIC>(me, p, c, 13, true, false, P, cell_ptr, aux, xc, L);
+    // a cell off the periodic edges has no wrapped neighbours: offsets below
+    // 13 (lexicographically negative) are then exactly the backward cells
+    bool edge = false;
+    for (int d = 0; d < 3; ++d) edge = edge || (g.periodic[d] && (ijk[d] == 0 || ijk[d] == g.n[d] - 1));
     auto neighbour = [&](int o) {
+        if (o < 13 && !near_dense && !edge) return;
         int32_t jijk[3];
         double sh[3];
         if (!neighbour_cell(g, ijk, o, jijk, sh, P.L)) return;
@@ -520,7 +550,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
         // sparse-sparse pairs once (forward cells); every pair with a dense
         // clique cell is taken here, from the sparse side, in both directions
         const bool fwd = jl > lc;
-        if (!fwd && !g.clique) return;
+        if (!fwd && !near_dense) return;
         if (!g.prune) {
             // tiny periodic grids: the same neighbour can be reached twice
             for (int o2 = 0; o2 < o; ++o2) {
@@ -530,8 +560,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
                 if (neighbour_cell(g, ijk, o2, j2, s2, P.L) && local_cell(g, j2) == jl) return;
             }
         }
-        const double q[3] = {x0[0] + sh[0], x0[1] + sh[1], x0[2] + sh[2]};
-        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, jijk, q, false, !fwd, g, P, cell_ptr, aux, xc, L);
+        visit_cell<PERIODIC>(me, p, g.cell_off + jl, o, false, !fwd, P, cell_ptr, aux, xc, L);
     };
     if (!g.prune) {
         for (int o = 0; o < 27; ++o)
@@ -539,10 +568,12 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
         return L.dmask;
     }
     // Offsets along each axis whose neighbour the query box [x - R, x + R] can
-    // reach: a superset of what the exact test in visit_cell keeps (its corners
-    // differ from these by f64 rounding, far below the extra margin m), so only
-    // about (1 + 2R / cell)^3 - 1 neighbours are visited instead of 26. A
-    // neighbour across a periodic edge is always taken and tested there.
+    // reach: a pair passing the predicate is within b (+ float rounding < m)
+    // per axis, so its partner lies inside the box with R = b + m, and the
+    // cell coordinates are monotone in x; the margin 2m covers the f64
+    // rounding of the cell corners. So only about (1 + 2R / cell)^3 - 1
+    // neighbours are visited instead of 26. A neighbour across a periodic
+    // edge is always visited.
     int32_t olo[3], ohi[3];
     for (int d = 0; d < 3; ++d) {
         const double c0 = g.origin[d] + (double)ijk[d] * g.cell[d];
@@ -626,7 +657,7 @@ __device__ void dense_entry(uint32_t p, int o, const float4& me, const RegionDev
     const uint32_t s = cell_ptr[jc];
     const int32_t dcell = aux[jc];
     const bool single = g.clique && !own && ncomp[dcell] == 1;
-    if (single && dsu_find(parent, croot[dcell]) == dsu_find(parent, (int32_t)p)) return;
+    if (single && dsu_find_near(parent, croot[dcell]) == dsu_find_near(parent, (int32_t)p)) return;
     const int32_t kk[3] = {g.k[0], g.k[1], g.k[2]};
     int32_t a[3], nb[3], mine[3] = {-1, -1, -1};
     for (int d = 0; d < 3; ++d) {
@@ -892,7 +923,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     // bit 2d+1 for the upper face
     for (uint32_t i = lane; i < nA; i += kWave) {
         const float4 a = xc[sA + i];
-        pa[i] = make_float4(a.x, a.y, a.z, __int_as_float(atomic_load_relaxed(&parent[sA + i])));
+        pa[i] = make_float4(a.x, a.y, a.z, __int_as_float(load_near(&parent[sA + i])));
         const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
         uint32_t f = 0;
         for (int d = 0; d < 3; ++d) {
@@ -999,7 +1030,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             if (ok) {
                 const uint32_t k = nb + (uint32_t)__popcll(m & below);
                 ib[k] = i;
-                if (denseB) q.w = __int_as_float(atomic_load_relaxed(&parent[sB + i]));
+                if (denseB) q.w = __int_as_float(load_near(&parent[sB + i]));
                 pb[k] = q;
             }
             nb += cnt;
@@ -1216,8 +1247,8 @@ __device__ void cell_cross(uint32_t d, const RegionDev& g, const LinkParams& P,
             if (lane == 0) needs_sub[d] = 1;
             continue;
         }
-        int32_t rA = dsu_find(parent, croot[d]);
-        int32_t rB = dsu_find(parent, croot[dB]);
+        int32_t rA = dsu_find_near(parent, croot[d]);
+        int32_t rB = dsu_find_near(parent, croot[dB]);
         if (rA == rB) continue;
         // Per axis, a sub-cell pair is a position and an option: along an axis
         // parallel to the shared face, A's sub-cell coordinate (kk positions)
@@ -1375,7 +1406,7 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
     const uint32_t* rowA = sub_ptr + (size_t)d * P.sub_stride;
     const uint32_t cs = cell_ptr[c];
     const uint32_t a0 = cs + rowA[si], a1 = cs + rowA[si + 1];
-    int32_t rA = dsu_find(parent, (int32_t)ent.z);
+    int32_t rA = dsu_find_near(parent, (int32_t)ent.z);
     // pairs of single-component dense cells are decided per cell pair by
     // k_dense_cell_cross. B's sub-cell is known by its local root (k_dense_
     // intra's component of that cell); the last kJoinedCache local roots found
@@ -1415,8 +1446,8 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
 #pragma unroll
                 for (int i = 0; i < kJoinedCache; ++i) known = known || joined[i] == lb;
                 if (known) continue;
-                const int32_t rB = dsu_find(parent, lb);
-                if (rB != rA) rA = dsu_find(parent, rA);
+                const int32_t rB = dsu_find_near(parent, lb);
+                if (rB != rA) rA = dsu_find_near(parent, rA);
                 if (rB == rA) {
                     remember(lb);
                     continue;
@@ -1894,6 +1925,10 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipMemcpyAsync(cnts, scal, sizeof(cnts), hipMemcpyDeviceToHost, s));
         PYFOF_HIP(hipStreamSynchronize(s));
         const uint32_t ndense = cnts[0], nmid = cnts[8];
+        if (ndense > 0) {
+            k_mark_near_dense<<<nblk(ndense), kBlk, 0, s>>>(dense_cells, ndense, d_coff, d_reg, H, aux, A.L);
+            PYFOF_LAUNCH_CHECK();
+        }
         uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
         uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);

@@ -99,8 +99,10 @@ __device__ __forceinline__ bool pred_flat(const float4& a, const float4& q, doub
 
 // Union-find over CRS positions; parent[x] == x marks a root. Parents always
 // point to smaller positions, so a component's root is its first CRS member
-// (atomic_dsu.c:31-35). Reads bypass the per-CU cache (agent-scope atomics)
-// so a retry after a failed CAS sees the hook that made it fail.
+// (atomic_dsu.c:31-35): the root of every tree is its minimum position.
+//
+// dsu_find reads at agent scope (bypassing the per-CU cache): every value is
+// the current one. The relabel uses it, after linking, for the exact roots.
 __device__ __forceinline__ int32_t dsu_find(int32_t* parent, int32_t x) {
     for (;;) {
         const int32_t px = atomic_load_relaxed(&parent[x]);
@@ -112,13 +114,46 @@ __device__ __forceinline__ int32_t dsu_find(int32_t* parent, int32_t x) {
     }
 }
 
+// A relaxed load at workgroup scope: served by the CU's vector L1 when the
+// line is there, so it may return an older value than another CU's latest
+// store to *p.
+__device__ __forceinline__ int32_t load_near(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// dsu_find for the linking kernels, through the L1. An older parent value is
+// still a smaller position of the same component (components only merge and
+// pointers only decrease), so the result is a node of x's component, at most
+// x, that was a root when its line was cached; path halving with such values
+// still stores pointers to smaller positions of the component.
+__device__ __forceinline__ int32_t dsu_find_near(int32_t* parent, int32_t x) {
+    for (;;) {
+        const int32_t px = load_near(&parent[x]);
+        if (px == x) return x;
+        const int32_t gp = load_near(&parent[px]);
+        if (gp == px) return px;
+        __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+    }
+}
+
+// Joins the components of a and b; returns a node of the joined component.
+// The hook is a CAS at L2 that succeeds only on a current root. Equal finds
+// mean one component. A hook of a current root hi under lo < hi is always
+// valid: hi is its component's minimum, so lo lies in the other component.
+// A failed CAS returns hi's current parent (read at L2), and the search goes
+// on from there: every failure lowers a + b, so the loop ends even when the
+// L1 still holds older values.
 __device__ __forceinline__ int32_t dsu_unite(int32_t* parent, int32_t a, int32_t b) {
     for (;;) {
-        a = dsu_find(parent, a);
-        b = dsu_find(parent, b);
+        a = dsu_find_near(parent, a);
+        b = dsu_find_near(parent, b);
         if (a == b) return a;
         const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
-        if (atomicCAS(&parent[hi], hi, lo) == hi) return lo;
+        const int32_t old = atomicCAS(&parent[hi], hi, lo);
+        if (old == hi) return lo;
+        if (a == hi) a = old;
+        else b = old;
     }
 }
 
