@@ -47,6 +47,18 @@ __global__ void k_iota_u32(uint32_t* p, uint64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Face bits of a coordinate x (the f32 CRS coordinate widened) in a cell
+// [lo, lo + cell) along one axis: bit 0 when x lies within R = b + m of the
+// lower face, bit 1 within R of the upper face, both with the rounding
+// margin 2m. A partner across a face can only be linked from a member that
+// has its bit (a linked pair is within b + float rounding < R per axis, and
+// the cell coordinates are monotone in x). Computed once per particle in
+// k_keys, with the same f64 expressions the linking kernels would use.
+__device__ __forceinline__ uint32_t face_bits(double x, double lo, double cell, double R, double m) {
+    return (x - R <= lo + 2.0 * m ? 1u : 0u) | (x + R >= lo + cell - 2.0 * m ? 2u : 0u);
+}
+
+// ---------------------------------------------------------------------------
 // per-region bounding box of x_rel (fof_fortran.f90:524-531)
 __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
                                                const Chunk* __restrict__ chunks,
@@ -89,29 +101,30 @@ __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
 }
 
 // reference cell id per particle (fof_fortran.f90:580-585, 0-based global id)
-// and, in input order, its x_crs = real(x_rel, 4) (fof_fortran.f90:608) with
-// the region in w: the one coalesced read of xp the binning makes after the
-// bounding box; k_gather then moves 16 B per particle instead of re-reading
-// the 24 B of xp at random.
+// and, in input order, its x_crs = real(x_rel, 4) (fof_fortran.f90:608): the
+// one coalesced read of xp the binning makes after the bounding box; k_gather
+// then moves 16 B per particle instead of re-reading the 24 B of xp at random.
+// The fourth component holds the particle's face bits (face_bits) as an
+// integer; the linking kernels find a particle's region from its cell.
 __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
                                                const Chunk* __restrict__ chunks,
                                                const RegionDev* __restrict__ reg,
                                                uint32_t* __restrict__ keys,
                                                uint32_t* __restrict__ vals,
-                                               float4* __restrict__ xr, double L, double Lh) {
+                                               float4* __restrict__ xr, double L, double Lh, double R,
+                                               double m) {
     const Chunk ch = chunks[blockIdx.x];
     const RegionDev& g = reg[ch.region];
     const uint32_t off = g.cell_off;
     const size_t a = (size_t)g.start;
     const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
-    const float wr = __int_as_float(ch.region);
     if (g.skip) {
         for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
             keys[i] = off;
             vals[i] = (uint32_t)i;
             xr[i] = make_float4((float)rel_coord(xp[3 * (size_t)i], xa0, L, Lh),
                                 (float)rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh),
-                                (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), wr);
+                                (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), 0.f);
         }
         return;
     }
@@ -127,7 +140,11 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
         const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
         keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
         vals[i] = (uint32_t)i;
-        xr[i] = make_float4((float)v0, (float)v1, (float)v2, wr);
+        const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+        const uint32_t fb = face_bits((double)f0, o0 + (double)(ix - 1) * c0, c0, R, m) |
+                            (face_bits((double)f1, o1 + (double)(iy - 1) * c1, c1, R, m) << 2) |
+                            (face_bits((double)f2, o2 + (double)(iz - 1) * c2, c2, R, m) << 4);
+        xr[i] = make_float4(f0, f1, f2, __uint_as_float(fb));
     }
 }
 
@@ -535,7 +552,6 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     int32_t ijk[3];
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
-    const double x0[3] = {(double)me.x, (double)me.y, (double)me.z};
     visit_cell<PERIODIC>(me, p, c, 13, true, false, P, cell_ptr, aux, xc, L);
     // a cell off the periodic edges has no wrapped neighbours: offsets below
     // 13 (lexicographically negative) are then exactly the backward cells
@@ -574,13 +590,14 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     // rounding of the cell corners. So only about (1 + 2R / cell)^3 - 1
     // neighbours are visited instead of 26. A neighbour across a periodic
     // edge is always visited.
+    // (the face bits k_keys stored in .w)
+    const uint32_t fb = __float_as_uint(me.w);
     int32_t olo[3], ohi[3];
     for (int d = 0; d < 3; ++d) {
-        const double c0 = g.origin[d] + (double)ijk[d] * g.cell[d];
         const bool wrap_lo = g.periodic[d] && ijk[d] == 0;
         const bool wrap_hi = g.periodic[d] && ijk[d] == g.n[d] - 1;
-        olo[d] = (wrap_lo || x0[d] - P.R <= c0 + 2.0 * P.m) ? -1 : 0;
-        ohi[d] = (wrap_hi || x0[d] + P.R >= c0 + g.cell[d] - 2.0 * P.m) ? 1 : 0;
+        olo[d] = (wrap_lo || ((fb >> (2 * d)) & 1u)) ? -1 : 0;
+        ohi[d] = (wrap_hi || ((fb >> (2 * d)) & 2u)) ? 1 : 0;
     }
     for (int oz = olo[2]; oz <= ohi[2]; ++oz)
         for (int oy = olo[1]; oy <= ohi[1]; ++oy)
@@ -599,12 +616,14 @@ __global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
                                                const uint32_t* __restrict__ skeys,
                                                const uint32_t* __restrict__ cell_ptr,
                                                const int32_t* __restrict__ aux,
-                                               const RegionDev* __restrict__ reg, int32_t* parent,
+                                               const RegionDev* __restrict__ reg,
+                                               const uint32_t* __restrict__ cell_off, int32_t* parent,
                                                uint32_t* __restrict__ dmask, LinkParams P) {
     const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
     if (p >= P.N) return;
     const float4 me = xc[p];
-    const int32_t rid = __float_as_int(me.w);
+    const uint32_t c = skeys[p];
+    const int32_t rid = P.nregions == 1 ? 0 : (int32_t)region_of_cell(cell_off, P.nregions, c);
     uint32_t mask = 0;
     // One pass per distinct region in the wave (almost always one): the
     // region record is then wave-uniform and read with scalar loads.
@@ -615,7 +634,6 @@ __global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
         pending = false;
         const RegionDev& g = reg[r0];
         if (g.skip) break;
-        const uint32_t c = skeys[p];
         if (g.clique && aux[c] >= 0) break;  // dense clique cells: k_dense_* kernels
         mask = g.is_periodic ? link_particle<true>(p, me, g, c, P, cell_ptr, aux, xc, parent)
                              : link_particle<false>(p, me, g, c, P, cell_ptr, aux, xc, parent);
@@ -716,7 +734,8 @@ __global__ void __launch_bounds__(kBlk) k_link_dense(const float4* __restrict__ 
                                                      const uint32_t* __restrict__ sub_list,
                                                      const int32_t* __restrict__ ncomp,
                                                      const int32_t* __restrict__ croot,
-                                                     const RegionDev* __restrict__ reg, int32_t* parent,
+                                                     const RegionDev* __restrict__ reg,
+                                                     const uint32_t* __restrict__ cell_off, int32_t* parent,
                                                      const uint32_t* __restrict__ dmask, LinkParams P) {
     const uint32_t pl = xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
     uint32_t mask = pl < P.N ? dmask[pl] : 0u;
@@ -730,8 +749,8 @@ __global__ void __launch_bounds__(kBlk) k_link_dense(const float4* __restrict__ 
         const int o = __builtin_ctz(m);
         if (lane == src) mask &= mask - 1u;
         const float4 me = xc[p];
-        const RegionDev& g = reg[__float_as_int(me.w)];
         const uint32_t c = skeys[p];
+        const RegionDev& g = reg[P.nregions == 1 ? 0u : region_of_cell(cell_off, P.nregions, c)];
         if (g.is_periodic)
             dense_entry<true>(p, o, me, g, c, P, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot, xc, parent);
         else
@@ -897,7 +916,6 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     // one round of loads. Pairs with another mid cell are taken once (forward
     // cells), pairs with any other cell in both directions.
     uint32_t n_s = 0, n_n = 0;
-    int32_t n_j0 = 0, n_j1 = 0, n_j2 = 0;
     bool n_dense = false, n_work = false;
     if (lane < 27 && lane != 13) {
         int32_t jijk[3];
@@ -912,9 +930,6 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                 n_n = cell_ptr[cB + 1] - n_s;
                 n_work = n_n > 0;
                 n_dense = dB;
-                n_j0 = jijk[0];
-                n_j1 = jijk[1];
-                n_j2 = jijk[2];
             }
         }
     }
@@ -924,13 +939,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     for (uint32_t i = lane; i < nA; i += kWave) {
         const float4 a = xc[sA + i];
         pa[i] = make_float4(a.x, a.y, a.z, __int_as_float(load_near(&parent[sA + i])));
-        const double x[3] = {(double)a.x, (double)a.y, (double)a.z};
-        uint32_t f = 0;
-        for (int d = 0; d < 3; ++d) {
-            if (x[d] - P.R <= lo[d] + 2.0 * P.m) f |= 1u << (2 * d);
-            if (x[d] + P.R >= lo[d] + g.cell[d] - 2.0 * P.m) f |= 2u << (2 * d);
-        }
-        fa[i] = (uint8_t)f;
+        fa[i] = (uint8_t)__float_as_uint(a.w);  // face bits (k_keys)
     }
     __syncthreads();
     uint64_t work = __ballot(n_work);
@@ -939,12 +948,18 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         work &= work - 1ull;
         const uint32_t sB = __shfl(n_s, o, kWave), nB = __shfl(n_n, o, kWave);
         const bool denseB = __shfl((int32_t)n_dense, o, kWave) != 0;
-        const int32_t jijk[3] = {__shfl(n_j0, o, kWave), __shfl(n_j1, o, kWave), __shfl(n_j2, o, kWave)};
         const int32_t off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
-        uint32_t need = 0;
+        // face bits a member needs to reach the neighbour: A's toward B, and
+        // B's toward A (the opposite faces)
+        uint32_t need = 0, needB = 0;
         for (int d = 0; d < 3; ++d) {
-            if (off[d] < 0) need |= 1u << (2 * d);
-            else if (off[d] > 0) need |= 2u << (2 * d);
+            if (off[d] < 0) {
+                need |= 1u << (2 * d);
+                needB |= 2u << (2 * d);
+            } else if (off[d] > 0) {
+                need |= 2u << (2 * d);
+                needB |= 1u << (2 * d);
+            }
         }
         // A's members that can reach B, in member order
         uint32_t na = 0;
@@ -957,8 +972,6 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         }
         if (na == 0) continue;
         __syncthreads();
-        double loB[3];
-        for (int d = 0; d < 3; ++d) loB[d] = g.origin[d] + (double)jijk[d] * g.cell[d];
         // B's members that can reach A are staged kWave at a time (a
         // non-dense B can be large); each full window is tested against A
         int32_t ja = -1, jb = -1;  // the parent pair this wave joined last
@@ -1015,11 +1028,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
             if (ok) {
                 q = xc[sB + i];
-                const double x[3] = {(double)q.x, (double)q.y, (double)q.z};
-                for (int d = 0; d < 3; ++d) {
-                    if (off[d] > 0) ok = ok && x[d] - P.R <= loB[d] + 2.0 * P.m;
-                    else if (off[d] < 0) ok = ok && x[d] + P.R >= loB[d] + g.cell[d] - 2.0 * P.m;
-                }
+                ok = (__float_as_uint(q.w) & needB) == needB;  // B's face bits toward A
             }
             const uint64_t m = __ballot(ok);
             const uint32_t cnt = (uint32_t)__popcll(m);
@@ -1545,7 +1554,8 @@ __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t*
 // anchor particle (its first CRS member) and its region
 __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
                         const uint32_t* __restrict__ cand_idx,
-                        const uint32_t* __restrict__ perm, const float4* __restrict__ xc,
+                        const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
+                        const uint32_t* __restrict__ cell_off, int32_t nreg,
                         const RegionDev* __restrict__ reg, uint32_t n, int32_t np_min,
                         int32_t* __restrict__ halo_id,
                         int32_t* __restrict__ root_out, uint32_t* __restrict__ anchor,
@@ -1559,12 +1569,13 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     // coalesced, before); in_halo follows from halo_id in a coalesced pass
     // (k_in_halo)
     if (cand) halo_id[i] = (int32_t)cand_idx[r] + 1;
-    if (root_out) root_out[i] = reg[__float_as_int(xc[p].w)].skip ? -1 : (int32_t)perm[r];
+    const uint32_t rg = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, skeys[p]);
+    if (root_out) root_out[i] = reg[rg].skip ? -1 : (int32_t)perm[r];
     // candidates are the positions whose size (0 for non-roots) reaches
     // np_min (fof_fortran.f90:866-872), numbered by cand_idx
     if ((int64_t)size[p] >= (int64_t)np_min) {
         anchor[cand_idx[p]] = i;
-        hregion[cand_idx[p]] = __float_as_int(xc[p].w);
+        hregion[cand_idx[p]] = (int32_t)rg;
     }
 }
 
@@ -1893,7 +1904,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* v0 = C.vals0.as<uint32_t>(N);
         uint32_t* v1 = C.vals1.as<uint32_t>(N);
         float4* xr = C.buf("xrel").as<float4>(N);
-        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, xr, A.L, P.Lh);
+        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, xr, A.L, P.Lh, P.R, P.m);
         PYFOF_LAUNCH_CHECK();
         const size_t tmp_bytes = std::max(radix_temp_bytes(N), scan_temp_bytes(ncell + 1));
         void* tmp = C.temp.get(std::max(tmp_bytes, scan_temp_bytes(N) + 64));
@@ -1965,7 +1976,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipEventRecord(C.ev_fork, s));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
         PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_fork, 0));
-        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, parent, dmask, P);
+        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, d_coff, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* croot = C.croot.as<int32_t>(std::max<uint32_t>(ndense, 1));
@@ -1991,7 +2002,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_mid, 0));
         if (ndense > 0) {
             k_link_dense<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, sub_ptr, sub_list, ncomp, croot,
-                                                   d_reg, parent, dmask, P);
+                                                   d_reg, d_coff, parent, dmask, P);
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipEventRecord(C.ev_join, s2));
@@ -2034,7 +2045,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         int32_t* hreg = C.hregion.as<int32_t>(N);
         k_fill_u32<<<nblk(N), kBlk, 0, s>>>(reinterpret_cast<uint32_t*>(o_id), N, 0u);
         PYFOF_LAUNCH_CHECK();
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, xc, d_reg, N, A.np_min, o_id,
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, skeys, d_coff, H, d_reg, N, A.np_min, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
         k_in_halo<<<nblk(N), kBlk, 0, s>>>(o_id, N, o_in);
