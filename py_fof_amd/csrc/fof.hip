@@ -95,8 +95,16 @@ __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
         double v = red[d][0];
         for (int k = 1; k < kBlk / kWave; ++k) v = d < 3 ? fmin(v, red[d][k]) : fmax(v, red[d][k]);
         unsigned long long* dst = bb + 6 * (size_t)ch.region + d;
-        if (d < 3) atomicMin(dst, dbl_to_ordered(v));
-        else atomicMax(dst, dbl_to_ordered(v));
+        // most chunks do not extend the box: read first, so that the
+        // thousands of workgroups of a region do not all queue atomics on
+        // the same six words
+        const unsigned long long u = dbl_to_ordered(v);
+        const unsigned long long cur = __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d < 3) {
+            if (u < cur) atomicMin(dst, u);
+        } else if (u > cur) {
+            atomicMax(dst, u);
+        }
     }
 }
 

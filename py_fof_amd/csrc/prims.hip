@@ -132,10 +132,25 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
     const int w = threadIdx.x / kWave;
+    if (base + kTile <= n && ((uintptr_t)(keys + base) & 15u) == 0) {
+        // a full tile: four keys per 16-byte load, all loads issued first
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
+        uint4 v[kIpt / 4];
+#pragma unroll
+        for (int j = 0; j < kIpt / 4; ++j) v[j] = k4[j * kBlock + threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < kIpt / 4; ++j) {
+            atomicAdd(&h[w][(v[j].x >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&h[w][(v[j].y >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&h[w][(v[j].z >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&h[w][(v[j].w >> shift) & (kRadix - 1)], 1u);
+        }
+    } else {
 #pragma unroll 4
-    for (int j = 0; j < kIpt; ++j) {
-        uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[w][(keys[i] >> shift) & (kRadix - 1)], 1u);
+        for (int j = 0; j < kIpt; ++j) {
+            uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+            if (i < n) atomicAdd(&h[w][(keys[i] >> shift) & (kRadix - 1)], 1u);
+        }
     }
     __syncthreads();
     for (int d = threadIdx.x; d < kRadix; d += kBlock) {
