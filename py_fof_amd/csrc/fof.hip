@@ -57,6 +57,7 @@ __global__ void k_iota_u32(uint32_t* p, uint64_t n) {
 __device__ __forceinline__ uint32_t face_bits(double x, double lo, double cell, double R, double m) {
     return (x - R <= lo + 2.0 * m ? 1u : 0u) | (x + R >= lo + cell - 2.0 * m ? 2u : 0u);
 }
+constexpr uint32_t kEdgeBit = 64u;  // beside the six face bits: the cell lies on the grid's boundary
 
 // ---------------------------------------------------------------------------
 // per-region bounding box of x_rel (fof_fortran.f90:524-531)
@@ -149,9 +150,13 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
         keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
         vals[i] = (uint32_t)i;
         const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+        // bit 6: the cell lies on the grid's boundary (a neighbour may wrap,
+        // or not exist)
+        const bool edge = ix == 1 || ix == n0 || iy == 1 || iy == n1 || iz == 1 || iz == n2;
         const uint32_t fb = face_bits((double)f0, o0 + (double)(ix - 1) * c0, c0, R, m) |
                             (face_bits((double)f1, o1 + (double)(iy - 1) * c1, c1, R, m) << 2) |
-                            (face_bits((double)f2, o2 + (double)(iz - 1) * c2, c2, R, m) << 4);
+                            (face_bits((double)f2, o2 + (double)(iz - 1) * c2, c2, R, m) << 4) |
+                            (edge ? kEdgeBit : 0u);
         xr[i] = make_float4(f0, f1, f2, __uint_as_float(fb));
     }
 }
@@ -449,6 +454,29 @@ __device__ __forceinline__ bool pair_linked_screened(const float4& a, const floa
     return pair_linked<PERIODIC>(a, q, P);
 }
 
+// Tests the particle `me` against the members [qq, e) of one cell. Two
+// partners at a time: their positions, and then the parents of the linked
+// ones, are loaded together; a partner whose parent is already the thread's
+// known root needs no union (pairs beat fours here: the wider batch spills
+// registers at this kernel's occupancy target).
+template <bool PERIODIC>
+__device__ __forceinline__ void scan_members(const float4& me, uint32_t qq, uint32_t e, const LinkParams& P,
+                                             const float4* __restrict__ xc, LinkCtx& L) {
+    for (; qq + 2 <= e; qq += 2) {
+        const float4 y0 = xc[qq], y1 = xc[qq + 1];
+        const bool h0 = pair_linked<PERIODIC>(me, y0, P), h1 = pair_linked<PERIODIC>(me, y1, P);
+        if (!(h0 || h1)) continue;
+        const int32_t q0 = (int32_t)qq;
+        const int32_t a0 = h0 ? load_near(&L.parent[q0]) : 0;
+        const int32_t a1 = h1 ? load_near(&L.parent[q0 + 1]) : 0;
+        if (h0 && a0 != L.rp) L.link(q0);
+        if (h1 && a1 != L.rp) L.link(q0 + 1);
+    }
+    for (; qq < e; ++qq) {
+        if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
+    }
+}
+
 // Tests the particle `me` (position p) against cell jc, reached by neighbour
 // offset o (13: its own cell). For the own cell only partners q > p are
 // tested. The caller visits only the neighbours the query box can reach
@@ -474,24 +502,7 @@ __device__ __forceinline__ void visit_cell(const float4& me, uint32_t p, uint32_
         L.dmask |= 1u << o;
         return;
     }
-    uint32_t qq = own ? p + 1 : s;
-    // two partners at a time: their positions, and then the parents of the
-    // linked ones, are loaded together; a partner whose parent is already the
-    // thread's known root needs no union (pairs beat fours here: the wider
-    // batch spills registers at this kernel's occupancy target)
-    for (; qq + 2 <= e; qq += 2) {
-        const float4 y0 = xc[qq], y1 = xc[qq + 1];
-        const bool h0 = pair_linked<PERIODIC>(me, y0, P), h1 = pair_linked<PERIODIC>(me, y1, P);
-        if (!(h0 || h1)) continue;
-        const int32_t q0 = (int32_t)qq;
-        const int32_t a0 = h0 ? load_near(&L.parent[q0]) : 0;
-        const int32_t a1 = h1 ? load_near(&L.parent[q0 + 1]) : 0;
-        if (h0 && a0 != L.rp) L.link(q0);
-        if (h1 && a1 != L.rp) L.link(q0 + 1);
-    }
-    for (; qq < e; ++qq) {
-        if (pair_linked<PERIODIC>(me, xc[qq], P)) L.link((int32_t)qq);
-    }
+    scan_members<PERIODIC>(me, own ? p + 1 : s, e, P, xc, L);
 }
 
 // Neighbour cell of (ijk + off) under the reference's rule (fof_fortran.f90:
@@ -557,10 +568,59 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     const bool near_dense = g.clique && own_cls == kAuxNearDense;
     LinkCtx L{parent, (int32_t)p, 0u};  // rp becomes a root at the first unite
     const uint32_t lc = c - g.cell_off;
+    // (the face bits and edge bit k_keys stored in .w)
+    const uint32_t fb = __float_as_uint(me.w);
+    if (own_cls < 0 && g.prune && !(fb & kEdgeBit) && (fb & 3u) != 3u && (fb & 12u) != 12u && (fb & 48u) != 48u) {
+        // Inside the grid every neighbour exists and none wraps: the one at
+        // offset (ox, oy, oz) is cell lc + ox + n0 (oy + n1 oz), forward
+        // exactly when the offset is lexicographically positive. With the
+        // box reaching at most one side per axis (the face bits), the cells
+        // it reaches are the seven subsets of the axes it crosses; their
+        // classes and member ranges are all loaded before any is scanned,
+        // so the loads of one particle overlap instead of queueing. (A dense
+        // own cell, in a region without cliques, takes the general path.)
+        const int32_t s1 = g.n[0], s2 = g.n[0] * g.n[1];
+        const int32_t dx = (fb & 1u) ? -1 : ((fb & 2u) ? 1 : 0);
+        const int32_t dy = (fb & 4u) ? -1 : ((fb & 8u) ? 1 : 0);
+        const int32_t dz = (fb & 16u) ? -1 : ((fb & 32u) ? 1 : 0);
+        int32_t cl[7];
+        uint32_t sb[7], eb[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int ax = k + 1;  // axes crossed: bit 0 x, bit 1 y, bit 2 z
+            const int32_t ox = (ax & 1) ? dx : 0, oy = (ax & 2) ? dy : 0, oz = (ax & 4) ? dz : 0;
+            const bool reach = ((ax & 1) == 0 || dx != 0) && ((ax & 2) == 0 || dy != 0) && ((ax & 4) == 0 || dz != 0);
+            const int o = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
+            cl[k] = -1;
+            sb[k] = eb[k] = 0;
+            if (reach && (o > 13 || near_dense)) {
+                const uint32_t jc = g.cell_off + (uint32_t)((int32_t)lc + ox + s1 * oy + s2 * oz);
+                cl[k] = aux[jc];
+                sb[k] = cell_ptr[jc];
+                eb[k] = cell_ptr[jc + 1];
+            }
+        }
+        scan_members<PERIODIC>(me, p + 1, cell_ptr[c + 1], P, xc, L);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int ax = k + 1;
+            const int32_t ox = (ax & 1) ? dx : 0, oy = (ax & 2) ? dy : 0, oz = (ax & 4) ? dz : 0;
+            const int o = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
+            const int32_t cls = cl[k];
+            if (sb[k] >= eb[k] || cls == kAuxMid) continue;  // empty or not reached; mid: k_cell_pair
+            if (cls >= 0) {
+                L.dmask |= 1u << o;  // dense: k_link_dense
+                continue;
+            }
+            if (o < 13) continue;  // a backward cell is visited for its sub-cells only
+            scan_members<PERIODIC>(me, sb[k], eb[k], P, xc, L);
+        }
+        return L.dmask;
+    }
+    visit_cell<PERIODIC>(me, p, c, 13, true, false, P, cell_ptr, aux, xc, L);
     int32_t ijk[3];
     double lo_unused[3];
     cell_lower_corner(g, lc, lo_unused, ijk);
-    visit_cell<PERIODIC>(me, p, c, 13, true, false, P, cell_ptr, aux, xc, L);
     // a cell off the periodic edges has no wrapped neighbours: offsets below
     // 13 (lexicographically negative) are then exactly the backward cells
     bool edge = false;
@@ -598,8 +658,6 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
     // rounding of the cell corners. So only about (1 + 2R / cell)^3 - 1
     // neighbours are visited instead of 26. A neighbour across a periodic
     // edge is always visited.
-    // (the face bits k_keys stored in .w)
-    const uint32_t fb = __float_as_uint(me.w);
     int32_t olo[3], ohi[3];
     for (int d = 0; d < 3; ++d) {
         const bool wrap_lo = g.periodic[d] && ijk[d] == 0;
@@ -1524,37 +1582,50 @@ __global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t
 // the contended one). A root, or a position past n, ends a run.
 constexpr int kCountChunks = 64;
 
+constexpr int kCountBatch = 16;  // chunks whose labels are loaded before any is counted
+
 __global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
     const int lane = __lane_id();
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t base = wave * kWave * kCountChunks;
     int32_t carry = -1;  // wave-uniform: the open run's label (>= 0) and count
     uint32_t ccount = 0;
-    for (int k = 0; k < kCountChunks; ++k) {
-        const uint64_t p = base + (uint64_t)k * kWave + (uint64_t)lane;
-        const int32_t l = p < n ? label[p] : -1;
-        const bool valid = p < n && l != (int32_t)p;
-        const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
-        const int32_t prev = __shfl_up(lab, 1, kWave);
-        const bool head = lane == 0 || prev != lab;
-        const uint64_t heads = __ballot(head);
-        const int32_t lab0 = __shfl(lab, 0, kWave);
-        const int lasthead = 63 - __clzll((long long)heads);
-        const bool last = k + 1 == kCountChunks;
-        const bool merge0 = carry >= 0 && lab0 == carry;
-        if (carry >= 0 && !merge0 && lane == 0) atomicAdd(&size[carry], ccount);
-        uint32_t cnt = 0;
-        if (valid && head) {
-            const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-            const int next = above ? __ffsll((long long)above) - 1 : 64;
-            cnt = (uint32_t)(next - lane) + (lane == 0 && merge0 ? ccount : 0u);
-            if (last || lane != lasthead) atomicAdd(&size[lab], cnt);
+    for (int k0 = 0; k0 < kCountChunks; k0 += kCountBatch) {
+        // the batch's loads go out together (the walk below is serial)
+        int32_t lb[kCountBatch];
+#pragma unroll
+        for (int b = 0; b < kCountBatch; ++b) {
+            const uint64_t p = base + (uint64_t)(k0 + b) * kWave + (uint64_t)lane;
+            lb[b] = p < n ? label[p] : -1;
         }
-        // the run at the last head stays open (its label is valid when >= 0)
-        const int32_t labL = __shfl(lab, lasthead, kWave);
-        const uint32_t cntL = __shfl(cnt, lasthead, kWave);
-        carry = (!last && labL >= 0) ? labL : -1;
-        ccount = carry >= 0 ? cntL : 0u;
+#pragma unroll
+        for (int b = 0; b < kCountBatch; ++b) {
+            const int k = k0 + b;
+            const uint64_t p = base + (uint64_t)k * kWave + (uint64_t)lane;
+            const int32_t l = lb[b];
+            const bool valid = p < n && l != (int32_t)p;
+            const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
+            const int32_t prev = __shfl_up(lab, 1, kWave);
+            const bool head = lane == 0 || prev != lab;
+            const uint64_t heads = __ballot(head);
+            const int32_t lab0 = __shfl(lab, 0, kWave);
+            const int lasthead = 63 - __clzll((long long)heads);
+            const bool last = k + 1 == kCountChunks;
+            const bool merge0 = carry >= 0 && lab0 == carry;
+            if (carry >= 0 && !merge0 && lane == 0) atomicAdd(&size[carry], ccount);
+            uint32_t cnt = 0;
+            if (valid && head) {
+                const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+                const int next = above ? __ffsll((long long)above) - 1 : 64;
+                cnt = (uint32_t)(next - lane) + (lane == 0 && merge0 ? ccount : 0u);
+                if (last || lane != lasthead) atomicAdd(&size[lab], cnt);
+            }
+            // the run at the last head stays open (its label is valid when >= 0)
+            const int32_t labL = __shfl(lab, lasthead, kWave);
+            const uint32_t cntL = __shfl(cnt, lasthead, kWave);
+            carry = (!last && labL >= 0) ? labL : -1;
+            ccount = carry >= 0 ? cntL : 0u;
+        }
     }
 }
 
@@ -1587,9 +1658,22 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     }
 }
 
+// four entries per thread when both arrays are 16-byte aligned (device
+// outputs of the caller may not be)
 __global__ void k_in_halo(const int32_t* __restrict__ halo_id, uint32_t n, int32_t* __restrict__ in_halo) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) in_halo[i] = halo_id[i] > 0 ? 1 : 0;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool vec = (((uintptr_t)halo_id | (uintptr_t)in_halo) & 15u) == 0;
+    if (vec) {
+        const uint64_t i = 4 * t;
+        if (i + 4 <= n) {
+            const int4 h = reinterpret_cast<const int4*>(halo_id)[t];
+            reinterpret_cast<int4*>(in_halo)[t] = make_int4(h.x > 0, h.y > 0, h.z > 0, h.w > 0);
+        } else {
+            for (uint64_t j = i; j < n; ++j) in_halo[j] = halo_id[j] > 0 ? 1 : 0;
+        }
+    } else {
+        for (uint64_t j = 4 * t; j < 4 * t + 4 && j < n; ++j) in_halo[j] = halo_id[j] > 0 ? 1 : 0;
+    }
 }
 
 // anchor and region of each valid halo under its new (bound) number
@@ -2056,7 +2140,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, skeys, d_coff, H, d_reg, N, A.np_min, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
-        k_in_halo<<<nblk(N), kBlk, 0, s>>>(o_id, N, o_in);
+        k_in_halo<<<nblk(((uint64_t)N + 3) / 4), kBlk, 0, s>>>(o_id, N, o_in);
         PYFOF_LAUNCH_CHECK();
         uint32_t nh = 0;
         PYFOF_HIP(hipMemcpyAsync(&nh, scal + 1, 4, hipMemcpyDeviceToHost, s));
