@@ -1171,7 +1171,7 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
     int32_t* lp = reinterpret_cast<int32_t*>(rp + P.sub_stride);
     uint32_t* mh = reinterpret_cast<uint32_t*>(lp + P.sub_stride);
     uint32_t* list = mh + P.sub_stride;
-    uint32_t* nl = list + P.sub_stride;  // [0] list size, [1] component count
+    uint32_t* nl = list + P.sub_stride;  // [0] list size, [1] component count, [2] roots counted
     float4* pos = reinterpret_cast<float4*>(lds + (size_t)16 * ((4 * P.sub_stride + 2 + 3) / 4));
     const uint32_t c = dense_cells[d];
     const uint32_t cs = cell_ptr[c], n_c = cell_ptr[c + 1] - cs;
@@ -1227,11 +1227,32 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
     static_assert(kFwdClassEnd[0] == 3 && kFwdClassEnd[1] == 9 && kFwdClassEnd[2] == 13 &&
                       kFwdClassEnd[3] == 16 && kFwdClassEnd[4] == 62,
                   "offset classes");
+    // after each class but the last, a cell that is already one component
+    // (one root among its non-empty sub-cells) has nothing left to join:
+    // most clump-centre cells are, after the nearest offsets
+    auto one_component = [&]() {
+        if (threadIdx.x == 0) nl[2] = 0;
+        __syncthreads();
+        uint32_t roots = 0;
+        for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
+            const int32_t A = (int32_t)(list[i] & 0xfffu);
+            roots += lp[A] == A ? 1u : 0u;
+        }
+        if (roots) atomicAdd(&nl[2], roots);
+        __syncthreads();
+        return nl[2] == 1u;
+    };
     offset_class(0, 3);
-    offset_class(3, 9);
-    offset_class(9, 13);
-    offset_class(13, 16);
-    offset_class(16, 62);
+    if (!one_component()) {
+        offset_class(3, 9);
+        if (!one_component()) {
+            offset_class(9, 13);
+            if (!one_component()) {
+                offset_class(13, 16);
+                if (!one_component()) offset_class(16, 62);
+            }
+        }
+    }
     for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
         const int32_t A = (int32_t)(list[i] & 0xfffu);
         // head of A: its smallest position (set as parent of every member)
@@ -1498,6 +1519,12 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
             if (i == next) joined[i] = lb;
         next = next + 1 == kJoinedCache ? 0 : next + 1;
     };
+    // the window's offsets run through a few neighbour cells; the last
+    // one's class and component count are kept, so an offset into the same
+    // cell costs one load (its sub-cell's local root) instead of three
+    // dependent ones
+    uint32_t last_cB = 0xffffffffu;
+    int32_t last_dB = -1, last_nc = 0;
     for (int oz = 0; oz < 5; ++oz) {
         const int32_t jz = jt[2][oz];
         if (jz < 0) continue;
@@ -1511,9 +1538,14 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
                 const uint32_t jl = (uint32_t)jx + (uint32_t)g.n[0] * ((uint32_t)jy + (uint32_t)g.n[1] * (uint32_t)jz);
                 if (jl <= lc) continue;
                 const uint32_t cB = g.cell_off + jl;
-                const int32_t dB = aux[cB];
+                if (cB != last_cB) {
+                    last_cB = cB;
+                    last_dB = aux[cB];
+                    last_nc = last_dB >= 0 ? ncomp[last_dB] : 0;
+                }
+                const int32_t dB = last_dB;
                 if (dB < 0) continue;  // sparse cell: below
-                if (single && ncomp[dB] == 1) continue;  // decided by k_dense_cell_cross
+                if (single && last_nc == 1) continue;  // decided by k_dense_cell_cross
                 const uint32_t sb = (uint32_t)st[0][ox] + (uint32_t)kk[0] * ((uint32_t)st[1][oy] + (uint32_t)kk[1] * (uint32_t)st[2][oz]);
                 const int32_t lb = sub_root[(size_t)dB * P.sub_stride + sb];
                 if (lb < 0) continue;  // empty sub-cell
