@@ -954,6 +954,7 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
 template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
+                          const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
                           uint16_t* ia, uint32_t* ib, uint8_t* fa, uint2* uq) {
     const int lane = __lane_id();
@@ -982,6 +983,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     // one round of loads. Pairs with another mid cell are taken once (forward
     // cells), pairs with any other cell in both directions.
     uint32_t n_s = 0, n_n = 0;
+    int32_t n_cls = -1;
     bool n_dense = false, n_work = false;
     if (lane < 27 && lane != 13) {
         int32_t jijk[3];
@@ -996,6 +998,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                 n_n = cell_ptr[cB + 1] - n_s;
                 n_work = n_n > 0;
                 n_dense = dB;
+                n_cls = clsB;
             }
         }
     }
@@ -1087,15 +1090,11 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             }
             __syncthreads();
         };
+        // stages the lanes' candidates (member i of B at position q, ok when
+        // its face bits reach A), flushing full windows
         uint32_t nb = 0;
-        for (uint32_t i0 = 0; i0 < nB; i0 += kWave) {
-            const uint32_t i = i0 + (uint32_t)lane;
-            bool ok = i < nB;
-            float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (ok) {
-                q = xc[sB + i];
-                ok = (__float_as_uint(q.w) & needB) == needB;  // B's face bits toward A
-            }
+        auto stage = [&](bool ok, float4 q, uint32_t i) {
+            ok = ok && (__float_as_uint(q.w) & needB) == needB;  // B's face bits toward A
             const uint64_t m = __ballot(ok);
             const uint32_t cnt = (uint32_t)__popcll(m);
             if (nb + cnt > (uint32_t)kWave) {
@@ -1109,6 +1108,70 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                 pb[k] = q;
             }
             nb += cnt;
+        };
+        if (!denseB) {
+            for (uint32_t i0 = 0; i0 < nB; i0 += kWave) {
+                const uint32_t i = i0 + (uint32_t)lane;
+                const bool ok = i < nB;
+                stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), i);
+            }
+        } else {
+            // A dense B holds its members by sub-cell: only the sub-cells
+            // within R + 4m of the face toward A can hold members with those
+            // face bits (sub-cell coordinates are monotone in x; the margin
+            // exceeds the face bits' 2m and the f64 rounding), so only they
+            // are loaded. Lanes take the box's sub-cells, a wave scan of
+            // their member counts spreads the members over the lanes.
+            const int32_t dB = __shfl(n_cls, o, kWave);
+            const uint32_t* row = sub_ptr + (size_t)dB * P.sub_stride;
+            int32_t blo[3], bn[3];
+            for (int d = 0; d < 3; ++d) {
+                const int32_t k = g.k[d];
+                const double reach = P.R + 4.0 * P.m;
+                if (off[d] > 0) {
+                    blo[d] = 0;
+                    bn[d] = (int32_t)fmin((double)k, floor(reach / g.sub[d]) + 1.0);
+                } else if (off[d] < 0) {
+                    blo[d] = (int32_t)fmax(0.0, floor((g.cell[d] - reach) / g.sub[d]));
+                    bn[d] = k - blo[d];
+                } else {
+                    blo[d] = 0;
+                    bn[d] = k;
+                }
+            }
+            const uint32_t nbox = (uint32_t)(bn[0] * bn[1] * bn[2]);
+            for (uint32_t b0 = 0; b0 < nbox; b0 += kWave) {
+                const uint32_t bi = b0 + (uint32_t)lane;
+                uint32_t t0 = 0, cnt = 0;
+                if (bi < nbox) {
+                    const uint32_t sx = (uint32_t)blo[0] + bi % (uint32_t)bn[0];
+                    const uint32_t sy = (uint32_t)blo[1] + (bi / (uint32_t)bn[0]) % (uint32_t)bn[1];
+                    const uint32_t sz = (uint32_t)blo[2] + bi / (uint32_t)(bn[0] * bn[1]);
+                    const uint32_t idx = sx + (uint32_t)g.k[0] * (sy + (uint32_t)g.k[1] * sz);
+                    t0 = row[idx];
+                    cnt = row[idx + 1] - t0;
+                }
+                uint32_t incl = cnt;
+                for (int sh = 1; sh < kWave; sh <<= 1) {
+                    const uint32_t v = __shfl_up(incl, sh, kWave);
+                    if (lane >= sh) incl += v;
+                }
+                const uint32_t total = __shfl(incl, kWave - 1, kWave);
+                for (uint32_t o0 = 0; o0 < total; o0 += kWave) {
+                    const uint32_t oo = o0 + (uint32_t)lane;
+                    // the box lane holding member oo: the first with incl > oo
+                    int L = 0;
+                    for (int step = kWave / 2; step > 0; step >>= 1) {
+                        const uint32_t v = (uint32_t)__shfl((int)incl, L + step - 1, kWave);
+                        if (v <= oo) L += step;
+                    }
+                    const uint32_t first = (uint32_t)__shfl((int)t0, L, kWave);
+                    const uint32_t before = (uint32_t)__shfl((int)(incl - cnt), L, kWave);
+                    const bool ok = oo < total;
+                    const uint32_t i = ok ? sub_list[sB + first + (oo - before)] - sB : 0u;
+                    stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), i);
+                }
+            }
         }
         if (nb) flush(nb);
     }
@@ -1120,6 +1183,8 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ cell_off,
                                                      const RegionDev* __restrict__ reg, int32_t nreg,
                                                      const int32_t* __restrict__ aux,
+                                                     const uint32_t* __restrict__ sub_ptr,
+                                                     const uint32_t* __restrict__ sub_list,
                                                      const float4* __restrict__ xc, int32_t* parent,
                                                      LinkParams P) {
     __shared__ float4 pa[kMidMax], pb[kWave];
@@ -1130,9 +1195,9 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa, uq);
+        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, ib, fa, uq);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, xc, parent, pa, pb, ia, ib, fa, uq);
+        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, ib, fa, uq);
 }
 
 template <bool PERIODIC>
@@ -2132,7 +2197,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipEventRecord(C.ev_join, s2));
         PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_dense, 0));
         if (nmid > 0) {
-            k_cell_pair<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, xc, parent, P);
+            k_cell_pair<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, sub_ptr, sub_list, xc,
+                                                parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipEventRecord(C.ev_pair, s3));
