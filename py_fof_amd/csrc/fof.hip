@@ -1,15 +1,19 @@
 // Friends-of-Friends on MI355X: cell binning, linking, relabel and catalog.
 //
-// Pipeline of one call (all on the context's stream):
+// Pipeline of one call (on the context's streams):
 //   k_bbox         per-region bounding box of the wrapped coordinates
 //   (host)         reference cell geometry per region (fof_fortran.f90:518-570)
-//   k_keys         reference cell id of every particle
+//   k_keys         reference cell id, float coordinates and face bits of every
+//                  particle
 //   radix sort     stable sort by cell id = the reference's CRS order
-//   k_run_ends     + inclusive max-scan -> cell_ptr
+//   k_cell_starts  cell_ptr from the sorted keys
 //   k_gather       float CRS coordinates (the reference's x_crs)
 //   k_dense_*      sub-cells (side <= (b - m)/sqrt 3) inside dense cells
-//   k_link         pair tests with the reference predicate + atomicCAS DSU
-//   k_flatten / k_count / k_cand / k_write   roots, sizes, numbering, outputs
+//   linking        k_link (sparse cells), k_cell_intra / k_cell_pair (mid
+//                  cells), k_dense_intra / k_dense_cell_cross / k_dense_cross
+//                  (dense cells), k_link_dense: reference predicate +
+//                  lock-free union-find
+//   k_label_count / scan / k_write   roots, sizes, numbering, outputs
 //   catalog        halo members grouped by halo id (fof_with_info, ang. mom.)
 #include <math.h>
 #include <string.h>
@@ -1661,47 +1665,56 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
                            sub_root, xc, parent);
 }
 
-// labels, and each root's own count: size[p] = 1 for a root, 0 otherwise
-// (plain stores; k_count then adds the other members)
-__global__ void k_flatten(int32_t* parent, int32_t* __restrict__ label, uint32_t* __restrict__ size, uint32_t n) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int32_t r = dsu_find(parent, (int32_t)p);
-    label[p] = r;
-    size[p] = r == (int32_t)p ? 1u : 0u;
-}
-
-// component sizes: the members that are not roots. Each wave walks
-// kCountChunks consecutive chunks of kWave positions; runs of equal labels
-// are summed per chunk, and the run that reaches a chunk's last lane is
-// carried into the next chunk instead of being added at once, so a large
-// component costs one atomic per wave rather than per chunk (its counter is
-// the contended one). A root, or a position past n, ends a run.
+// Chunks of kWave positions per wave of k_label_count, and chunks whose
+// parents are loaded together.
 constexpr int kCountChunks = 64;
 
-constexpr int kCountBatch = 16;  // chunks whose labels are loaded before any is counted
+constexpr int kCountBatch = 16;
 
-__global__ void k_count(const int32_t* __restrict__ label, uint32_t n, uint32_t* __restrict__ size) {
+// Labels and component sizes in one pass (after linking; nothing changes
+// the forest meanwhile). Each wave walks kCountChunks consecutive chunks of
+// kWave positions, kCountBatch chunks at a time: the parents of the batch's
+// positions are followed to their roots with all the batch's loads of a step
+// issued together (most paths are a few steps after the path halving of
+// linking), and the labels written. Runs of equal labels are counted per
+// chunk, every member included (size was zero-filled), and the run that
+// reaches a chunk's last lane is carried into the next chunk instead of
+// being added at once, so a large component costs one atomic per wave
+// rather than per chunk (its counter is the contended one). A position past
+// n ends a run.
+__global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __restrict__ label, uint32_t n,
+                              uint32_t* __restrict__ size) {
     const int lane = __lane_id();
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t base = wave * kWave * kCountChunks;
     int32_t carry = -1;  // wave-uniform: the open run's label (>= 0) and count
     uint32_t ccount = 0;
     for (int k0 = 0; k0 < kCountChunks; k0 += kCountBatch) {
-        // the batch's loads go out together (the walk below is serial)
-        int32_t lb[kCountBatch];
+        int32_t x[kCountBatch];
 #pragma unroll
         for (int b = 0; b < kCountBatch; ++b) {
             const uint64_t p = base + (uint64_t)(k0 + b) * kWave + (uint64_t)lane;
-            lb[b] = p < n ? label[p] : -1;
+            x[b] = p < n ? parent[p] : -1;
+        }
+        for (;;) {
+            int32_t y[kCountBatch];
+#pragma unroll
+            for (int b = 0; b < kCountBatch; ++b) y[b] = x[b] >= 0 ? parent[x[b]] : -1;
+            bool moved = false;
+#pragma unroll
+            for (int b = 0; b < kCountBatch; ++b) {
+                moved = moved || y[b] != x[b];
+                x[b] = y[b];
+            }
+            if (!__ballot(moved)) break;
         }
 #pragma unroll
         for (int b = 0; b < kCountBatch; ++b) {
             const int k = k0 + b;
             const uint64_t p = base + (uint64_t)k * kWave + (uint64_t)lane;
-            const int32_t l = lb[b];
-            const bool valid = p < n && l != (int32_t)p;
-            const int32_t lab = valid ? l : -2 - lane;  // unique: never continues a run
+            const bool valid = p < n;
+            if (valid) label[p] = x[b];
+            const int32_t lab = valid ? x[b] : -2 - lane;  // unique: never continues a run
             const int32_t prev = __shfl_up(lab, 1, kWave);
             const bool head = lane == 0 || prev != lab;
             const uint64_t heads = __ballot(head);
@@ -2220,9 +2233,9 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         int32_t* label = C.label.as<int32_t>(N);
         uint32_t* size = C.size.as<uint32_t>(N);
         uint32_t* cidx = C.cidx.as<uint32_t>(N);
-        k_flatten<<<nblk(N), kBlk, 0, s>>>(parent, label, size, N);
+        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(size, N, 0u);
         PYFOF_LAUNCH_CHECK();
-        k_count<<<nblk((N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(label, N, size);
+        k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size);
         PYFOF_LAUNCH_CHECK();
         // candidates (fof_fortran.f90:866-872): positions whose component
         // size (0 for non-roots) reaches np_min, numbered in position order
