@@ -1,11 +1,12 @@
 // Device-wide exclusive scan and stable LSD radix sort (keys + values, u32).
 //
-// Both work on tiles of 4096 elements per 256-thread workgroup (4 waves of 64).
-// The radix sort is the classic reduce-then-scan form: a per-tile digit
-// histogram, one device scan of the digit-major [digit][tile] table, then a
-// scatter in which each tile ranks its items stably: each wave ranks its own
-// contiguous quarter with ballots and a per-wave running count, and one
-// barrier combines the waves in order.
+// The scans work on tiles of 4096 elements per 256-thread workgroup (4 waves
+// of 64), the sort on tiles of 8192 keys per 512-thread workgroup. The radix
+// sort is the classic reduce-then-scan form: a per-tile digit histogram, one
+// device scan of the digit-major [digit][tile] table, then a scatter in
+// which each tile ranks its items stably: each wave ranks its own contiguous
+// share with ballots and a per-wave running count, and one barrier combines
+// the waves in order.
 #include "prims.hpp"
 
 namespace pyfof {
@@ -17,6 +18,10 @@ constexpr int kIpt = 16;
 constexpr int kTile = kBlock * kIpt;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
+// The radix sort's tiles: 8192 keys per 512-thread workgroup, so that a
+// tile's output runs of one digit are 32 keys (128 B) long on average.
+constexpr int kSBlock = 512;
+constexpr int kSTile = kSBlock * kIpt;
 
 struct OpSum {
     static __device__ __host__ uint32_t id() { return 0u; }
@@ -123,21 +128,21 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restrict__ keys, uint64_t n,
+__global__ void __launch_bounds__(kSBlock) k_radix_hist(const uint32_t* __restrict__ keys, uint64_t n,
                                                        int shift, uint32_t* __restrict__ hist,
                                                        uint32_t ntiles) {
-    __shared__ uint32_t h[kBlock / kWave][kRadix];
+    __shared__ uint32_t h[kSBlock / kWave][kRadix];
     uint32_t* hf = &h[0][0];
-    for (int i = threadIdx.x; i < (kBlock / kWave) * kRadix; i += kBlock) hf[i] = 0;
+    for (int i = threadIdx.x; i < (kSBlock / kWave) * kRadix; i += kSBlock) hf[i] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const uint64_t base = (uint64_t)blockIdx.x * kSTile;
     const int w = threadIdx.x / kWave;
-    if (base + kTile <= n && ((uintptr_t)(keys + base) & 15u) == 0) {
+    if (base + kSTile <= n && ((uintptr_t)(keys + base) & 15u) == 0) {
         // a full tile: four keys per 16-byte load, all loads issued first
         const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
         uint4 v[kIpt / 4];
 #pragma unroll
-        for (int j = 0; j < kIpt / 4; ++j) v[j] = k4[j * kBlock + threadIdx.x];
+        for (int j = 0; j < kIpt / 4; ++j) v[j] = k4[j * kSBlock + threadIdx.x];
 #pragma unroll
         for (int j = 0; j < kIpt / 4; ++j) {
             atomicAdd(&h[w][(v[j].x >> shift) & (kRadix - 1)], 1u);
@@ -148,47 +153,47 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
     } else {
 #pragma unroll 4
         for (int j = 0; j < kIpt; ++j) {
-            uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+            uint64_t i = base + (uint64_t)j * kSBlock + threadIdx.x;
             if (i < n) atomicAdd(&h[w][(keys[i] >> shift) & (kRadix - 1)], 1u);
         }
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) {
+    for (int d = threadIdx.x; d < kRadix; d += kSBlock) {
         uint32_t c = 0;
-        for (int k = 0; k < kBlock / kWave; ++k) c += h[k][d];
+        for (int k = 0; k < kSBlock / kWave; ++k) c += h[k][d];
         hist[(uint64_t)d * ntiles + blockIdx.x] = c;
     }
 }
 
-// Each wave ranks a contiguous quarter of the tile, 64 items per round: lanes
+// Each wave ranks a contiguous eighth of the tile, 64 items per round: lanes
 // with equal digits are matched by one ballot per digit bit, and a running
 // count per digit in the wave's own LDS row gives each item its rank among
 // the wave's earlier items (the rows need no barrier: a wave's lanes run in
-// lockstep). One barrier then turns the four rows into prefixes in wave
+// lockstep). One barrier then turns the eight rows into prefixes in wave
 // order, so the rank of an item follows (wave, round, lane) = item order:
 // the sort is stable. The ranked tile is then laid out in LDS in digit order
 // and written from there, so that neighbouring lanes store neighbouring
 // addresses (runs of one digit) instead of scattering lane by lane.
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
-                                                          const uint32_t* __restrict__ vals,
-                                                          uint32_t* __restrict__ okeys,
-                                                          uint32_t* __restrict__ ovals, uint64_t n,
-                                                          int shift,
-                                                          const uint32_t* __restrict__ hist,
-                                                          uint32_t ntiles) {
-    static_assert(kBlock == kRadix, "one thread per digit in the digit scans");
-    constexpr int kWaves = kBlock / kWave;
-    constexpr int kRounds = kTile / kBlock;  // rounds of 64 items per wave
+__global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ vals,
+                                                           uint32_t* __restrict__ okeys,
+                                                           uint32_t* __restrict__ ovals, uint64_t n,
+                                                           int shift,
+                                                           const uint32_t* __restrict__ hist,
+                                                           uint32_t ntiles) {
+    static_assert(kSBlock >= kRadix, "one thread per digit in the digit scans");
+    constexpr int kWaves = kSBlock / kWave;
+    constexpr int kRounds = kSTile / kSBlock;  // rounds of 64 items per wave
     __shared__ uint32_t wc[kWaves][kRadix];  // per-wave counts, then per-wave offsets in a digit
     __shared__ uint32_t lstart[kRadix];      // first tile-local position of each digit
     __shared__ uint32_t gstart[kRadix];      // first output position of each digit of this tile
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ uint32_t sk[kTile], sv[kTile];
+    __shared__ uint32_t wsum[kRadix / kWave];
+    __shared__ uint32_t sk[kSTile], sv[kSTile];
     const int t = threadIdx.x, w = t / kWave;
     uint32_t* wf = &wc[0][0];
-    for (int i = t; i < kWaves * kRadix; i += kBlock) wf[i] = 0;
+    for (int i = t; i < kWaves * kRadix; i += kSBlock) wf[i] = 0;
     __syncthreads();
-    const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kSTile;
     const uint64_t base = tile0 + (uint64_t)w * (kRounds * kWave);
     const uint64_t lt = lanemask_lt();
     uint32_t kk[kRounds], vv[kRounds], rk[kRounds];
@@ -214,20 +219,27 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
     }
     __syncthreads();
     {
-        const int d = t;
-        const uint32_t c0 = wc[0][d], c1 = wc[1][d], c2 = wc[2][d], c3 = wc[3][d];
-        wc[0][d] = 0;
-        wc[1][d] = c0;
-        wc[2][d] = c0 + c1;
-        wc[3][d] = c0 + c1 + c2;
-        const uint32_t tot = c0 + c1 + c2 + c3;
+        // threads 0 .. kRadix - 1: one digit each (waves 0 .. kRadix / kWave - 1)
+        uint32_t tot = 0;
+        if (t < kRadix) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < kWaves; ++k) {
+                const uint32_t c = wc[k][t];
+                wc[k][t] = acc;
+                acc += c;
+            }
+            tot = acc;
+            gstart[t] = hist[(uint64_t)t * ntiles + blockIdx.x];
+        }
         const uint32_t inc = wave_inclusive_scan<OpSum>(tot);
-        if (__lane_id() == kWave - 1) wsum[w] = inc;
-        gstart[d] = hist[(uint64_t)d * ntiles + blockIdx.x];
+        if (t < kRadix && __lane_id() == kWave - 1) wsum[w] = inc;
         __syncthreads();
-        uint32_t off = 0;
-        for (int k = 0; k < w; ++k) off += wsum[k];
-        lstart[d] = off + inc - tot;
+        if (t < kRadix) {
+            uint32_t off = 0;
+            for (int k = 0; k < w; ++k) off += wsum[k];
+            lstart[t] = off + inc - tot;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -241,10 +253,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
     }
     __syncthreads();
     const uint64_t rem = n - tile0;
-    const uint32_t cnt = rem < (uint64_t)kTile ? (uint32_t)rem : (uint32_t)kTile;
+    const uint32_t cnt = rem < (uint64_t)kSTile ? (uint32_t)rem : (uint32_t)kSTile;
 #pragma unroll
     for (int j = 0; j < kRounds; ++j) {
-        const uint32_t lp = (uint32_t)(j * kBlock + t);
+        const uint32_t lp = (uint32_t)(j * kSBlock + t);
         if (lp >= cnt) continue;
         const uint32_t k = sk[lp];
         const uint32_t d = (k >> shift) & (kRadix - 1);
@@ -293,7 +305,7 @@ int bits_for(uint64_t max_key) {
 size_t scan_temp_bytes(uint64_t n) { return scan_levels_bytes(n); }
 
 size_t radix_temp_bytes(uint64_t n) {
-    const uint64_t ntiles = n == 0 ? 1 : div_up(n, kTile);
+    const uint64_t ntiles = n == 0 ? 1 : div_up(n, kSTile);
     const uint64_t m = ntiles * kRadix;
     return ((m * sizeof(uint32_t) + 255) / 256) * 256 + scan_levels_bytes(m);
 }
@@ -316,17 +328,17 @@ void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void*
 bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
                       uint64_t n, int bits, void* temp, hipStream_t s) {
     if (n <= 1 || bits <= 0) return false;
-    const uint32_t ntiles = div_up(n, kTile);
+    const uint32_t ntiles = div_up(n, kSTile);
     const uint64_t m = (uint64_t)ntiles * kRadix;
     uint32_t* hist = static_cast<uint32_t*>(temp);
     uint32_t* stemp = hist + ((m * sizeof(uint32_t) + 255) / 256) * 256 / sizeof(uint32_t);
     uint32_t *ik = keys, *iv = vals, *ok = alt_keys, *ov = alt_vals;
     bool in_alt = false;
     for (int shift = 0; shift < bits; shift += kRadixBits) {
-        k_radix_hist<<<ntiles, kBlock, 0, s>>>(ik, n, shift, hist, ntiles);
+        k_radix_hist<<<ntiles, kSBlock, 0, s>>>(ik, n, shift, hist, ntiles);
         PYFOF_LAUNCH_CHECK();
         scan_rec<OpSum>(InPlain{hist}, hist, m, nullptr, stemp, 0, s);
-        k_radix_scatter<<<ntiles, kBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);
+        k_radix_scatter<<<ntiles, kSBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);
         PYFOF_LAUNCH_CHECK();
         uint32_t* tk = ik; ik = ok; ok = tk;
         uint32_t* tv = iv; iv = ov; ov = tv;
