@@ -1520,7 +1520,7 @@ __global__ void __launch_bounds__(kWave) k_dense_cell_cross(const uint32_t* __re
         cell_cross<false>(d, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
 }
 
-constexpr int kJoinedCache = 4;  // local roots remembered as joined in dense_cross
+constexpr int kJoinedCache = 8;  // local roots remembered as joined in dense_cross
 
 // One thread per non-empty clique sub-cell A of a cell with several local
 // components, or next to such a cell: offsets into forward dense coarse cells,
