@@ -264,7 +264,7 @@ constexpr int32_t kAuxNearDense = -3;
 constexpr uint32_t kMidMin = 6, kMidMax = 255;  // particles of a k_cell_intra cell
 constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
 
-constexpr int kMarkPer = 8;                       // consecutive cells per thread
+constexpr int kMarkPer = 4;                       // consecutive cells per thread
 
 __global__ void __launch_bounds__(kMarkBlk) k_dense_mark(const uint32_t* __restrict__ cell_ptr, uint32_t ncell,
                              const uint32_t* __restrict__ cell_off, const RegionDev* __restrict__ reg,
@@ -1667,7 +1667,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
 
 // Chunks of kWave positions per wave of k_label_count, and chunks whose
 // parents are loaded together.
-constexpr int kCountChunks = 64;
+constexpr int kCountChunks = 128;
 
 constexpr int kCountBatch = 16;
 
