@@ -9,6 +9,8 @@
 // the waves in order.
 #include "prims.hpp"
 
+#include <algorithm>
+
 namespace pyfof {
 
 namespace {
@@ -16,8 +18,11 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kIpt = 16;
 constexpr int kTile = kBlock * kIpt;
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
+// Digits of at most 8 bits. A sort of `bits` key bits makes the fewest
+// passes of 8-bit digits, ceil(bits / 8), with the digit width spread evenly
+// over them (28 bits: four 7-bit passes): fewer digits per pass mean shorter
+// histograms and longer output runs of one digit per tile.
+constexpr int kMaxRadixBits = 8;
 // The radix sort's tiles: 8192 keys per 512-thread workgroup, so that a
 // tile's output runs of one digit are 32 keys (128 B) long on average.
 constexpr int kSBlock = 512;
@@ -128,9 +133,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out
     }
 }
 
+template <int kRadixBits>
 __global__ void __launch_bounds__(kSBlock) k_radix_hist(const uint32_t* __restrict__ keys, uint64_t n,
                                                        int shift, uint32_t* __restrict__ hist,
                                                        uint32_t ntiles) {
+    constexpr int kRadix = 1 << kRadixBits;
     __shared__ uint32_t h[kSBlock / kWave][kRadix];
     uint32_t* hf = &h[0][0];
     for (int i = threadIdx.x; i < (kSBlock / kWave) * kRadix; i += kSBlock) hf[i] = 0;
@@ -174,6 +181,7 @@ __global__ void __launch_bounds__(kSBlock) k_radix_hist(const uint32_t* __restri
 // the sort is stable. The ranked tile is then laid out in LDS in digit order
 // and written from there, so that neighbouring lanes store neighbouring
 // addresses (runs of one digit) instead of scattering lane by lane.
+template <int kRadixBits>
 __global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __restrict__ keys,
                                                            const uint32_t* __restrict__ vals,
                                                            uint32_t* __restrict__ okeys,
@@ -181,13 +189,14 @@ __global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __res
                                                            int shift,
                                                            const uint32_t* __restrict__ hist,
                                                            uint32_t ntiles) {
+    constexpr int kRadix = 1 << kRadixBits;
     static_assert(kSBlock >= kRadix, "one thread per digit in the digit scans");
     constexpr int kWaves = kSBlock / kWave;
     constexpr int kRounds = kSTile / kSBlock;  // rounds of 64 items per wave
     __shared__ uint32_t wc[kWaves][kRadix];  // per-wave counts, then per-wave offsets in a digit
     __shared__ uint32_t lstart[kRadix];      // first tile-local position of each digit
     __shared__ uint32_t gstart[kRadix];      // first output position of each digit of this tile
-    __shared__ uint32_t wsum[kRadix / kWave];
+    __shared__ uint32_t wsum[kRadix > kWave ? kRadix / kWave : 1];
     __shared__ uint32_t sk[kSTile], sv[kSTile];
     const int t = threadIdx.x, w = t / kWave;
     uint32_t* wf = &wc[0][0];
@@ -306,7 +315,7 @@ size_t scan_temp_bytes(uint64_t n) { return scan_levels_bytes(n); }
 
 size_t radix_temp_bytes(uint64_t n) {
     const uint64_t ntiles = n == 0 ? 1 : div_up(n, kSTile);
-    const uint64_t m = ntiles * kRadix;
+    const uint64_t m = ntiles << kMaxRadixBits;
     return ((m * sizeof(uint32_t) + 255) / 256) * 256 + scan_levels_bytes(m);
 }
 
@@ -329,17 +338,35 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32
                       uint64_t n, int bits, void* temp, hipStream_t s) {
     if (n <= 1 || bits <= 0) return false;
     const uint32_t ntiles = div_up(n, kSTile);
-    const uint64_t m = (uint64_t)ntiles * kRadix;
+    const uint64_t m_max = (uint64_t)ntiles << kMaxRadixBits;
     uint32_t* hist = static_cast<uint32_t*>(temp);
-    uint32_t* stemp = hist + ((m * sizeof(uint32_t) + 255) / 256) * 256 / sizeof(uint32_t);
+    uint32_t* stemp = hist + ((m_max * sizeof(uint32_t) + 255) / 256) * 256 / sizeof(uint32_t);
     uint32_t *ik = keys, *iv = vals, *ok = alt_keys, *ov = alt_vals;
     bool in_alt = false;
-    for (int shift = 0; shift < bits; shift += kRadixBits) {
-        k_radix_hist<<<ntiles, kSBlock, 0, s>>>(ik, n, shift, hist, ntiles);
-        PYFOF_LAUNCH_CHECK();
-        scan_rec<OpSum>(InPlain{hist}, hist, m, nullptr, stemp, 0, s);
-        k_radix_scatter<<<ntiles, kSBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);
-        PYFOF_LAUNCH_CHECK();
+    const int passes = (bits + kMaxRadixBits - 1) / kMaxRadixBits;
+    // digits narrower than 4 bits only occur in a single pass (bits <= 3),
+    // where the wider digit's extra bits are zero in every key
+    const int width = std::max(4, (bits + passes - 1) / passes);
+    for (int shift = 0; shift < bits; shift += width) {
+        const uint64_t m = (uint64_t)ntiles << width;
+        switch (width) {
+#define PYFOF_RADIX_PASS(W)                                                                          \
+    case W:                                                                                          \
+        k_radix_hist<W><<<ntiles, kSBlock, 0, s>>>(ik, n, shift, hist, ntiles);                      \
+        PYFOF_LAUNCH_CHECK();                                                                        \
+        scan_rec<OpSum>(InPlain{hist}, hist, m, nullptr, stemp, 0, s);                               \
+        k_radix_scatter<W><<<ntiles, kSBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);       \
+        PYFOF_LAUNCH_CHECK();                                                                        \
+        break;
+            PYFOF_RADIX_PASS(4)
+            PYFOF_RADIX_PASS(5)
+            PYFOF_RADIX_PASS(6)
+            PYFOF_RADIX_PASS(7)
+            PYFOF_RADIX_PASS(8)
+#undef PYFOF_RADIX_PASS
+            default:
+                throw std::runtime_error("radix_sort_pairs: digit width out of range");
+        }
         uint32_t* tk = ik; ik = ok; ok = tk;
         uint32_t* tv = iv; iv = ov; ov = tv;
         in_alt = !in_alt;
