@@ -74,13 +74,30 @@ __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
     const size_t a = (size_t)reg_start[ch.region];
     const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
-        const double v0 = rel_coord(xp[3 * (size_t)i], xa0, L, Lh);
-        const double v1 = rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh);
-        const double v2 = rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh);
-        mn[0] = fmin(mn[0], v0); mx[0] = fmax(mx[0], v0);
-        mn[1] = fmin(mn[1], v1); mx[1] = fmax(mx[1], v1);
-        mn[2] = fmin(mn[2], v2); mx[2] = fmax(mx[2], v2);
+    // kBboxRows rows per thread loaded before any is used (rel_coord's rare
+    // fmod branch would otherwise keep each load waiting for the previous
+    // row's arithmetic); a row past the chunk's end repeats the thread's
+    // first row, which leaves the box unchanged
+    constexpr int kBboxRows = 8;
+    for (int32_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBboxRows * kBlk) {
+        double v[kBboxRows][3];
+#pragma unroll
+        for (int u = 0; u < kBboxRows; ++u) {
+            const int32_t i = i0 + u * kBlk;
+            const size_t j = 3 * (size_t)(i < ch.end ? i : i0);
+            v[u][0] = xp[j];
+            v[u][1] = xp[j + 1];
+            v[u][2] = xp[j + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kBboxRows; ++u) {
+            const double v0 = rel_coord(v[u][0], xa0, L, Lh);
+            const double v1 = rel_coord(v[u][1], xa1, L, Lh);
+            const double v2 = rel_coord(v[u][2], xa2, L, Lh);
+            mn[0] = fmin(mn[0], v0); mx[0] = fmax(mx[0], v0);
+            mn[1] = fmin(mn[1], v1); mx[1] = fmax(mx[1], v1);
+            mn[2] = fmin(mn[2], v2); mx[2] = fmax(mx[2], v2);
+        }
     }
     __shared__ double red[6][kBlk / kWave];
 #pragma unroll
@@ -144,24 +161,42 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
     const double o0 = g.origin[0], o1 = g.origin[1], o2 = g.origin[2];
     const double c0 = g.cell[0], c1 = g.cell[1], c2 = g.cell[2];
     const int32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
-        const double v0 = rel_coord(xp[3 * (size_t)i], xa0, L, Lh);
-        const double v1 = rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh);
-        const double v2 = rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh);
-        const int32_t ix = ref_cell_coord(v0, o0, c0, n0);
-        const int32_t iy = ref_cell_coord(v1, o1, c1, n1);
-        const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
-        keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
-        vals[i] = (uint32_t)i;
-        const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
-        // bit 6: the cell lies on the grid's boundary (a neighbour may wrap,
-        // or not exist)
-        const bool edge = ix == 1 || ix == n0 || iy == 1 || iy == n1 || iz == 1 || iz == n2;
-        const uint32_t fb = face_bits((double)f0, o0 + (double)(ix - 1) * c0, c0, R, m) |
-                            (face_bits((double)f1, o1 + (double)(iy - 1) * c1, c1, R, m) << 2) |
-                            (face_bits((double)f2, o2 + (double)(iz - 1) * c2, c2, R, m) << 4) |
-                            (edge ? kEdgeBit : 0u);
-        xr[i] = make_float4(f0, f1, f2, __uint_as_float(fb));
+    // kKeyRows rows per thread loaded before any is used (as in k_bbox); a
+    // row past the chunk's end is loaded from the thread's first row and
+    // not stored
+    constexpr int kKeyRows = 4;
+    for (int32_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kKeyRows * kBlk) {
+        double xv[kKeyRows][3];
+#pragma unroll
+        for (int u = 0; u < kKeyRows; ++u) {
+            const int32_t i = i0 + u * kBlk;
+            const size_t j = 3 * (size_t)(i < ch.end ? i : i0);
+            xv[u][0] = xp[j];
+            xv[u][1] = xp[j + 1];
+            xv[u][2] = xp[j + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kKeyRows; ++u) {
+            const int32_t i = i0 + u * kBlk;
+            if (i >= ch.end) break;
+            const double v0 = rel_coord(xv[u][0], xa0, L, Lh);
+            const double v1 = rel_coord(xv[u][1], xa1, L, Lh);
+            const double v2 = rel_coord(xv[u][2], xa2, L, Lh);
+            const int32_t ix = ref_cell_coord(v0, o0, c0, n0);
+            const int32_t iy = ref_cell_coord(v1, o1, c1, n1);
+            const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
+            keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
+            vals[i] = (uint32_t)i;
+            const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
+            // bit 6: the cell lies on the grid's boundary (a neighbour may wrap,
+            // or not exist)
+            const bool edge = ix == 1 || ix == n0 || iy == 1 || iy == n1 || iz == 1 || iz == n2;
+            const uint32_t fb = face_bits((double)f0, o0 + (double)(ix - 1) * c0, c0, R, m) |
+                                (face_bits((double)f1, o1 + (double)(iy - 1) * c1, c1, R, m) << 2) |
+                                (face_bits((double)f2, o2 + (double)(iz - 1) * c2, c2, R, m) << 4) |
+                                (edge ? kEdgeBit : 0u);
+            xr[i] = make_float4(f0, f1, f2, __uint_as_float(fb));
+        }
     }
 }
 
