@@ -984,18 +984,19 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
 // dense-cell kernels take no pairs with mid cells). Only members within R = b + m
 // of the shared face, edge or corner can link, so both sides are filtered to
 // those first (the same margins as k_link's pruning) and the remaining pairs
-// are tested by all lanes. Against a dense B, where most such pairs link,
-// each staged member carries its union-find parent (in .w, which the
-// predicates do not read; k_cell_intra and k_dense_intra leave every member
-// pointing at its local root): a pair whose two parents this wave has just
-// joined is connected already and is not tested, and a passing pair joins
-// the two parents. k_link does nothing for A's particles.
+// are tested by all lanes. Each staged member carries its union-find parent
+// (in .w, which the predicates do not read; k_cell_intra and k_dense_intra
+// leave every member pointing at its local root), so most linked pairs of two
+// cells share one pair of parents: a pair with equal parents, or with the two
+// parents this wave has just joined, is connected already and is not tested,
+// and a passing pair joins the two parents, each distinct pair of a round
+// once. k_link does nothing for A's particles.
 template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
                           const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
-                          uint16_t* ia, uint32_t* ib, uint8_t* fa, uint2* uq) {
+                          uint16_t* ia, uint8_t* fa, uint2* uq) {
     const int lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
     // linked pairs, queued and joined one per lane once kWave are waiting (a
@@ -1085,47 +1086,30 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
         int32_t ja = -1, jb = -1;  // the parent pair this wave joined last
         auto flush = [&](uint32_t nb) {
             __syncthreads();
-            if (!denseB) {
-                for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
-                    const uint32_t t = t0 + (uint32_t)lane;
-                    bool hit = false;
-                    uint32_t i = 0, j = 0;
-                    if (t < na * nb) {
-                        i = t / nb;
-                        j = t - i * nb;
-                        hit = pair_linked_screened<PERIODIC>(pa[ia[i]], pb[j], P);
-                    }
-                    const uint64_t hits = __ballot(hit);
-                    if (hit) uq[nq + (uint32_t)__popcll(hits & below)] = make_uint2(sA + ia[i], sB + ib[j]);
-                    nq += (uint32_t)__popcll(hits);
-                    if (nq >= (uint32_t)kWave) drain(false);
+            for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
+                const uint32_t t = t0 + (uint32_t)lane;
+                int32_t xa = 0, xb = 0;
+                bool hit = false;
+                if (t < na * nb) {
+                    const uint32_t i = t / nb, j = t - i * nb;
+                    const float4 a = pa[ia[i]], q = pb[j];
+                    xa = __float_as_int(a.w);
+                    xb = __float_as_int(q.w);
+                    if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked_screened<PERIODIC>(a, q, P);
                 }
-            } else {
-                for (uint32_t t0 = 0; t0 < na * nb; t0 += kWave) {
-                    const uint32_t t = t0 + (uint32_t)lane;
-                    int32_t xa = 0, xb = 0;
-                    bool hit = false;
-                    if (t < na * nb) {
-                        const uint32_t i = t / nb, j = t - i * nb;
-                        const float4 a = pa[ia[i]], q = pb[j];
-                        xa = __float_as_int(a.w);
-                        xb = __float_as_int(q.w);
-                        if (xa != xb && !(xa == ja && xb == jb)) hit = pair_linked_screened<PERIODIC>(a, q, P);
-                    }
-                    // each distinct parent pair of the round is queued once
-                    uint64_t m = __ballot(hit);
-                    while (m) {
-                        const int l = __ffsll((long long)m) - 1;
-                        const int32_t la = __shfl(xa, l, kWave), lb = __shfl(xb, l, kWave);
-                        if (lane == 0) uq[nq] = make_uint2((uint32_t)la, (uint32_t)lb);
-                        ++nq;
-                        ja = la;
-                        jb = lb;
-                        hit = hit && !(xa == la && xb == lb);
-                        m = __ballot(hit);
-                    }
-                    if (nq >= (uint32_t)kWave) drain(false);
+                // each distinct parent pair of the round is queued once
+                uint64_t m = __ballot(hit);
+                while (m) {
+                    const int l = __ffsll((long long)m) - 1;
+                    const int32_t la = __shfl(xa, l, kWave), lb = __shfl(xb, l, kWave);
+                    if (lane == 0) uq[nq] = make_uint2((uint32_t)la, (uint32_t)lb);
+                    ++nq;
+                    ja = la;
+                    jb = lb;
+                    hit = hit && !(xa == la && xb == lb);
+                    m = __ballot(hit);
                 }
+                if (nq >= (uint32_t)kWave) drain(false);
             }
             __syncthreads();
         };
@@ -1142,8 +1126,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             }
             if (ok) {
                 const uint32_t k = nb + (uint32_t)__popcll(m & below);
-                ib[k] = i;
-                if (denseB) q.w = __int_as_float(load_near(&parent[sB + i]));
+                q.w = __int_as_float(load_near(&parent[sB + i]));
                 pb[k] = q;
             }
             nb += cnt;
@@ -1228,15 +1211,14 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      LinkParams P) {
     __shared__ float4 pa[kMidMax], pb[kWave];
     __shared__ uint16_t ia[kMidMax];
-    __shared__ uint32_t ib[kWave];
     __shared__ uint8_t fa[kMidMax];
     __shared__ uint2 uq[2 * kWave];
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, ib, fa, uq);
+        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, ib, fa, uq);
+        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
 }
 
 template <bool PERIODIC>
