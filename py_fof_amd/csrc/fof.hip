@@ -908,11 +908,12 @@ __device__ __forceinline__ void lds_unite(int32_t* lp, int32_t a, int32_t b) {
 // so joining them where they are found would run the union-find loops with
 // a lane or two of 64. The linked pairs are queued in LDS instead and joined
 // 64 at a time, one per lane.
-constexpr uint32_t kIntraQueue = 2 * kWave;
+constexpr uint32_t kIntraQueue = 3 * kWave;
 
 template <bool PERIODIC>
 __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const float4* __restrict__ xc,
-                           int32_t* parent, float4* pos, int32_t* lp, uint16_t* hq) {
+                           int32_t* parent, float4* __restrict__ pos, int32_t* __restrict__ lp,
+                           uint16_t* __restrict__ hq) {
     const int lane = __lane_id();
     const uint64_t below = lanemask_lt();
     for (uint32_t i = lane; i < n; i += kWave) {
@@ -922,36 +923,58 @@ __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const fl
     __syncthreads();
     // every unordered pair once, all lanes busy: pair t = (m - 1) n + k is
     // (k, (k + m) mod n) for m = 1 .. (n - 1) / 2 and every k, then, for even
-    // n, (k, k + n / 2) for k < n / 2
+    // n, (k, k + n / 2) for k < n / 2. Pair t + kWave is (k + r, m + q) with
+    // kWave = q n + r (one more m where k + r wraps).
     const uint32_t total = n * ((n - 1) / 2) + ((n & 1u) ? 0u : n / 2);
-    uint32_t k = (uint32_t)lane, m = 1;
-    while (k >= n) { k -= n; ++m; }
-    uint32_t nq = 0;  // queued pairs (wave-uniform)
-    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
-        bool hit = false;
-        uint32_t j = 0;
-        if (t0 + (uint32_t)lane < total) {
-            j = k + m;
-            if (j >= n) j -= n;
-            hit = pair_linked_screened<PERIODIC>(pos[k], pos[j], P);
+    const uint32_t q64 = (uint32_t)kWave / n, r64 = (uint32_t)kWave - q64 * n;
+    uint32_t k = (uint32_t)lane % n, m = 1 + (uint32_t)lane / n;
+    auto advance = [&](uint32_t& kk, uint32_t& mm) {
+        kk += r64;
+        mm += q64;
+        if (kk >= n) {
+            kk -= n;
+            ++mm;
         }
-        const uint64_t hits = __ballot(hit);
-        if (hit) hq[nq + (uint32_t)__popcll(hits & below)] = (uint16_t)(k | (j << 8));
-        nq += (uint32_t)__popcll(hits);
-        if (nq >= (uint32_t)kWave) {
-            // one queued pair per lane; the rest (fewer than kWave) moves to the front
+    };
+    uint32_t nq = 0;  // queued pairs (wave-uniform)
+    // two rounds of kWave pairs per trip, their LDS loads issued together
+    for (uint32_t t0 = 0; t0 < total; t0 += 2 * kWave) {
+        uint32_t k1 = k, m1 = m;
+        advance(k1, m1);
+        bool hit0 = false, hit1 = false;
+        uint32_t j0 = 0, j1 = 0;
+        if (t0 + (uint32_t)lane < total) {
+            j0 = k + m;
+            if (j0 >= n) j0 -= n;
+            hit0 = pair_linked_screened<PERIODIC>(pos[k], pos[j0], P);
+        }
+        if (t0 + (uint32_t)(kWave + lane) < total) {
+            j1 = k1 + m1;
+            if (j1 >= n) j1 -= n;
+            hit1 = pair_linked_screened<PERIODIC>(pos[k1], pos[j1], P);
+        }
+        const uint64_t h0 = __ballot(hit0), h1 = __ballot(hit1);
+        const uint32_t c0 = (uint32_t)__popcll(h0);
+        if (hit0) hq[nq + (uint32_t)__popcll(h0 & below)] = (uint16_t)(k | (j0 << 8));
+        if (hit1) hq[nq + c0 + (uint32_t)__popcll(h1 & below)] = (uint16_t)(k1 | (j1 << 8));
+        nq += c0 + (uint32_t)__popcll(h1);
+        while (nq >= (uint32_t)kWave) {
+            // one queued pair per lane; the rest moves to the front
             __syncthreads();
             const uint32_t e = hq[lane];
             const uint32_t rest = nq - (uint32_t)kWave;
-            const uint32_t moved = (uint32_t)lane < rest ? hq[kWave + lane] : 0u;
+            const uint32_t moved0 = (uint32_t)lane < rest ? hq[kWave + lane] : 0u;
+            const uint32_t moved1 = (uint32_t)(kWave + lane) < rest ? hq[2 * kWave + lane] : 0u;
             lds_unite(lp, (int32_t)(e & 0xffu), (int32_t)(e >> 8));
             __syncthreads();
-            if ((uint32_t)lane < rest) hq[lane] = (uint16_t)moved;
+            if ((uint32_t)lane < rest) hq[lane] = (uint16_t)moved0;
+            if ((uint32_t)(kWave + lane) < rest) hq[kWave + lane] = (uint16_t)moved1;
             nq = rest;
             __syncthreads();
         }
-        k += kWave;
-        while (k >= n) { k -= n; ++m; }
+        k = k1;
+        m = m1;
+        advance(k, m);
     }
     __syncthreads();
     if ((uint32_t)lane < nq) {
@@ -1113,10 +1136,11 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             }
             __syncthreads();
         };
-        // stages the lanes' candidates (member i of B at position q, ok when
-        // its face bits reach A), flushing full windows
+        // stages the lanes' candidates (a member of B at position q with
+        // parent pr, loaded together; ok when its face bits reach A),
+        // flushing full windows
         uint32_t nb = 0;
-        auto stage = [&](bool ok, float4 q, uint32_t i) {
+        auto stage = [&](bool ok, float4 q, int32_t pr) {
             ok = ok && (__float_as_uint(q.w) & needB) == needB;  // B's face bits toward A
             const uint64_t m = __ballot(ok);
             const uint32_t cnt = (uint32_t)__popcll(m);
@@ -1126,7 +1150,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             }
             if (ok) {
                 const uint32_t k = nb + (uint32_t)__popcll(m & below);
-                q.w = __int_as_float(load_near(&parent[sB + i]));
+                q.w = __int_as_float(pr);
                 pb[k] = q;
             }
             nb += cnt;
@@ -1135,7 +1159,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             for (uint32_t i0 = 0; i0 < nB; i0 += kWave) {
                 const uint32_t i = i0 + (uint32_t)lane;
                 const bool ok = i < nB;
-                stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), i);
+                stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), ok ? load_near(&parent[sB + i]) : 0);
             }
         } else {
             // A dense B holds its members by sub-cell: only the sub-cells
@@ -1191,7 +1215,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                     const uint32_t before = (uint32_t)__shfl((int)(incl - cnt), L, kWave);
                     const bool ok = oo < total;
                     const uint32_t i = ok ? sub_list[sB + first + (oo - before)] - sB : 0u;
-                    stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), i);
+                    stage(ok, ok ? xc[sB + i] : make_float4(0.f, 0.f, 0.f, 0.f), ok ? load_near(&parent[sB + i]) : 0);
                 }
             }
         }
