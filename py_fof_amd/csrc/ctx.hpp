@@ -118,9 +118,12 @@ struct Ctx {
     // further streams of the linking phase: the sparse-particle kernels run
     // on stream2 and the mid-cell kernels on stream3, beside the dense-cell
     // kernels on stream (fork after binning; ev_dense / ev_mid: the local
-    // components of dense / mid cells are written; joins before the relabel)
+    // components of dense / mid cells are written; joins before the relabel).
+    // In the binning, the coordinate gather runs on stream2 beside the cell
+    // starts and classes (ev_sort: keys sorted; ev_gather: gather done).
     hipStream_t stream2 = nullptr, stream3 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_dense = nullptr, ev_mid = nullptr, ev_join = nullptr, ev_pair = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_dense = nullptr, ev_mid = nullptr, ev_join = nullptr, ev_pair = nullptr,
+               ev_sort = nullptr, ev_gather = nullptr;
     hipEvent_t ev[kNumEv] = {};
     bool ev_recorded[kNumEv] = {};
     DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
@@ -210,14 +213,14 @@ struct Ctx {
         PYFOF_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
         PYFOF_HIP(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
         for (auto& e : ev) PYFOF_HIP(hipEventCreate(&e));
-        for (hipEvent_t* e : {&ev_fork, &ev_dense, &ev_mid, &ev_join, &ev_pair})
+        for (hipEvent_t* e : {&ev_fork, &ev_dense, &ev_mid, &ev_join, &ev_pair, &ev_sort, &ev_gather})
             PYFOF_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     ~Ctx() {
         (void)hipSetDevice(device);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         for (auto& e : fft_ev) if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {ev_fork, ev_dense, ev_mid, ev_join, ev_pair})
+        for (hipEvent_t e : {ev_fork, ev_dense, ev_mid, ev_join, ev_pair, ev_sort, ev_gather})
             if (e) (void)hipEventDestroy(e);
         if (stream3) (void)hipStreamDestroy(stream3);
         if (stream2) (void)hipStreamDestroy(stream2);

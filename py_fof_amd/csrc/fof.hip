@@ -2154,6 +2154,17 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s);
         uint32_t* skeys = alt ? k1 : k0;
         uint32_t* perm = alt ? v1 : v0;
+        // the gather of the float coordinates (random reads) runs on stream2
+        // beside the cell starts and classes (streaming); the dense-cell
+        // build, the first user of xc and parent, waits for it
+        hipStream_t s2 = C.stream2, s3 = C.stream3;
+        float4* xc = C.xc.as<float4>(N);
+        int32_t* parent = C.parent.as<int32_t>(N);
+        PYFOF_HIP(hipEventRecord(C.ev_sort, s));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_sort, 0));
+        k_gather<<<nblk(N), kBlk, 0, s2>>>(xr, perm, xc, parent, N);
+        PYFOF_LAUNCH_CHECK();
+        PYFOF_HIP(hipEventRecord(C.ev_gather, s2));
         uint32_t* scal = C.scal.as<uint32_t>(64);
         k_fill_u32<<<1, 64, 0, s>>>(scal, 64, 0u);
         PYFOF_LAUNCH_CHECK();
@@ -2162,10 +2173,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         k_cell_starts<<<nblk((uint64_t)N + 1), kBlk, 0, s>>>(skeys, N, ncell, cell_ptr, gaps, scal + 16);
         PYFOF_LAUNCH_CHECK();
         k_fill_gaps<<<1024, kBlk, 0, s>>>(gaps, scal + 16, cell_ptr);
-        PYFOF_LAUNCH_CHECK();
-        float4* xc = C.xc.as<float4>(N);
-        int32_t* parent = C.parent.as<int32_t>(N);
-        k_gather<<<nblk(N), kBlk, 0, s>>>(xr, perm, xc, parent, N);
         PYFOF_LAUNCH_CHECK();
         // dense cells and their sub-cell lists
         int32_t* aux = C.aux.as<int32_t>(ncell);
@@ -2183,6 +2190,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             k_mark_near_dense<<<nblk(ndense), kBlk, 0, s>>>(dense_cells, ndense, d_coff, d_reg, H, aux, A.L);
             PYFOF_LAUNCH_CHECK();
         }
+        PYFOF_HIP(hipStreamWaitEvent(s, C.ev_gather, 0));
         uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
         uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);
@@ -2212,7 +2220,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // k_dense_intra alone, beside k_cell_intra. k_cell_pair and
         // k_link_dense join mid or sparse members to dense ones and wait for
         // both intra-cell kernels.
-        hipStream_t s2 = C.stream2, s3 = C.stream3;
         // per-particle masks of dense cells to scan, in the buffer that holds
         // the labels after linking
         uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
@@ -2255,6 +2262,14 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
                                                 parent, P);
             PYFOF_LAUNCH_CHECK();
         }
+        // the relabel's zero fills (component sizes, halo ids) on s3 beside
+        // the other linking kernels: nothing in the linking touches them
+        uint32_t* size = C.size.as<uint32_t>(N);
+        int32_t* o_id = dev_mem ? A.halo_id : C.out_id.as<int32_t>(N);
+        k_fill_u32<<<nblk(N), kBlk, 0, s3>>>(size, N, 0u);
+        PYFOF_LAUNCH_CHECK();
+        k_fill_u32<<<nblk(N), kBlk, 0, s3>>>(reinterpret_cast<uint32_t*>(o_id), N, 0u);
+        PYFOF_LAUNCH_CHECK();
         PYFOF_HIP(hipEventRecord(C.ev_pair, s3));
         if (nsubs > 0) {
             k_dense_cell_cross<<<ndense, kWave, 0, s>>>(dense_cells, ndense, cell_ptr, d_coff, d_reg, H,
@@ -2272,23 +2287,17 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
 
         // ---- roots, sizes, candidate numbering, outputs
         int32_t* label = C.label.as<int32_t>(N);
-        uint32_t* size = C.size.as<uint32_t>(N);
         uint32_t* cidx = C.cidx.as<uint32_t>(N);
-        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(size, N, 0u);
-        PYFOF_LAUNCH_CHECK();
         k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size);
         PYFOF_LAUNCH_CHECK();
         // candidates (fof_fortran.f90:866-872): positions whose component
         // size (0 for non-roots) reaches np_min, numbered in position order
         exclusive_count_at_least_u32(size, A.np_min, cidx, N, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
-        int32_t* o_id = dev_mem ? A.halo_id : C.out_id.as<int32_t>(N);
         int32_t* o_root = nullptr;
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
-        k_fill_u32<<<nblk(N), kBlk, 0, s>>>(reinterpret_cast<uint32_t*>(o_id), N, 0u);
-        PYFOF_LAUNCH_CHECK();
         k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, skeys, d_coff, H, d_reg, N, A.np_min, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
