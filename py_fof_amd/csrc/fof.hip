@@ -1042,37 +1042,39 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     double lo[3];
     cell_lower_corner(g, lc, lo, ijk);
     const uint32_t sA = cell_ptr[c], nA = cell_ptr[c + 1] - sA;
-    // lane o < 27 looks up neighbour o: its cell, class and members, all in
-    // one round of loads. Pairs with another mid cell are taken once (forward
-    // cells), pairs with any other cell in both directions.
-    uint32_t n_s = 0, n_n = 0;
+    // lane o < 27 looks up neighbour o: its cell, class and member range, in
+    // one round of loads together with A's first kWave members and their
+    // parents. Pairs with another mid cell are taken once (forward cells),
+    // pairs with any other cell in both directions.
+    uint32_t n_s = 0, n_e = 0;
     int32_t n_cls = -1;
-    bool n_dense = false, n_work = false;
+    bool n_fwd = false;
     if (lane < 27 && lane != 13) {
         int32_t jijk[3];
         double sh[3];
         if (neighbour_cell(g, ijk, lane, jijk, sh, P.L)) {
             const uint32_t jl = local_cell(g, jijk);
             const uint32_t cB = g.cell_off + jl;
-            const int32_t clsB = aux[cB];
-            const bool dB = clsB >= 0;
-            if (!(jl <= lc && clsB == kAuxMid)) {
-                n_s = cell_ptr[cB];
-                n_n = cell_ptr[cB + 1] - n_s;
-                n_work = n_n > 0;
-                n_dense = dB;
-                n_cls = clsB;
-            }
+            n_cls = aux[cB];
+            n_s = cell_ptr[cB];
+            n_e = cell_ptr[cB + 1];
+            n_fwd = jl > lc;
         }
     }
     // A's members and their face bits: bit 2d when the member lies within
     // R = b + m of the lower face of axis d (with the 2m pruning margin),
     // bit 2d+1 for the upper face
+    uint32_t fa0 = 0;  // face bits of member `lane` (the first kWave members)
     for (uint32_t i = lane; i < nA; i += kWave) {
         const float4 a = xc[sA + i];
         pa[i] = make_float4(a.x, a.y, a.z, __int_as_float(load_near(&parent[sA + i])));
-        fa[i] = (uint8_t)__float_as_uint(a.w);  // face bits (k_keys)
+        const uint32_t f = __float_as_uint(a.w) & 0xffu;  // face bits (k_keys)
+        fa[i] = (uint8_t)f;
+        if (i < (uint32_t)kWave) fa0 = f;
     }
+    const uint32_t n_n = n_e - n_s;
+    const bool n_dense = n_cls >= 0;
+    const bool n_work = n_n > 0 && !(!n_fwd && n_cls == kAuxMid);
     __syncthreads();
     uint64_t work = __ballot(n_work);
     while (work) {
@@ -1093,9 +1095,16 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                 needB |= 1u << (2 * d);
             }
         }
-        // A's members that can reach B, in member order
-        uint32_t na = 0;
-        for (uint32_t i0 = 0; i0 < nA; i0 += kWave) {
+        // A's members that can reach B, in member order (the first kWave
+        // from registers)
+        uint32_t na;
+        {
+            const bool ok = (uint32_t)lane < nA && (fa0 & need) == need;
+            const uint64_t m = __ballot(ok);
+            if (ok) ia[(uint32_t)__popcll(m & below)] = (uint16_t)lane;
+            na = (uint32_t)__popcll(m);
+        }
+        for (uint32_t i0 = kWave; i0 < nA; i0 += kWave) {
             const uint32_t i = i0 + (uint32_t)lane;
             const bool ok = i < nA && (fa[i] & need) == need;
             const uint64_t m = __ballot(ok);
