@@ -1384,12 +1384,17 @@ __device__ void dense_intra(uint32_t d, const RegionDev& g, const LinkParams& P,
     int32_t* srow = sub_root + (size_t)d * P.sub_stride;
     for (int i = threadIdx.x; i < K; i += kBlk)
         if (rp[i + 1] == rp[i]) srow[i] = -1;
+    // Every member's parent is its sub-cell's head (smallest position) since
+    // k_dense_build: the heads take their component's root, then every
+    // position of the cell (cs .. cs + n_c, in order) its head's new parent.
     for (uint32_t i = threadIdx.x; i < na; i += kBlk) {
         const int32_t A = (int32_t)(list[i] & 0xfffu);
         const int32_t root = (int32_t)mh[lds_find(lp, A)];
         srow[A] = root;
-        for (uint32_t t = rp[A]; t < rp[A + 1]; ++t) parent[sub_list[cs + t]] = root;
+        parent[parent[sub_list[cs + rp[A]]]] = root;
     }
+    __syncthreads();
+    for (uint32_t p = cs + threadIdx.x; p < cs + n_c; p += kBlk) parent[p] = parent[parent[p]];
     if (threadIdx.x == 0) {
         ncomp[d] = (int32_t)nl[1];
         croot[d] = nl[1] == 1 ? (int32_t)mh[lds_find(lp, (int32_t)(list[0] & 0xfffu))] : -1;
