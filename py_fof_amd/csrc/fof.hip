@@ -1608,19 +1608,28 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
     for (int ax = 0; ax < 3; ++ax)
         if (sa[ax] < 2 || sa[ax] > kk[ax] - 3) interior = false;
     if (interior) return;
-    // per axis, for offsets -2..2: neighbour cell coordinate (or -1) and sub-coordinate
-    int32_t jt[3][5], st[3][5];
+    // Per axis, the window's offsets -2..2 (o = 0..4) fall into up to three
+    // parts by the coarse cell they reach (shift -1, 0, +1): the offsets
+    // [olo, ohi] of each part and that cell's coordinate (or -1). Cells are
+    // then taken one at a time, each checked once, and only their offsets
+    // visited.
+    int32_t olo[3][3], ohi[3][3], jc3[3][3];
     for (int ax = 0; ax < 3; ++ax) {
-        for (int o = 0; o < 5; ++o) {
-            int32_t f = sa[ax] + o - 2, j = ijk[ax];
-            if (f < 0) { f += kk[ax]; j -= 1; }
-            else if (f >= kk[ax]) { f -= kk[ax]; j += 1; }
+        const int32_t k = kk[ax], a = sa[ax];
+        // sub-coordinate f = a + o - 2: below 0, in [0, k), at k or above
+        olo[ax][0] = 0;
+        ohi[ax][0] = min(4, 1 - a);
+        olo[ax][1] = max(0, 2 - a);
+        ohi[ax][1] = min(4, k + 1 - a);
+        olo[ax][2] = max(0, k + 2 - a);
+        ohi[ax][2] = 4;
+        for (int part = 0; part < 3; ++part) {
+            int32_t j = ijk[ax] + part - 1;
             if (j < 0 || j >= g.n[ax]) {
                 if (g.periodic[ax]) j = j < 0 ? j + g.n[ax] : j - g.n[ax];
                 else j = -1;
             }
-            jt[ax][o] = j;
-            st[ax][o] = f;
+            jc3[ax][part] = j;
         }
     }
     const uint32_t* rowA = sub_ptr + (size_t)d * P.sub_stride;
@@ -1643,52 +1652,54 @@ __device__ __forceinline__ void dense_cross(const uint4 ent, const RegionDev& g,
             if (i == next) joined[i] = lb;
         next = next + 1 == kJoinedCache ? 0 : next + 1;
     };
-    // the window's offsets run through a few neighbour cells; the last
-    // one's class and component count are kept, so an offset into the same
-    // cell costs one load (its sub-cell's local root) instead of three
-    // dependent ones
-    uint32_t last_cB = 0xffffffffu;
-    int32_t last_dB = -1, last_nc = 0;
-    for (int oz = 0; oz < 5; ++oz) {
-        const int32_t jz = jt[2][oz];
-        if (jz < 0) continue;
-        for (int oy = 0; oy < 5; ++oy) {
-            const int32_t jy = jt[1][oy];
-            if (jy < 0) continue;
-            for (int ox = 0; ox < 5; ++ox) {
-                const int32_t jx = jt[0][ox];
-                if (jx < 0) continue;
+    // a neighbour cell's class and component count are loaded once for all
+    // the offsets into it; each offset then costs one load (its sub-cell's
+    // local root)
+    for (int pz = 0; pz < 3; ++pz) {
+        const int32_t jz = jc3[2][pz];
+        if (jz < 0 || olo[2][pz] > ohi[2][pz]) continue;
+        for (int py = 0; py < 3; ++py) {
+            const int32_t jy = jc3[1][py];
+            if (jy < 0 || olo[1][py] > ohi[1][py]) continue;
+            for (int px = 0; px < 3; ++px) {
+                const int32_t jx = jc3[0][px];
+                if (jx < 0 || olo[0][px] > ohi[0][px]) continue;
                 if (jx == ijk[0] && jy == ijk[1] && jz == ijk[2]) continue;  // same cell: intra
                 const uint32_t jl = (uint32_t)jx + (uint32_t)g.n[0] * ((uint32_t)jy + (uint32_t)g.n[1] * (uint32_t)jz);
                 if (jl <= lc) continue;
                 const uint32_t cB = g.cell_off + jl;
-                if (cB != last_cB) {
-                    last_cB = cB;
-                    last_dB = aux[cB];
-                    last_nc = last_dB >= 0 ? ncomp[last_dB] : 0;
-                }
-                const int32_t dB = last_dB;
-                if (dB < 0) continue;  // sparse cell: below
-                if (single && last_nc == 1) continue;  // decided by k_dense_cell_cross
-                const uint32_t sb = (uint32_t)st[0][ox] + (uint32_t)kk[0] * ((uint32_t)st[1][oy] + (uint32_t)kk[1] * (uint32_t)st[2][oz]);
-                const int32_t lb = sub_root[(size_t)dB * P.sub_stride + sb];
-                if (lb < 0) continue;  // empty sub-cell
-                bool known = false;
-#pragma unroll
-                for (int i = 0; i < kJoinedCache; ++i) known = known || joined[i] == lb;
-                if (known) continue;
-                const int32_t rB = dsu_find_near(parent, lb);
-                if (rB != rA) rA = dsu_find_near(parent, rA);
-                if (rB == rA) {
-                    remember(lb);
-                    continue;
-                }
+                const int32_t dB = aux[cB];
+                if (dB < 0) continue;  // sparse cell: k_link
+                if (single && ncomp[dB] == 1) continue;  // decided by k_dense_cell_cross
                 const uint32_t* rowB = sub_ptr + (size_t)dB * P.sub_stride;
+                const int32_t* rootB = sub_root + (size_t)dB * P.sub_stride;
                 const uint32_t bs = cell_ptr[cB];
-                const uint32_t b0 = bs + rowB[sb], b1 = bs + rowB[sb + 1];
-                if (cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P)) {
-                    rA = dsu_unite(parent, rA, rB);
-                    remember(lb);
+                for (int oz = olo[2][pz]; oz <= ohi[2][pz]; ++oz) {
+                    const int32_t fz = sa[2] + oz - 2 - (pz - 1) * kk[2];
+                    for (int oy = olo[1][py]; oy <= ohi[1][py]; ++oy) {
+                        const int32_t fy = sa[1] + oy - 2 - (py - 1) * kk[1];
+                        for (int ox = olo[0][px]; ox <= ohi[0][px]; ++ox) {
+                            const int32_t fx = sa[0] + ox - 2 - (px - 1) * kk[0];
+                            const uint32_t sb = (uint32_t)fx + (uint32_t)kk[0] * ((uint32_t)fy + (uint32_t)kk[1] * (uint32_t)fz);
+                            const int32_t lb = rootB[sb];
+                            if (lb < 0) continue;  // empty sub-cell
+                            bool known = false;
+#pragma unroll
+                            for (int i = 0; i < kJoinedCache; ++i) known = known || joined[i] == lb;
+                            if (known) continue;
+                            const int32_t rB = dsu_find_near(parent, lb);
+                            if (rB != rA) rA = dsu_find_near(parent, rA);
+                            if (rB == rA) {
+                                remember(lb);
+                                continue;
+                            }
+                            const uint32_t b0 = bs + rowB[sb], b1 = bs + rowB[sb + 1];
+                            if (cliques_linked<PERIODIC>(a0, a1, b0, b1, sub_list, xc, P)) {
+                                rA = dsu_unite(parent, rA, rB);
+                                remember(lb);
+                            }
+                        }
+                    }
                 }
             }
         }
