@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
 // Negative values read as "not dense" everywhere.
 constexpr int32_t kAuxMid = -2;
 constexpr int32_t kAuxNearDense = -3;
-constexpr uint32_t kMidMin = 6, kMidMax = 255;  // particles of a k_cell_intra cell
+constexpr uint32_t kMidMin = 7, kMidMax = 255;  // particles of a k_cell_intra cell
 constexpr int kMarkBlk = 1024;                    // threads of k_dense_mark
 
 constexpr int kMarkPer = 4;                       // consecutive cells per thread
