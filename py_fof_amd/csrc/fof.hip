@@ -1806,31 +1806,32 @@ __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __res
 }
 
 // in_halo / halo_id per particle (fof_fortran.f90:209-223); per candidate the
-// anchor particle (its first CRS member) and its region
-__global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ size,
-                        const uint32_t* __restrict__ cand_idx,
+// anchor particle (its first CRS member) and its region. hnum[q] is the
+// halo number (from 1) of a candidate position q, 0 for any other.
+__global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ hnum,
                         const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
                         const uint32_t* __restrict__ cell_off, int32_t nreg,
-                        const RegionDev* __restrict__ reg, uint32_t n, int32_t np_min,
+                        const RegionDev* __restrict__ reg, uint32_t n,
                         int32_t* __restrict__ halo_id,
                         int32_t* __restrict__ root_out, uint32_t* __restrict__ anchor,
                         int32_t* __restrict__ hregion) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int32_t r = label[p];
-    const bool cand = (int64_t)size[r] >= (int64_t)np_min;
+    const uint32_t h = hnum[r];
     const uint32_t i = perm[p];
     // the one scattered write per halo member (halo_id was zero-filled,
     // coalesced, before); in_halo follows from halo_id in a coalesced pass
     // (k_in_halo)
-    if (cand) halo_id[i] = (int32_t)cand_idx[r] + 1;
+    if (h) halo_id[i] = (int32_t)h;
     const uint32_t rg = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, skeys[p]);
     if (root_out) root_out[i] = reg[rg].skip ? -1 : (int32_t)perm[r];
     // candidates are the positions whose size (0 for non-roots) reaches
-    // np_min (fof_fortran.f90:866-872), numbered by cand_idx
-    if ((int64_t)size[p] >= (int64_t)np_min) {
-        anchor[cand_idx[p]] = i;
-        hregion[cand_idx[p]] = (int32_t)rg;
+    // np_min (fof_fortran.f90:866-872), numbered in position order
+    const uint32_t hp = hnum[p];
+    if (hp) {
+        anchor[hp - 1] = i;
+        hregion[hp - 1] = (int32_t)rg;
     }
 }
 
@@ -2216,6 +2217,37 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipStreamWaitEvent(s, C.ev_gather, 0));
+
+        // ---- linking. k_dense_intra and k_cell_intra write the parents of
+        // dense-clique and mid-cell members directly (local components), so
+        // every atomic union that may reach those members waits for them.
+        // Three streams: the sparse-particle kernels on s2, the mid-cell
+        // kernels on s3, the dense-cell kernels on s. k_link touches neither
+        // kind of member (it joins sparse particles, and the dense-cell
+        // members of regions without cliques, which no plain store writes),
+        // and k_cell_intra only mid-cell members, so both start once the
+        // cell classes and coordinates are in place, beside k_dense_build
+        // (which writes the parents of dense-clique members only). The
+        // dense-cell pair kernels join dense clique members only, whose
+        // union-find paths stay among dense members until k_cell_pair or
+        // k_link_dense joins another kind: they wait for k_dense_intra alone.
+        // k_cell_pair and k_link_dense join mid or sparse members to dense
+        // ones and wait for both intra-cell kernels.
+        // per-particle masks of dense cells to scan, in the buffer that holds
+        // the labels after linking
+        uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
+        PYFOF_HIP(hipEventRecord(C.ev_fork, s));
+        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
+        PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_fork, 0));
+        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, d_coff, parent, dmask, P);
+        PYFOF_LAUNCH_CHECK();
+        if (nmid > 0) {
+            k_cell_intra<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
+            PYFOF_LAUNCH_CHECK();
+        }
+        PYFOF_HIP(hipEventRecord(C.ev_mid, s3));
+
+        // ---- dense cells' sub-cell lists
         uint32_t* sub_ptr = C.sub_ptr.as<uint32_t>((size_t)std::max<uint32_t>(ndense, 1) * P.sub_stride);
         uint32_t* sub_list = C.sub_list.as<uint32_t>(N);
         uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);
@@ -2232,27 +2264,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         C.last_ncell = ncell;
         C.last_ndense = ndense;
 
-        // ---- linking. k_dense_intra and k_cell_intra write the parents of
-        // dense-clique and mid-cell members directly (local components), so
-        // every atomic union that may reach those members waits for them.
-        // Three streams: the sparse-particle kernels on s2, the mid-cell
-        // kernels on s3, the dense-cell kernels on s. k_link touches neither
-        // kind of member (it joins sparse particles, and the dense-cell
-        // members of regions without cliques, which no plain store writes),
-        // so it starts at once. The dense-cell pair kernels join dense clique
-        // members only, whose union-find paths stay among dense members until
-        // k_cell_pair or k_link_dense joins another kind: they wait for
-        // k_dense_intra alone, beside k_cell_intra. k_cell_pair and
-        // k_link_dense join mid or sparse members to dense ones and wait for
-        // both intra-cell kernels.
-        // per-particle masks of dense cells to scan, in the buffer that holds
-        // the labels after linking
-        uint32_t* dmask = reinterpret_cast<uint32_t*>(C.label.as<int32_t>(N));
-        PYFOF_HIP(hipEventRecord(C.ev_fork, s));
-        PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
-        PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_fork, 0));
-        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, d_coff, parent, dmask, P);
-        PYFOF_LAUNCH_CHECK();
         int32_t* ncomp = C.ncomp.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* croot = C.croot.as<int32_t>(std::max<uint32_t>(ndense, 1));
         int32_t* needs_sub = C.needs_sub.as<int32_t>(std::max<uint32_t>(ndense, 1));
@@ -2268,11 +2279,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipEventRecord(C.ev_dense, s));
-        if (nmid > 0) {
-            k_cell_intra<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
-            PYFOF_LAUNCH_CHECK();
-        }
-        PYFOF_HIP(hipEventRecord(C.ev_mid, s3));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_dense, 0));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_mid, 0));
         if (ndense > 0) {
@@ -2312,18 +2318,19 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
 
         // ---- roots, sizes, candidate numbering, outputs
         int32_t* label = C.label.as<int32_t>(N);
-        uint32_t* cidx = C.cidx.as<uint32_t>(N);
+        uint32_t* hnum = C.cidx.as<uint32_t>(N);
         k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size);
         PYFOF_LAUNCH_CHECK();
         // candidates (fof_fortran.f90:866-872): positions whose component
-        // size (0 for non-roots) reaches np_min, numbered in position order
-        exclusive_count_at_least_u32(size, A.np_min, cidx, N, scal + 1, tmp, s);
+        // size (0 for non-roots) reaches np_min, numbered from 1 in position
+        // order (0 for the other positions)
+        number_at_least_u32(size, A.np_min, hnum, N, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
         int32_t* o_root = nullptr;
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, size, cidx, perm, skeys, d_coff, H, d_reg, N, A.np_min, o_id,
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, hnum, perm, skeys, d_coff, H, d_reg, N, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
         k_in_halo<<<nblk(((uint64_t)N + 3) / 4), kBlk, 0, s>>>(o_id, N, o_in);
