@@ -278,15 +278,67 @@ __global__ void __launch_bounds__(kBlk) k_fill_gaps(const uint4* __restrict__ ga
     }
 }
 
-// x_crs in CRS order (fof_fortran.f90:608): the input-order copy permuted.
+// Longest run of a cell whose face mask k_gather ORs member by member past
+// the end of a wave; a longer cell gets every bit.
+constexpr uint32_t kFaceRun = 8;
+
+// x_crs in CRS order (fof_fortran.f90:608): the input-order copy permuted;
+// the initial forest parent[p] = p; and per non-empty cell the OR of its
+// members' six face bits (fmask; bit 2d: a member lies within R of the
+// cell's lower face of axis d, bit 2d + 1: of the upper face), so that a
+// neighbour cell with no member near the shared face can be skipped
+// unread. The wave holding a cell's first position writes its mask: the OR
+// over the wave's positions of that cell, extended past the wave's end for
+// a cell of up to kFaceRun particles.
 __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
                                                  const uint32_t* __restrict__ perm,
+                                                 const uint32_t* __restrict__ skeys,
                                                  float4* __restrict__ xc, int32_t* __restrict__ parent,
-                                                 uint32_t n) {
+                                                 uint8_t* __restrict__ fmask, uint32_t n) {
     const uint32_t p = blockIdx.x * kBlk + threadIdx.x;
-    if (p >= n) return;
-    xc[p] = xr[perm[p]];
-    parent[p] = (int32_t)p;
+    const int lane = __lane_id();
+    const bool valid = p < n;
+    uint32_t key = 0xffffffffu, fb = 0;
+    if (valid) {
+        const float4 x = xr[perm[p]];
+        key = skeys[p];
+        xc[p] = x;
+        parent[p] = (int32_t)p;
+        fb = __float_as_uint(x.w) & 63u;
+    }
+    // does a run of this cell start at p (the previous position is another
+    // cell)? (every lane takes part in the shuffle: a lane masked off would
+    // read as zero to the others)
+    const uint32_t kup = __shfl_up(key, 1, kWave);
+    const uint32_t kprev = lane == 0 ? (valid && p > 0 ? skeys[p - 1] : 0xffffffffu) : kup;
+    uint32_t head = valid && (p == 0 || kprev != key) ? 1u : 0u;
+    // segmented inclusive OR over the runs of equal keys (a run's lanes are
+    // consecutive), with whether the run's first position is in this wave
+    uint32_t acc = fb;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t ku = __shfl_up(key, off, kWave), au = __shfl_up(acc, off, kWave);
+        const uint32_t hu = __shfl_up(head, off, kWave);
+        if (lane >= off && ku == key) {
+            acc |= au;
+            head |= hu;
+        }
+    }
+    const uint32_t knext = __shfl_down(key, 1, kWave);
+    const bool tail = lane == kWave - 1 || knext != key;
+    if (!valid || !tail || !head) return;
+    if (lane == kWave - 1) {
+        // the cell may go on past the wave
+        uint32_t q = p + 1, extra = 0;
+        while (q < n && skeys[q] == key) {
+            if (++extra >= kFaceRun) {
+                acc = 63u;
+                break;
+            }
+            acc |= __float_as_uint(xr[perm[q]].w) & 63u;
+            ++q;
+        }
+    }
+    fmask[key] = (uint8_t)acc;
 }
 
 // marks cells with at least kDenseMin particles in regions with sub-cells
@@ -598,6 +650,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
                                                   uint32_t c, const LinkParams& P,
                                                   const uint32_t* __restrict__ cell_ptr,
                                                   const int32_t* __restrict__ aux,
+                                                  const uint8_t* __restrict__ fmask,
                                                   const float4* __restrict__ xc, int32_t* parent) {
     // a mid cell's pairs are all k_cell_intra's and k_cell_pair's
     const int32_t own_cls = aux[c];
@@ -623,7 +676,7 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
         const int32_t dy = (fb & 4u) ? -1 : ((fb & 8u) ? 1 : 0);
         const int32_t dz = (fb & 16u) ? -1 : ((fb & 32u) ? 1 : 0);
         int32_t cl[7];
-        uint32_t sb[7], eb[7];
+        uint32_t sb[7], eb[7], fm[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             const int ax = k + 1;  // axes crossed: bit 0 x, bit 1 y, bit 2 z
@@ -631,12 +684,13 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
             const bool reach = ((ax & 1) == 0 || dx != 0) && ((ax & 2) == 0 || dy != 0) && ((ax & 4) == 0 || dz != 0);
             const int o = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
             cl[k] = -1;
-            sb[k] = eb[k] = 0;
+            sb[k] = eb[k] = fm[k] = 0;
             if (reach && (o > 13 || near_dense)) {
                 const uint32_t jc = g.cell_off + (uint32_t)((int32_t)lc + ox + s1 * oy + s2 * oz);
                 cl[k] = aux[jc];
                 sb[k] = cell_ptr[jc];
                 eb[k] = cell_ptr[jc + 1];
+                fm[k] = fmask[jc];
             }
         }
         scan_members<PERIODIC>(me, p + 1, cell_ptr[c + 1], P, xc, L);
@@ -652,6 +706,11 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
                 continue;
             }
             if (o < 13) continue;  // a backward cell is visited for its sub-cells only
+            // a partner lies within R of the neighbour's faces toward this
+            // cell on every axis crossed; skip a cell with no such member
+            const uint32_t need = ((ax & 1) ? (dx < 0 ? 2u : 1u) : 0u) | ((ax & 2) ? (dy < 0 ? 8u : 4u) : 0u) |
+                                  ((ax & 4) ? (dz < 0 ? 32u : 16u) : 0u);
+            if ((fm[k] & need) != need) continue;
             scan_members<PERIODIC>(me, sb[k], eb[k], P, xc, L);
         }
         return L.dmask;
@@ -721,6 +780,7 @@ __global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
                                                const uint32_t* __restrict__ skeys,
                                                const uint32_t* __restrict__ cell_ptr,
                                                const int32_t* __restrict__ aux,
+                                               const uint8_t* __restrict__ fmask,
                                                const RegionDev* __restrict__ reg,
                                                const uint32_t* __restrict__ cell_off, int32_t* parent,
                                                uint32_t* __restrict__ dmask, LinkParams P) {
@@ -740,8 +800,8 @@ __global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
         const RegionDev& g = reg[r0];
         if (g.skip) break;
         if (g.clique && aux[c] >= 0) break;  // dense clique cells: k_dense_* kernels
-        mask = g.is_periodic ? link_particle<true>(p, me, g, c, P, cell_ptr, aux, xc, parent)
-                             : link_particle<false>(p, me, g, c, P, cell_ptr, aux, xc, parent);
+        mask = g.is_periodic ? link_particle<true>(p, me, g, c, P, cell_ptr, aux, fmask, xc, parent)
+                             : link_particle<false>(p, me, g, c, P, cell_ptr, aux, fmask, xc, parent);
     }
     dmask[p] = mask;
 }
@@ -1017,6 +1077,7 @@ __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict
 template <bool PERIODIC>
 __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
                           const uint32_t* __restrict__ cell_ptr, const int32_t* __restrict__ aux,
+                          const uint8_t* __restrict__ fmask,
                           const uint32_t* __restrict__ sub_ptr, const uint32_t* __restrict__ sub_list,
                           const float4* __restrict__ xc, int32_t* parent, float4* pa, float4* pb,
                           uint16_t* ia, uint8_t* fa, uint2* uq) {
@@ -1046,7 +1107,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     // one round of loads together with A's first kWave members and their
     // parents. Pairs with another mid cell are taken once (forward cells),
     // pairs with any other cell in both directions.
-    uint32_t n_s = 0, n_e = 0;
+    uint32_t n_s = 0, n_e = 0, n_fm = 0;
     int32_t n_cls = -1;
     bool n_fwd = false;
     if (lane < 27 && lane != 13) {
@@ -1058,6 +1119,7 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
             n_cls = aux[cB];
             n_s = cell_ptr[cB];
             n_e = cell_ptr[cB + 1];
+            n_fm = fmask[cB];
             n_fwd = jl > lc;
         }
     }
@@ -1074,7 +1136,17 @@ __device__ void cell_pair(uint32_t c, const RegionDev& g, const LinkParams& P,
     }
     const uint32_t n_n = n_e - n_s;
     const bool n_dense = n_cls >= 0;
-    const bool n_work = n_n > 0 && !(!n_fwd && n_cls == kAuxMid);
+    // B's members toward A: a cell none of whose members lies near the
+    // shared face(s) holds no partner (the cell's face mask, k_gather)
+    uint32_t n_needB = 0;
+    if (lane < 27) {
+        const int32_t off[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+        for (int d = 0; d < 3; ++d) {
+            if (off[d] < 0) n_needB |= 2u << (2 * d);
+            else if (off[d] > 0) n_needB |= 1u << (2 * d);
+        }
+    }
+    const bool n_work = n_n > 0 && !(!n_fwd && n_cls == kAuxMid) && (n_fm & n_needB) == n_needB;
     __syncthreads();
     uint64_t work = __ballot(n_work);
     while (work) {
@@ -1238,6 +1310,7 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ cell_off,
                                                      const RegionDev* __restrict__ reg, int32_t nreg,
                                                      const int32_t* __restrict__ aux,
+                                                     const uint8_t* __restrict__ fmask,
                                                      const uint32_t* __restrict__ sub_ptr,
                                                      const uint32_t* __restrict__ sub_list,
                                                      const float4* __restrict__ xc, int32_t* parent,
@@ -1249,9 +1322,9 @@ __global__ void __launch_bounds__(kWave) k_cell_pair(const uint32_t* __restrict_
     const uint32_t c = mid_cells[xcd_block(blockIdx.x, gridDim.x)];
     const RegionDev& g = reg[nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c)];
     if (g.is_periodic)
-        cell_pair<true>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
+        cell_pair<true>(c, g, P, cell_ptr, aux, fmask, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
     else
-        cell_pair<false>(c, g, P, cell_ptr, aux, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
+        cell_pair<false>(c, g, P, cell_ptr, aux, fmask, sub_ptr, sub_list, xc, parent, pa, pb, ia, fa, uq);
 }
 
 template <bool PERIODIC>
@@ -2188,7 +2261,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         int32_t* parent = C.parent.as<int32_t>(N);
         PYFOF_HIP(hipEventRecord(C.ev_sort, s));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_sort, 0));
-        k_gather<<<nblk(N), kBlk, 0, s2>>>(xr, perm, xc, parent, N);
+        uint8_t* fmask = C.buf("fmask").as<uint8_t>(ncell);
+        k_gather<<<nblk(N), kBlk, 0, s2>>>(xr, perm, skeys, xc, parent, fmask, N);
         PYFOF_LAUNCH_CHECK();
         PYFOF_HIP(hipEventRecord(C.ev_gather, s2));
         uint32_t* scal = C.scal.as<uint32_t>(64);
@@ -2239,7 +2313,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipEventRecord(C.ev_fork, s));
         PYFOF_HIP(hipStreamWaitEvent(s2, C.ev_fork, 0));
         PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_fork, 0));
-        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, d_reg, d_coff, parent, dmask, P);
+        k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, fmask, d_reg, d_coff, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         if (nmid > 0) {
             k_cell_intra<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
@@ -2289,8 +2363,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_HIP(hipEventRecord(C.ev_join, s2));
         PYFOF_HIP(hipStreamWaitEvent(s3, C.ev_dense, 0));
         if (nmid > 0) {
-            k_cell_pair<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, sub_ptr, sub_list, xc,
-                                                parent, P);
+            k_cell_pair<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, aux, fmask, sub_ptr, sub_list,
+                                                xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         // the relabel's zero fills (component sizes, halo ids) on s3 beside
