@@ -1816,18 +1816,21 @@ constexpr int kCountBatch = 16;
 // positions are followed to their roots with all the batch's loads of a step
 // issued together (most paths are a few steps after the path halving of
 // linking), and the labels written. Runs of equal labels are counted per
-// chunk, every member included (size was zero-filled), and the run that
-// reaches a chunk's last lane is carried into the next chunk instead of
-// being added at once, so a large component costs one atomic per wave
-// rather than per chunk (its counter is the contended one). A position past
-// n ends a run.
+// chunk (size was zero-filled), and the run that reaches a chunk's last lane
+// is carried into the next chunk instead of being added at once, so a large
+// component costs one atomic per wave rather than per chunk (its counter is
+// the contended one). A position past n ends a run. A root is the smallest
+// position of its component, so the run holding it starts there: the root
+// itself is counted by the root bitmap (bit p of roots: position p is a
+// root), not in size, and an isolated particle costs no atomic at all;
+// a component's size is size[r] + 1 at its root r.
 __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __restrict__ label, uint32_t n,
-                              uint32_t* __restrict__ size) {
+                              uint32_t* __restrict__ size, unsigned long long* __restrict__ roots) {
     const int lane = __lane_id();
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t base = wave * kWave * kCountChunks;
-    int32_t carry = -1;  // wave-uniform: the open run's label (>= 0) and count
-    uint32_t ccount = 0;
+    int32_t carry = -1;  // wave-uniform: the open run's label (>= 0), count
+    uint32_t ccount = 0, croot = 0;  // and whether it began at its root
     for (int k0 = 0; k0 < kCountChunks; k0 += kCountBatch) {
         int32_t x[kCountBatch];
 #pragma unroll
@@ -1854,6 +1857,8 @@ __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __res
             const bool valid = p < n;
             if (valid) label[p] = x[b];
             const int32_t lab = valid ? x[b] : -2 - lane;  // unique: never continues a run
+            const uint64_t rbits = __ballot(valid && x[b] == (int32_t)p);
+            if (lane == 0 && valid) roots[wave * kCountChunks + (uint64_t)k] = rbits;
             const int32_t prev = __shfl_up(lab, 1, kWave);
             const bool head = lane == 0 || prev != lab;
             const uint64_t heads = __ballot(head);
@@ -1861,19 +1866,22 @@ __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __res
             const int lasthead = 63 - __clzll((long long)heads);
             const bool last = k + 1 == kCountChunks;
             const bool merge0 = carry >= 0 && lab0 == carry;
-            if (carry >= 0 && !merge0 && lane == 0) atomicAdd(&size[carry], ccount);
-            uint32_t cnt = 0;
+            if (carry >= 0 && !merge0 && lane == 0 && ccount > croot) atomicAdd(&size[carry], ccount - croot);
+            uint32_t cnt = 0, rs = 0;
             if (valid && head) {
                 const uint64_t above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
                 const int next = above ? __ffsll((long long)above) - 1 : 64;
                 cnt = (uint32_t)(next - lane) + (lane == 0 && merge0 ? ccount : 0u);
-                if (last || lane != lasthead) atomicAdd(&size[lab], cnt);
+                rs = lane == 0 && merge0 ? croot : (lab == (int32_t)p ? 1u : 0u);
+                if ((last || lane != lasthead) && cnt > rs) atomicAdd(&size[lab], cnt - rs);
             }
             // the run at the last head stays open (its label is valid when >= 0)
             const int32_t labL = __shfl(lab, lasthead, kWave);
             const uint32_t cntL = __shfl(cnt, lasthead, kWave);
+            const uint32_t rsL = __shfl(rs, lasthead, kWave);
             carry = (!last && labL >= 0) ? labL : -1;
             ccount = carry >= 0 ? cntL : 0u;
+            croot = carry >= 0 ? rsL : 0u;
         }
     }
 }
@@ -1899,8 +1907,8 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     if (h) halo_id[i] = (int32_t)h;
     const uint32_t rg = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, skeys[p]);
     if (root_out) root_out[i] = reg[rg].skip ? -1 : (int32_t)perm[r];
-    // candidates are the positions whose size (0 for non-roots) reaches
-    // np_min (fof_fortran.f90:866-872), numbered in position order
+    // candidates are the positions whose component size (0 for non-roots)
+    // reaches np_min (fof_fortran.f90:866-872), numbered in position order
     const uint32_t hp = hnum[p];
     if (hp) {
         anchor[hp - 1] = i;
@@ -2393,12 +2401,18 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         // ---- roots, sizes, candidate numbering, outputs
         int32_t* label = C.label.as<int32_t>(N);
         uint32_t* hnum = C.cidx.as<uint32_t>(N);
-        k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size);
+        unsigned long long* roots = C.buf("roots").as<unsigned long long>(((uint64_t)N + kWave - 1) / kWave);
+        k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size,
+                                                                                              roots);
         PYFOF_LAUNCH_CHECK();
         // candidates (fof_fortran.f90:866-872): positions whose component
         // size (0 for non-roots) reaches np_min, numbered from 1 in position
         // order (0 for the other positions)
-        number_at_least_u32(size, A.np_min, hnum, N, scal + 1, tmp, s);
+        // (a component's size is size[r] + 1 at its root r, and size is 0 at
+        // every other position: for np_min >= 2 a position qualifies exactly
+        // when size >= np_min - 1, without the root bitmap)
+        if (A.np_min >= 2) number_at_least_u32(size, nullptr, (int64_t)A.np_min - 1, hnum, N, scal + 1, tmp, s);
+        else number_at_least_u32(size, roots, A.np_min, hnum, N, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
         int32_t* o_root = nullptr;
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);

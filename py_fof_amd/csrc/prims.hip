@@ -42,10 +42,14 @@ struct InPlain {
     const uint32_t* p;
     __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
 };
-struct InAtLeast {  // 1 where (int64) p[i] >= thr
+struct InAtLeast {  // 1 where (int64) p[i] + bit i of bits (if given) >= thr
     const uint32_t* p;
+    const unsigned long long* bits;
     int64_t thr;
-    __device__ uint32_t operator()(uint64_t i) const { return (int64_t)p[i] >= thr ? 1u : 0u; }
+    __device__ uint32_t operator()(uint64_t i) const {
+        const int64_t b = bits ? (int64_t)((bits[i >> 6] >> (i & 63)) & 1ull) : 0;
+        return (int64_t)p[i] + b >= thr ? 1u : 0u;
+    }
 };
 
 template <class Op>
@@ -330,9 +334,10 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t*
     scan_rec<OpSum>(InPlain{in}, out, n, total, static_cast<uint32_t*>(temp), 0, s);
 }
 
-void number_at_least_u32(const uint32_t* in, int64_t thr, uint32_t* out, uint64_t n, uint32_t* total,
-                         void* temp, hipStream_t s) {
-    scan_rec<OpSum, InAtLeast, true>(InAtLeast{in, thr}, out, n, total, static_cast<uint32_t*>(temp), 0, s);
+void number_at_least_u32(const uint32_t* in, const unsigned long long* bits, int64_t thr, uint32_t* out,
+                         uint64_t n, uint32_t* total, void* temp, hipStream_t s) {
+    scan_rec<OpSum, InAtLeast, true>(InAtLeast{in, bits, thr}, out, n, total, static_cast<uint32_t*>(temp), 0,
+                                     s);
 }
 
 void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp,

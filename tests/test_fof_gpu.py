@@ -103,6 +103,16 @@ def test_clumpy_small_periodic():
     _check(g, r)
 
 
+@pytest.mark.parametrize("np_min", [1, 2, 3])
+def test_clumpy_small_halo_minimum(np_min):
+    # the candidate test counts a root through the root bitmap for np_min 1
+    # and through size >= np_min - 1 from 2 on; isolated particles and
+    # components spanning many count chunks both occur here
+    xp = synth.clumpy_plummer(n=65536, L=40.0, n_clumps=80, seed=9)
+    g, r = _run_both([xp], 40.0, 0.2, np_min)
+    _check(g, r)
+
+
 def test_clumpy_plummer_262k_bit_exact():
     xp = synth.clumpy_plummer()  # 262,144 particles, L = 64, seed 0
     g, r = _run_both([xp], 64.0, 0.2, 10)
