@@ -140,7 +140,6 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
                                                const Chunk* __restrict__ chunks,
                                                const RegionDev* __restrict__ reg,
                                                uint32_t* __restrict__ keys,
-                                               uint32_t* __restrict__ vals,
                                                float4* __restrict__ xr, double L, double Lh, double R,
                                                double m) {
     const Chunk ch = chunks[blockIdx.x];
@@ -151,7 +150,6 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
     if (g.skip) {
         for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
             keys[i] = off;
-            vals[i] = (uint32_t)i;
             xr[i] = make_float4((float)rel_coord(xp[3 * (size_t)i], xa0, L, Lh),
                                 (float)rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh),
                                 (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), 0.f);
@@ -186,7 +184,6 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
             const int32_t iy = ref_cell_coord(v1, o1, c1, n1);
             const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
             keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
-            vals[i] = (uint32_t)i;
             const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
             // bit 6: the cell lies on the grid's boundary (a neighbour may wrap,
             // or not exist)
@@ -2254,11 +2251,11 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* v0 = C.vals0.as<uint32_t>(N);
         uint32_t* v1 = C.vals1.as<uint32_t>(N);
         float4* xr = C.buf("xrel").as<float4>(N);
-        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, v0, xr, A.L, P.Lh, P.R, P.m);
+        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, xr, A.L, P.Lh, P.R, P.m);
         PYFOF_LAUNCH_CHECK();
         const size_t tmp_bytes = std::max(radix_temp_bytes(N), scan_temp_bytes(ncell + 1));
         void* tmp = C.temp.get(std::max(tmp_bytes, scan_temp_bytes(N) + 64));
-        const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s);
+        const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s, true);
         uint32_t* skeys = alt ? k1 : k0;
         uint32_t* perm = alt ? v1 : v0;
         // the gather of the float coordinates (random reads) runs on stream2

@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __res
         const uint32_t before = wc[w][d];
         if (valid && r == 0) wc[w][d] = before + (uint32_t)__popcll(m);
         kk[j] = k;
-        vv[j] = valid ? vals[idx] : 0u;
+        vv[j] = valid ? (vals ? vals[idx] : (uint32_t)idx) : 0u;  // no vals: the input index
         rk[j] = before + r;
     }
     __syncthreads();
@@ -345,14 +345,31 @@ void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void*
     scan_rec<OpMax>(InPlain{in}, out, n, nullptr, static_cast<uint32_t*>(temp), 1, s);
 }
 
+namespace {
+
+__global__ void __launch_bounds__(kBlock) k_iota_u32(uint32_t* __restrict__ out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)i;
+}
+
+}  // namespace
+
 bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
-                      uint64_t n, int bits, void* temp, hipStream_t s) {
-    if (n <= 1 || bits <= 0) return false;
+                      uint64_t n, int bits, void* temp, hipStream_t s, bool vals_iota) {
+    if (n <= 1 || bits <= 0) {
+        if (vals_iota && n > 0) {
+            k_iota_u32<<<div_up(n, kBlock), kBlock, 0, s>>>(vals, n);
+            PYFOF_LAUNCH_CHECK();
+        }
+        return false;
+    }
     const uint32_t ntiles = div_up(n, kSTile);
     const uint64_t m_max = (uint64_t)ntiles << kMaxRadixBits;
     uint32_t* hist = static_cast<uint32_t*>(temp);
     uint32_t* stemp = hist + ((m_max * sizeof(uint32_t) + 255) / 256) * 256 / sizeof(uint32_t);
-    uint32_t *ik = keys, *iv = vals, *ok = alt_keys, *ov = alt_vals;
+    // with vals_iota the first pass reads no values: each pair's value is
+    // its input index
+    uint32_t *ik = keys, *iv = vals_iota ? nullptr : vals, *ok = alt_keys, *ov = alt_vals;
     bool in_alt = false;
     const int passes = (bits + kMaxRadixBits - 1) / kMaxRadixBits;
     // digits narrower than 4 bits only occur in a single pass (bits <= 3),
@@ -379,7 +396,7 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32
                 throw std::runtime_error("radix_sort_pairs: digit width out of range");
         }
         uint32_t* tk = ik; ik = ok; ok = tk;
-        uint32_t* tv = iv; iv = ov; ov = tv;
+        uint32_t* tv = iv ? iv : vals; iv = ov; ov = tv;
         in_alt = !in_alt;
     }
     return in_alt;
