@@ -770,10 +770,12 @@ __device__ __forceinline__ uint32_t link_particle(uint32_t p, const float4& me, 
 // One thread per particle outside dense clique cells: pairs with sparse cells
 // (own cell and forward neighbours), and the mask of dense cells to scan,
 // written for every particle (0 when none).
-// Occupancy targets (6 and 5 waves per SIMD) chosen by measurement on the
-// 134M-particle clump set: both kernels wait on chains of dependent loads and
-// DSU atomics, so more resident waves beat the few registers spilled.
-__global__ void __launch_bounds__(kBlk, 6) k_link(const float4* __restrict__ xc,
+// Occupancy target (5 waves per SIMD) chosen by measurement on the
+// 134M-particle clump set: the kernel waits on chains of dependent loads and
+// DSU atomics, but the face-mask loads of its neighbour cells need 92
+// registers, and spilling them (at 6 waves) wrote scratch lines to HBM and
+// cost more than the sixth wave gained.
+__global__ void __launch_bounds__(kBlk, 5) k_link(const float4* __restrict__ xc,
                                                const uint32_t* __restrict__ skeys,
                                                const uint32_t* __restrict__ cell_ptr,
                                                const int32_t* __restrict__ aux,
