@@ -127,7 +127,7 @@ struct Ctx {
     hipEvent_t ev[kNumEv] = {};
     bool ev_recorded[kNumEv] = {};
     DevBuf xp, vp, mp, chunks, regions, regstart, cell_off, bbox, keys0, keys1, vals0, vals1, ends,
-        xc, parent, label, size, flag, cidx, aux, dense, mid, sub_ptr, sub_list, subs, ncomp, croot, needs_sub, sub_root, temp, out_in, out_id,
+        xc, parent, label, size, flag, aux, dense, mid, sub_ptr, sub_list, subs, ncomp, croot, needs_sub, sub_root, temp, out_in, out_id,
         out_root, anchor, hregion, scal, cat_keys, cat_vals, cat_keys2, cat_vals2, hoff, am_h, am,
         pin;
     // last call's sizes and catalog
@@ -230,7 +230,7 @@ struct Ctx {
     template <class F>
     void for_each_buf(F f) {
         for (DevBuf* b : {&xp, &vp, &mp, &chunks, &regions, &regstart, &cell_off, &bbox, &keys0, &keys1, &vals0,
-                          &vals1, &ends, &xc, &parent, &label, &size, &flag, &cidx, &aux, &dense,
+                          &vals1, &ends, &xc, &parent, &label, &size, &flag, &aux, &dense,
                           &sub_ptr, &sub_list, &subs, &ncomp, &croot, &needs_sub, &sub_root, &temp, &out_in, &out_id, &out_root, &anchor,
                           &hregion, &scal, &cat_keys, &cat_vals, &cat_keys2, &cat_vals2, &hoff,
                           &am_h, &am, &pin, &rec_phi, &rec_idx})

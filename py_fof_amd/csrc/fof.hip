@@ -1885,10 +1885,41 @@ __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __res
     }
 }
 
+// Candidates (fof_fortran.f90:866-872): positions whose component size
+// (0 for non-roots) reaches np_min. A component's size is size[r] + 1 at its
+// root r (k_label_count) and size is 0 at every other position, so a position
+// qualifies when size + its root bit >= np_min; for np_min >= 2 that is
+// size >= np_min - 1 (roots == nullptr, thr = np_min - 1). One wave per 64
+// positions writes their candidate bits (cbits, bit q of word q / 64) and
+// count (ccnt, scanned afterwards into the count before each word).
+__global__ void k_cand_bits(const uint32_t* __restrict__ size, const unsigned long long* __restrict__ roots,
+                            int64_t thr, uint32_t n, unsigned long long* __restrict__ cbits,
+                            uint32_t* __restrict__ ccnt) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n;
+    int64_t v = valid ? (int64_t)size[p] : 0;
+    if (roots && valid) v += (int64_t)((roots[p >> 6] >> (p & 63)) & 1ull);
+    const uint64_t b = __ballot(valid && v >= thr);
+    if (__lane_id() == 0 && valid) {
+        cbits[p >> 6] = b;
+        ccnt[p >> 6] = (uint32_t)__popcll(b);
+    }
+}
+
+// Halo number (from 1, in position order) of a candidate position q, 0 for
+// any other: the candidates before q's word plus those below q in it.
+__device__ __forceinline__ uint32_t halo_number(const unsigned long long* __restrict__ cbits,
+                                                const uint32_t* __restrict__ cpre, uint32_t q) {
+    const unsigned long long w = cbits[q >> 6];
+    const unsigned long long bit = 1ull << (q & 63u);
+    if (!(w & bit)) return 0u;
+    return cpre[q >> 6] + (uint32_t)__popcll(w & (bit - 1ull)) + 1u;
+}
+
 // in_halo / halo_id per particle (fof_fortran.f90:209-223); per candidate the
-// anchor particle (its first CRS member) and its region. hnum[q] is the
-// halo number (from 1) of a candidate position q, 0 for any other.
-__global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __restrict__ hnum,
+// anchor particle (its first CRS member) and its region.
+__global__ void k_write(const int32_t* __restrict__ label, const unsigned long long* __restrict__ cbits,
+                        const uint32_t* __restrict__ cpre,
                         const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
                         const uint32_t* __restrict__ cell_off, int32_t nreg,
                         const RegionDev* __restrict__ reg, uint32_t n,
@@ -1898,7 +1929,7 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int32_t r = label[p];
-    const uint32_t h = hnum[r];
+    const uint32_t h = halo_number(cbits, cpre, (uint32_t)r);
     const uint32_t i = perm[p];
     // the one scattered write per halo member (halo_id was zero-filled,
     // coalesced, before); in_halo follows from halo_id in a coalesced pass
@@ -1908,7 +1939,7 @@ __global__ void k_write(const int32_t* __restrict__ label, const uint32_t* __res
     if (root_out) root_out[i] = reg[rg].skip ? -1 : (int32_t)perm[r];
     // candidates are the positions whose component size (0 for non-roots)
     // reaches np_min (fof_fortran.f90:866-872), numbered in position order
-    const uint32_t hp = hnum[p];
+    const uint32_t hp = halo_number(cbits, cpre, p);
     if (hp) {
         anchor[hp - 1] = i;
         hregion[hp - 1] = (int32_t)rg;
@@ -2399,25 +2430,26 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
 
         // ---- roots, sizes, candidate numbering, outputs
         int32_t* label = C.label.as<int32_t>(N);
-        uint32_t* hnum = C.cidx.as<uint32_t>(N);
         unsigned long long* roots = C.buf("roots").as<unsigned long long>(((uint64_t)N + kWave - 1) / kWave);
         k_label_count<<<nblk(((uint64_t)N + kCountChunks - 1) / kCountChunks), kBlk, 0, s>>>(parent, label, N, size,
                                                                                               roots);
         PYFOF_LAUNCH_CHECK();
-        // candidates (fof_fortran.f90:866-872): positions whose component
-        // size (0 for non-roots) reaches np_min, numbered from 1 in position
-        // order (0 for the other positions)
-        // (a component's size is size[r] + 1 at its root r, and size is 0 at
-        // every other position: for np_min >= 2 a position qualifies exactly
-        // when size >= np_min - 1, without the root bitmap)
-        if (A.np_min >= 2) number_at_least_u32(size, nullptr, (int64_t)A.np_min - 1, hnum, N, scal + 1, tmp, s);
-        else number_at_least_u32(size, roots, A.np_min, hnum, N, scal + 1, tmp, s);
+        // candidates (k_cand_bits), numbered from 1 in position order
+        const uint64_t nwords = ((uint64_t)N + kWave - 1) / kWave;
+        unsigned long long* cbits = C.buf("cbits").as<unsigned long long>(nwords);
+        uint32_t* cpre = C.buf("ccnt").as<uint32_t>(nwords);
+        if (A.np_min >= 2)
+            k_cand_bits<<<nblk(N), kBlk, 0, s>>>(size, nullptr, (int64_t)A.np_min - 1, N, cbits, cpre);
+        else
+            k_cand_bits<<<nblk(N), kBlk, 0, s>>>(size, roots, A.np_min, N, cbits, cpre);
+        PYFOF_LAUNCH_CHECK();
+        exclusive_scan_u32(cpre, cpre, nwords, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
         int32_t* o_root = nullptr;
         if (A.root_out) o_root = dev_mem ? A.root_out : C.out_root.as<int32_t>(N);
         uint32_t* anchor = C.anchor.as<uint32_t>(N);
         int32_t* hreg = C.hregion.as<int32_t>(N);
-        k_write<<<nblk(N), kBlk, 0, s>>>(label, hnum, perm, skeys, d_coff, H, d_reg, N, o_id,
+        k_write<<<nblk(N), kBlk, 0, s>>>(label, cbits, cpre, perm, skeys, d_coff, H, d_reg, N, o_id,
                                          o_root, anchor, hreg);
         PYFOF_LAUNCH_CHECK();
         k_in_halo<<<nblk(((uint64_t)N + 3) / 4), kBlk, 0, s>>>(o_id, N, o_in);

@@ -37,19 +37,10 @@ struct OpMax {
     static __device__ uint32_t f(uint32_t a, uint32_t b) { return a > b ? a : b; }
 };
 
-// Scan inputs: the array itself, or a 0/1 predicate on it.
+// Scan input: the array itself.
 struct InPlain {
     const uint32_t* p;
     __device__ uint32_t operator()(uint64_t i) const { return p[i]; }
-};
-struct InAtLeast {  // 1 where (int64) p[i] + bit i of bits (if given) >= thr
-    const uint32_t* p;
-    const unsigned long long* bits;
-    int64_t thr;
-    __device__ uint32_t operator()(uint64_t i) const {
-        const int64_t b = bits ? (int64_t)((bits[i >> 6] >> (i & 63)) & 1ull) : 0;
-        return (int64_t)p[i] + b >= thr ? 1u : 0u;
-    }
 };
 
 template <class Op>
@@ -93,10 +84,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const In in, uint64_t n,
 __device__ inline int pad_idx(int e) { return e + (e >> 4); }
 
 // Scans one tile per block. offsets[block] (exclusive prefix of the tiles
-// before it) is folded in; inclusive selects the inclusive form. kNumber
-// (exclusive sums of 0/1 inputs): an element of input 1 gets its exclusive
-// prefix + 1, an element of input 0 gets 0.
-template <class Op, class In, bool kNumber = false>
+// before it) is folded in; inclusive selects the inclusive form.
+template <class Op, class In>
 __global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out, uint64_t n,
                                                       const uint32_t* __restrict__ offsets,
                                                       uint32_t* __restrict__ total, int inclusive) {
@@ -111,12 +100,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out
         tile[pad_idx(e)] = i < n ? in(i) : Op::id();
     }
     __syncthreads();
-    uint32_t loc[kIpt], val[kIpt];
+    uint32_t loc[kIpt];
     uint32_t s = Op::id();
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
         const uint32_t v = tile[pad_idx(t * kIpt + j)];
-        val[j] = v;
         loc[j] = inclusive ? Op::f(s, v) : s;
         s = Op::f(s, v);
     }
@@ -129,10 +117,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const In in, uint32_t* out
     for (int k = 0; k < w; ++k) woff = Op::f(woff, wtot[k]);
     const uint32_t texcl = __lane_id() == 0 ? woff : Op::f(woff, excl_in_wave);
 #pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        const uint32_t pre = Op::f(texcl, loc[j]);
-        tile[pad_idx(t * kIpt + j)] = kNumber ? (val[j] ? pre + 1u : 0u) : pre;
-    }
+    for (int j = 0; j < kIpt; ++j) tile[pad_idx(t * kIpt + j)] = Op::f(texcl, loc[j]);
     if (total && blockIdx.x == gridDim.x - 1 && t == kBlock - 1) *total = Op::f(texcl, s);
     __syncthreads();
 #pragma unroll
@@ -294,13 +279,13 @@ size_t scan_levels_bytes(uint64_t n) {
     return b + 256;
 }
 
-template <class Op, class In, bool kNumber = false>
+template <class Op, class In>
 void scan_rec(const In in, uint32_t* out, uint64_t n, uint32_t* total, uint32_t* temp,
               int inclusive, hipStream_t s) {
     if (n == 0) return;
     const uint32_t nb = div_up(n, kTile);
     if (nb == 1) {
-        k_scan_down<Op, In, kNumber><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total, inclusive);
+        k_scan_down<Op, In><<<1, kBlock, 0, s>>>(in, out, n, nullptr, total, inclusive);
         PYFOF_LAUNCH_CHECK();
         return;
     }
@@ -309,7 +294,7 @@ void scan_rec(const In in, uint32_t* out, uint64_t n, uint32_t* total, uint32_t*
     k_scan_reduce<Op, In><<<nb, kBlock, 0, s>>>(in, n, partial);
     PYFOF_LAUNCH_CHECK();
     scan_rec<Op>(InPlain{partial}, partial, nb, nullptr, next, 0, s);
-    k_scan_down<Op, In, kNumber><<<nb, kBlock, 0, s>>>(in, out, n, partial, total, inclusive);
+    k_scan_down<Op, In><<<nb, kBlock, 0, s>>>(in, out, n, partial, total, inclusive);
     PYFOF_LAUNCH_CHECK();
 }
 
@@ -332,12 +317,6 @@ size_t radix_temp_bytes(uint64_t n) {
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* temp,
                         hipStream_t s) {
     scan_rec<OpSum>(InPlain{in}, out, n, total, static_cast<uint32_t*>(temp), 0, s);
-}
-
-void number_at_least_u32(const uint32_t* in, const unsigned long long* bits, int64_t thr, uint32_t* out,
-                         uint64_t n, uint32_t* total, void* temp, hipStream_t s) {
-    scan_rec<OpSum, InAtLeast, true>(InAtLeast{in, bits, thr}, out, n, total, static_cast<uint32_t*>(temp), 0,
-                                     s);
 }
 
 void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp,

@@ -14,14 +14,6 @@ size_t radix_temp_bytes(uint64_t n);
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total,
                         void* temp, hipStream_t s);
 
-// Numbers the elements with v(i) = (int64) in[i] + bit i of bits >= thr
-// from 1 in index order (bits: one bit per element, 64 per word, or null
-// for none): out[i] = 1 + #{j < i : v(j) >= thr} for those, 0 for the
-// others, i < n; *total (device) the full count. in and out must not alias.
-// temp as exclusive_scan_u32.
-void number_at_least_u32(const uint32_t* in, const unsigned long long* bits, int64_t thr, uint32_t* out,
-                         uint64_t n, uint32_t* total, void* temp, hipStream_t s);
-
 // out[i] = max(in[0..i]) (inclusive), i < n. in and out may alias.
 void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp,
                             hipStream_t s);
