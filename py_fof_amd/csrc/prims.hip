@@ -197,7 +197,11 @@ __global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __res
     uint32_t* wf = &wc[0][0];
     for (int i = t; i < kWaves * kRadix; i += kSBlock) wf[i] = 0;
     __syncthreads();
-    const uint64_t tile0 = (uint64_t)blockIdx.x * kSTile;
+    // consecutive tiles on one XCD: the two ends of a digit's run of output
+    // positions are written by neighbouring tiles, whose partial lines then
+    // meet in the same L2
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t tile0 = (uint64_t)tile * kSTile;
     const uint64_t base = tile0 + (uint64_t)w * (kRounds * kWave);
     const uint64_t lt = lanemask_lt();
     uint32_t kk[kRounds], vv[kRounds], rk[kRounds];
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(kSBlock) k_radix_scatter(const uint32_t* __res
                 acc += c;
             }
             tot = acc;
-            gstart[t] = hist[(uint64_t)t * ntiles + blockIdx.x];
+            gstart[t] = hist[(uint64_t)t * ntiles + tile];
         }
         const uint32_t inc = wave_inclusive_scan<OpSum>(tot);
         if (t < kRadix && __lane_id() == kWave - 1) wsum[w] = inc;
