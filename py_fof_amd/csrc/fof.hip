@@ -1889,20 +1889,30 @@ __global__ void k_label_count(const int32_t* __restrict__ parent, int32_t* __res
 // (0 for non-roots) reaches np_min. A component's size is size[r] + 1 at its
 // root r (k_label_count) and size is 0 at every other position, so a position
 // qualifies when size + its root bit >= np_min; for np_min >= 2 that is
-// size >= np_min - 1 (roots == nullptr, thr = np_min - 1). One wave per 64
-// positions writes their candidate bits (cbits, bit q of word q / 64) and
-// count (ccnt, scanned afterwards into the count before each word).
+// size >= np_min - 1 (roots == nullptr, thr = np_min - 1). Each wave takes
+// kCandWords words of 64 positions (their loads issued together) and writes
+// their candidate bits (cbits, bit q of word q / 64) and counts (ccnt,
+// scanned afterwards into the count before each word).
+constexpr int kCandWords = 4;
 __global__ void k_cand_bits(const uint32_t* __restrict__ size, const unsigned long long* __restrict__ roots,
                             int64_t thr, uint32_t n, unsigned long long* __restrict__ cbits,
                             uint32_t* __restrict__ ccnt) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = p < n;
-    int64_t v = valid ? (int64_t)size[p] : 0;
-    if (roots && valid) v += (int64_t)((roots[p >> 6] >> (p & 63)) & 1ull);
-    const uint64_t b = __ballot(valid && v >= thr);
-    if (__lane_id() == 0 && valid) {
-        cbits[p >> 6] = b;
-        ccnt[p >> 6] = (uint32_t)__popcll(b);
+    const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave * kCandWords;
+    const int lane = __lane_id();
+    int64_t v[kCandWords];
+#pragma unroll
+    for (int j = 0; j < kCandWords; ++j) {
+        const uint64_t p = (w0 + j) * kWave + (uint64_t)lane;
+        v[j] = p < n ? (int64_t)size[p] : INT64_MIN;
+        if (roots && p < n) v[j] += (int64_t)((roots[w0 + j] >> lane) & 1ull);
+    }
+#pragma unroll
+    for (int j = 0; j < kCandWords; ++j) {
+        const uint64_t b = __ballot(v[j] >= thr);
+        if (lane == 0 && (w0 + j) * kWave < n) {
+            cbits[w0 + j] = b;
+            ccnt[w0 + j] = (uint32_t)__popcll(b);
+        }
     }
 }
 
@@ -1910,10 +1920,11 @@ __global__ void k_cand_bits(const uint32_t* __restrict__ size, const unsigned lo
 // any other: the candidates before q's word plus those below q in it.
 __device__ __forceinline__ uint32_t halo_number(const unsigned long long* __restrict__ cbits,
                                                 const uint32_t* __restrict__ cpre, uint32_t q) {
+    // both loads issued together (the count is used only for a candidate)
     const unsigned long long w = cbits[q >> 6];
+    const uint32_t pre = cpre[q >> 6];
     const unsigned long long bit = 1ull << (q & 63u);
-    if (!(w & bit)) return 0u;
-    return cpre[q >> 6] + (uint32_t)__popcll(w & (bit - 1ull)) + 1u;
+    return (w & bit) ? pre + (uint32_t)__popcll(w & (bit - 1ull)) + 1u : 0u;
 }
 
 // in_halo / halo_id per particle (fof_fortran.f90:209-223); per candidate the
@@ -2438,10 +2449,11 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         const uint64_t nwords = ((uint64_t)N + kWave - 1) / kWave;
         unsigned long long* cbits = C.buf("cbits").as<unsigned long long>(nwords);
         uint32_t* cpre = C.buf("ccnt").as<uint32_t>(nwords);
+        const uint32_t cand_blocks = nblk((nwords + kCandWords - 1) / kCandWords * kWave);
         if (A.np_min >= 2)
-            k_cand_bits<<<nblk(N), kBlk, 0, s>>>(size, nullptr, (int64_t)A.np_min - 1, N, cbits, cpre);
+            k_cand_bits<<<cand_blocks, kBlk, 0, s>>>(size, nullptr, (int64_t)A.np_min - 1, N, cbits, cpre);
         else
-            k_cand_bits<<<nblk(N), kBlk, 0, s>>>(size, roots, A.np_min, N, cbits, cpre);
+            k_cand_bits<<<cand_blocks, kBlk, 0, s>>>(size, roots, A.np_min, N, cbits, cpre);
         PYFOF_LAUNCH_CHECK();
         exclusive_scan_u32(cpre, cpre, nwords, scal + 1, tmp, s);
         int32_t* o_in = dev_mem ? A.in_halo : C.out_in.as<int32_t>(N);
