@@ -1807,7 +1807,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
 // parents are loaded together.
 constexpr int kCountChunks = 128;
 
-constexpr int kCountBatch = 16;
+constexpr int kCountBatch = 4;
 
 // Labels and component sizes in one pass (after linking; nothing changes
 // the forest meanwhile). Each wave walks kCountChunks consecutive chunks of
