@@ -1647,7 +1647,7 @@ __global__ void __launch_bounds__(kWave) k_dense_cell_cross(const uint32_t* __re
         cell_cross<false>(d, g, P, dense_cells, cell_ptr, aux, ncomp, croot, needs_sub, sub_ptr, sub_list, xc, parent);
 }
 
-constexpr int kJoinedCache = 8;  // local roots remembered as joined in dense_cross
+constexpr int kJoinedCache = 6;  // local roots remembered as joined in dense_cross
 
 // One thread per non-empty clique sub-cell A of a cell with several local
 // components, or next to such a cell: offsets into forward dense coarse cells,
@@ -1805,7 +1805,7 @@ __global__ void __launch_bounds__(kBlk) k_dense_cross(const uint4* __restrict__ 
 
 // Chunks of kWave positions per wave of k_label_count, and chunks whose
 // parents are loaded together.
-constexpr int kCountChunks = 128;
+constexpr int kCountChunks = 256;
 
 constexpr int kCountBatch = 4;
 
