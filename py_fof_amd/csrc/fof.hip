@@ -275,18 +275,14 @@ __global__ void __launch_bounds__(kBlk) k_fill_gaps(const uint4* __restrict__ ga
     }
 }
 
-// Longest run of a cell whose face mask k_gather ORs member by member past
-// the end of a wave; a longer cell gets every bit.
-constexpr uint32_t kFaceRun = 8;
-
 // x_crs in CRS order (fof_fortran.f90:608): the input-order copy permuted;
 // the initial forest parent[p] = p; and per non-empty cell the OR of its
 // members' six face bits (fmask; bit 2d: a member lies within R of the
 // cell's lower face of axis d, bit 2d + 1: of the upper face), so that a
 // neighbour cell with no member near the shared face can be skipped
 // unread. The wave holding a cell's first position writes its mask: the OR
-// over the wave's positions of that cell, extended past the wave's end for
-// a cell of up to kFaceRun particles.
+// over the wave's positions of that cell, or every bit for a cell that goes
+// on past the wave's end (a conservative mask).
 __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
                                                  const uint32_t* __restrict__ perm,
                                                  const uint32_t* __restrict__ skeys,
@@ -323,18 +319,7 @@ __global__ void __launch_bounds__(kBlk) k_gather(const float4* __restrict__ xr,
     const uint32_t knext = __shfl_down(key, 1, kWave);
     const bool tail = lane == kWave - 1 || knext != key;
     if (!valid || !tail || !head) return;
-    if (lane == kWave - 1) {
-        // the cell may go on past the wave
-        uint32_t q = p + 1, extra = 0;
-        while (q < n && skeys[q] == key) {
-            if (++extra >= kFaceRun) {
-                acc = 63u;
-                break;
-            }
-            acc |= __float_as_uint(xr[perm[q]].w) & 63u;
-            ++q;
-        }
-    }
+    if (lane == kWave - 1 && p + 1 < n && skeys[p + 1] == key) acc = 63u;  // the cell goes on
     fmask[key] = (uint8_t)acc;
 }
 
