@@ -6,14 +6,17 @@
 //   k_keys         reference cell id, float coordinates and face bits of every
 //                  particle
 //   radix sort     stable sort by cell id = the reference's CRS order
-//   k_cell_starts  cell_ptr from the sorted keys
-//   k_gather       float CRS coordinates (the reference's x_crs)
-//   k_dense_*      sub-cells (side <= (b - m)/sqrt 3) inside dense cells
+//   k_cell_starts  cell_ptr from the sorted keys (beside k_gather)
+//   k_gather       float CRS coordinates (the reference's x_crs) and the
+//                  cells' face masks (second stream)
+//   k_dense_*      cell classes; sub-cells (side <= (b - m)/sqrt 3) inside
+//                  dense cells
 //   linking        k_link (sparse cells), k_cell_intra / k_cell_pair (mid
 //                  cells), k_dense_intra / k_dense_cell_cross / k_dense_cross
 //                  (dense cells), k_link_dense: reference predicate +
-//                  lock-free union-find
-//   k_label_count / scan / k_write   roots, sizes, numbering, outputs
+//                  lock-free union-find, on three streams
+//   k_label_count / k_cand_bits / k_write   roots, sizes, candidate
+//                  numbering, outputs
 //   catalog        halo members grouped by halo id (fof_with_info, ang. mom.)
 #include <math.h>
 #include <string.h>
