@@ -427,8 +427,11 @@ __global__ void __launch_bounds__(kBlk) k_dense_build(
     const float4* __restrict__ xc, uint32_t* __restrict__ sub_ptr, int32_t sub_stride,
     uint32_t* __restrict__ sub_list, int32_t* __restrict__ parent, uint4* __restrict__ subs,
     uint32_t* __restrict__ nsubs) {
-    __shared__ uint32_t cnt[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim + 1];
-    __shared__ uint32_t minp[kMaxSubPerDim * kMaxSubPerDim * kMaxSubPerDim];
+    // per sub-cell counts (then offsets) and smallest positions: sub_stride
+    // (the largest sub-cell count of any region, plus one) words each
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(dyn_lds);
+    uint32_t* minp = cnt + sub_stride;
     __shared__ uint32_t nonempty, base;
     const uint32_t c = dense_cells[blockIdx.x];
     const uint32_t r = nreg == 1 ? 0u : region_of_cell(cell_off, nreg, c);
@@ -2364,7 +2367,8 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint4* subs = C.subs.as<uint4>(ndense > 0 ? N : 1);
         uint32_t nsubs = 0;
         if (ndense > 0) {
-            k_dense_build<<<ndense, kBlk, 0, s>>>(dense_cells, cell_ptr, d_coff, d_reg, H, xc, sub_ptr,
+            k_dense_build<<<ndense, kBlk, 2 * (size_t)P.sub_stride * sizeof(uint32_t), s>>>(
+                dense_cells, cell_ptr, d_coff, d_reg, H, xc, sub_ptr,
                                                   P.sub_stride, sub_list, parent, subs, scal + 3);
             PYFOF_LAUNCH_CHECK();
             PYFOF_HIP(hipMemcpyAsync(&nsubs, scal + 3, 4, hipMemcpyDeviceToHost, s));
