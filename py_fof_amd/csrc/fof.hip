@@ -1035,6 +1035,17 @@ __device__ void cell_intra(uint32_t s, uint32_t n, const LinkParams& P, const fl
     for (uint32_t i = lane; i < n; i += kWave) parent[s + i] = (int32_t)s + lds_find(lp, (int32_t)i);
 }
 
+// k_cell_intra runs beside k_link, which ends the linking phase (13 ms there
+// against 3.9 ms alone on the 134M-particle clump set). At the occupancy its
+// own 5.5 KB of LDS allows (29 one-wave workgroups per CU) it holds CU slots
+// k_link needs; launched with 11 KiB of LDS in all (14 per CU), the call was
+// 0.15-0.2 ms faster in each of three paired runs (3 to 5 waves per SIMD
+// gained 0.0-0.28 ms in eight more). The padding is dynamic LDS the kernel
+// does not touch.
+constexpr size_t kIntraLdsStatic = sizeof(float4) * kMidMax + sizeof(int32_t) * kMidMax +
+                                   sizeof(uint16_t) * kIntraQueue;
+constexpr size_t kIntraLdsPad = 11264 - kIntraLdsStatic;
+
 __global__ void __launch_bounds__(kWave) k_cell_intra(const uint32_t* __restrict__ mid_cells,
                                                       const uint32_t* __restrict__ cell_ptr,
                                                       const uint32_t* __restrict__ cell_off,
@@ -2331,6 +2342,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipStreamWaitEvent(s, C.ev_gather, 0));
+        C.mark(kEvBin);
 
         // ---- linking. k_dense_intra and k_cell_intra write the parents of
         // dense-clique and mid-cell members directly (local components), so
@@ -2356,7 +2368,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         k_link<<<nblk(N), kBlk, 0, s2>>>(xc, skeys, cell_ptr, aux, fmask, d_reg, d_coff, parent, dmask, P);
         PYFOF_LAUNCH_CHECK();
         if (nmid > 0) {
-            k_cell_intra<<<nmid, kWave, 0, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
+            k_cell_intra<<<nmid, kWave, kIntraLdsPad, s3>>>(mid_cells, cell_ptr, d_coff, d_reg, H, xc, parent, P);
             PYFOF_LAUNCH_CHECK();
         }
         PYFOF_HIP(hipEventRecord(C.ev_mid, s3));
@@ -2374,7 +2386,6 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
             PYFOF_HIP(hipMemcpyAsync(&nsubs, scal + 3, 4, hipMemcpyDeviceToHost, s));
             PYFOF_HIP(hipStreamSynchronize(s));
         }
-        C.mark(kEvBin);
         C.last_n = N;
         C.last_ncell = ncell;
         C.last_ndense = ndense;
