@@ -133,32 +133,14 @@ __global__ void __launch_bounds__(kBlk) k_bbox(const double* __restrict__ xp,
     }
 }
 
-// reference cell id per particle (fof_fortran.f90:580-585, 0-based global id)
-// and, in input order, its x_crs = real(x_rel, 4) (fof_fortran.f90:608): the
-// one coalesced read of xp the binning makes after the bounding box; k_gather
-// then moves 16 B per particle instead of re-reading the 24 B of xp at random.
-// The fourth component holds the particle's face bits (face_bits) as an
-// integer; the linking kernels find a particle's region from its cell.
-__global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
-                                               const Chunk* __restrict__ chunks,
-                                               const RegionDev* __restrict__ reg,
-                                               uint32_t* __restrict__ keys,
-                                               float4* __restrict__ xr, double L, double Lh, double R,
-                                               double m) {
-    const Chunk ch = chunks[blockIdx.x];
-    const RegionDev& g = reg[ch.region];
+// k_keys' rows of a chunk of a region with cells: cell id, float coordinates
+// and face bits of each particle; count(i, key) sees every key written.
+template <class Count>
+__device__ __forceinline__ void keys_rows(const double* __restrict__ xp, const Chunk& ch, const RegionDev& g,
+                                          uint32_t* __restrict__ keys, float4* __restrict__ xr, double L,
+                                          double Lh, double R, double m, double xa0, double xa1, double xa2,
+                                          Count& count) {
     const uint32_t off = g.cell_off;
-    const size_t a = (size_t)g.start;
-    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
-    if (g.skip) {
-        for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
-            keys[i] = off;
-            xr[i] = make_float4((float)rel_coord(xp[3 * (size_t)i], xa0, L, Lh),
-                                (float)rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh),
-                                (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), 0.f);
-        }
-        return;
-    }
     const double o0 = g.origin[0], o1 = g.origin[1], o2 = g.origin[2];
     const double c0 = g.cell[0], c1 = g.cell[1], c2 = g.cell[2];
     const int32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
@@ -186,7 +168,9 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
             const int32_t ix = ref_cell_coord(v0, o0, c0, n0);
             const int32_t iy = ref_cell_coord(v1, o1, c1, n1);
             const int32_t iz = ref_cell_coord(v2, o2, c2, n2);
-            keys[i] = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
+            const uint32_t key = off + (uint32_t)(ix - 1) + (uint32_t)n0 * ((uint32_t)(iy - 1) + (uint32_t)n1 * (uint32_t)(iz - 1));
+            keys[i] = key;
+            count(i, key);
             const float f0 = (float)v0, f1 = (float)v1, f2 = (float)v2;
             // bit 6: the cell lies on the grid's boundary (a neighbour may wrap,
             // or not exist)
@@ -200,6 +184,61 @@ __global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
     }
 }
 
+// reference cell id per particle (fof_fortran.f90:580-585, 0-based global id)
+// and, in input order, its x_crs = real(x_rel, 4) (fof_fortran.f90:608): the
+// one coalesced read of xp the binning makes after the bounding box; k_gather
+// then moves 16 B per particle instead of re-reading the 24 B of xp at random.
+// The fourth component holds the particle's face bits (face_bits) as an
+// integer; the linking kernels find a particle's region from its cell.
+// With hist, the kernel also counts the keys' low digits (key & dmask) per
+// tile of the radix sort's first pass (radix_first_hist's layout, entries
+// zeroed before): a chunk (kKeyChunk positions) spans at most two tiles,
+// counted in LDS and added once per non-zero count, so the sort reads the
+// keys once less.
+constexpr int32_t kKeyChunk = 4096;
+static_assert(kKeyChunk <= (int32_t)radix_tile_keys, "a chunk spans at most two sort tiles");
+__global__ void __launch_bounds__(kBlk) k_keys(const double* __restrict__ xp,
+                                               const Chunk* __restrict__ chunks,
+                                               const RegionDev* __restrict__ reg,
+                                               uint32_t* __restrict__ keys,
+                                               float4* __restrict__ xr, double L, double Lh, double R,
+                                               double m, uint32_t* __restrict__ hist, uint32_t ntiles,
+                                               uint32_t dmask) {
+    __shared__ uint32_t th[2][256];
+    const Chunk ch = chunks[blockIdx.x];
+    const RegionDev& g = reg[ch.region];
+    const uint32_t off = g.cell_off;
+    const size_t a = (size_t)g.start;
+    const double xa0 = xp[3 * a], xa1 = xp[3 * a + 1], xa2 = xp[3 * a + 2];
+    const uint32_t t0 = (uint32_t)ch.begin / radix_tile_keys;
+    if (hist) {
+        for (int e = threadIdx.x; e < 2 * 256; e += kBlk) (&th[0][0])[e] = 0u;
+        __syncthreads();
+    }
+    auto count = [&](int32_t i, uint32_t key) {
+        if (hist) atomicAdd(&th[(uint32_t)i / radix_tile_keys - t0][key & dmask], 1u);
+    };
+    if (g.skip) {
+        for (int32_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlk) {
+            keys[i] = off;
+            count(i, off);
+            xr[i] = make_float4((float)rel_coord(xp[3 * (size_t)i], xa0, L, Lh),
+                                (float)rel_coord(xp[3 * (size_t)i + 1], xa1, L, Lh),
+                                (float)rel_coord(xp[3 * (size_t)i + 2], xa2, L, Lh), 0.f);
+        }
+    } else {
+        keys_rows(xp, ch, g, keys, xr, L, Lh, R, m, xa0, xa1, xa2, count);
+    }
+    if (hist) {
+        __syncthreads();
+        const uint32_t nd = dmask + 1u;
+        for (uint32_t e = threadIdx.x; e < 2u * nd; e += kBlk) {
+            const uint32_t tt = e / nd, d = e - tt * nd;
+            const uint32_t c = th[tt][d];
+            if (c) atomicAdd(&hist[(size_t)d * ntiles + t0 + tt], c);
+        }
+    }
+}
 // ends[c] = 1 + last position of cell c (0 for empty cells, zero-filled before)
 __global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t* __restrict__ ends) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2174,7 +2213,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
     }
 
     // regions and chunks (host)
-    constexpr int32_t kChunk = 4096;
+    constexpr int32_t kChunk = kKeyChunk;  // k_keys counts a chunk's keys in at most two sort tiles
     std::vector<int32_t> start(H);
     std::vector<Chunk> chunks;
     {
@@ -2303,11 +2342,22 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         uint32_t* v0 = C.vals0.as<uint32_t>(N);
         uint32_t* v1 = C.vals1.as<uint32_t>(N);
         float4* xr = C.buf("xrel").as<float4>(N);
-        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, xr, A.L, P.Lh, P.R, P.m);
-        PYFOF_LAUNCH_CHECK();
         const size_t tmp_bytes = std::max(radix_temp_bytes(N), scan_temp_bytes(ncell + 1));
         void* tmp = C.temp.get(std::max(tmp_bytes, scan_temp_bytes(N) + 64));
-        const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, bits_for(ncell - 1), tmp, s, true);
+        // the sort's first histogram counted by k_keys
+        const int kbits = bits_for(ncell - 1), kwidth = radix_first_width(kbits);
+        uint32_t* khist = nullptr;
+        uint32_t ktiles = 0;
+        if (N > 1 && kwidth > 0) {
+            ktiles = (uint32_t)((N + radix_tile_keys - 1) / radix_tile_keys);
+            khist = radix_first_hist(tmp);
+            k_fill_u32<<<nblk((uint64_t)ktiles << kwidth), kBlk, 0, s>>>(khist, (uint64_t)ktiles << kwidth, 0u);
+            PYFOF_LAUNCH_CHECK();
+        }
+        k_keys<<<nch, kBlk, 0, s>>>(xp, d_chunks, d_reg, k0, xr, A.L, P.Lh, P.R, P.m, khist, ktiles,
+                                    kwidth > 0 ? (1u << kwidth) - 1u : 0u);
+        PYFOF_LAUNCH_CHECK();
+        const bool alt = radix_sort_pairs(k0, v0, k1, v1, N, kbits, tmp, s, true, khist != nullptr);
         uint32_t* skeys = alt ? k1 : k0;
         uint32_t* perm = alt ? v1 : v0;
         // the gather of the float coordinates (random reads) runs on stream2

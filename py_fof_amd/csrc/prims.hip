@@ -337,8 +337,21 @@ __global__ void __launch_bounds__(kBlock) k_iota_u32(uint32_t* __restrict__ out,
 
 }  // namespace
 
+static_assert(radix_tile_keys == (uint32_t)kSTile, "tile of the radix passes");
+
+int radix_first_width(int bits) {
+    if (bits <= 0) return 0;
+    const int passes = (bits + kMaxRadixBits - 1) / kMaxRadixBits;
+    // digits narrower than 4 bits only occur in a single pass (bits <= 3),
+    // where the wider digit's extra bits are zero in every key
+    return std::max(4, (bits + passes - 1) / passes);
+}
+
+uint32_t* radix_first_hist(void* temp) { return static_cast<uint32_t*>(temp); }
+
 bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
-                      uint64_t n, int bits, void* temp, hipStream_t s, bool vals_iota) {
+                      uint64_t n, int bits, void* temp, hipStream_t s, bool vals_iota,
+                      bool first_hist_ready) {
     if (n <= 1 || bits <= 0) {
         if (vals_iota && n > 0) {
             k_iota_u32<<<div_up(n, kBlock), kBlock, 0, s>>>(vals, n);
@@ -354,17 +367,16 @@ bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32
     // its input index
     uint32_t *ik = keys, *iv = vals_iota ? nullptr : vals, *ok = alt_keys, *ov = alt_vals;
     bool in_alt = false;
-    const int passes = (bits + kMaxRadixBits - 1) / kMaxRadixBits;
-    // digits narrower than 4 bits only occur in a single pass (bits <= 3),
-    // where the wider digit's extra bits are zero in every key
-    const int width = std::max(4, (bits + passes - 1) / passes);
+    const int width = radix_first_width(bits);
     for (int shift = 0; shift < bits; shift += width) {
         const uint64_t m = (uint64_t)ntiles << width;
         switch (width) {
 #define PYFOF_RADIX_PASS(W)                                                                          \
     case W:                                                                                          \
-        k_radix_hist<W><<<ntiles, kSBlock, 0, s>>>(ik, n, shift, hist, ntiles);                      \
-        PYFOF_LAUNCH_CHECK();                                                                        \
+        if (shift > 0 || !first_hist_ready) {                                                        \
+            k_radix_hist<W><<<ntiles, kSBlock, 0, s>>>(ik, n, shift, hist, ntiles);                  \
+            PYFOF_LAUNCH_CHECK();                                                                    \
+        }                                                                                            \
         scan_rec<OpSum>(InPlain{hist}, hist, m, nullptr, stemp, 0, s);                               \
         k_radix_scatter<W><<<ntiles, kSBlock, 0, s>>>(ik, iv, ok, ov, n, shift, hist, ntiles);       \
         PYFOF_LAUNCH_CHECK();                                                                        \
