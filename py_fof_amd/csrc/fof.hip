@@ -4,7 +4,7 @@
 //   k_bbox         per-region bounding box of the wrapped coordinates
 //   (host)         reference cell geometry per region (fof_fortran.f90:518-570)
 //   k_keys         reference cell id, float coordinates and face bits of every
-//                  particle
+//                  particle, and the sort's first digit histogram
 //   radix sort     stable sort by cell id = the reference's CRS order
 //   k_cell_starts  cell_ptr from the sorted keys (beside k_gather)
 //   k_gather       float CRS coordinates (the reference's x_crs) and the
