@@ -250,11 +250,15 @@ __global__ void k_run_ends(const uint32_t* __restrict__ skeys, uint32_t n, uint3
 // cell_ptr[c] = the first CRS position whose cell is c or later, for
 // c = 0..ncell (cell_ptr[ncell] = n), straight from the sorted keys:
 // position p (0 <= p <= n) owns the entries key[p-1] + 1 .. key[p] (from 0
-// for p = 0, up to ncell for p = n), which are p. A wave's 64 positions own
+// for p = 0, up to ncell for p = n), which are p. Each lane reads four
+// consecutive positions (one 16-byte load), so a wave's 256 positions own
 // one contiguous range of entries; up to kWaveSpan of them are written by
 // all lanes in turn, each entry's owner found by a binary search over the
-// lanes' last entries. In a wave with a longer range each lane writes its own
-// run of up to kGapInline entries, and a longer run (empty space) is cut into
+// lanes' last entries and then among the owner lane's four. (With one
+// position per lane the kernel had four times the waves and, beside the
+// gather, finished 2 ms later, holding back k_dense_mark.) In a wave with a
+// longer range each lane writes the runs of its positions, up to kGapInline
+// entries each, and a longer run (empty space) is cut into
 // records of kGapChunk entries that k_fill_gaps writes with whole
 // workgroups. Records needed: at most (ncell + 1) / kGapInline.
 constexpr uint32_t kGapInline = 64, kGapChunk = 4096, kWaveSpan = 1024;
@@ -264,46 +268,71 @@ __host__ __device__ inline uint64_t gap_records(uint64_t ncell) { return (ncell 
 __global__ void k_cell_starts(const uint32_t* __restrict__ skeys, uint32_t n, uint64_t ncell,
                               uint32_t* __restrict__ cell_ptr, uint4* __restrict__ gaps,
                               uint32_t* __restrict__ ngaps) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four consecutive positions per lane, 256 per wave
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = __lane_id();
-    // last entry owned by p; positions past n own nothing (ncell again);
-    // ncell < 2^32 - 1, so every entry index fits 32 bits
-    const uint32_t hi32 = p < n ? skeys[p] : (uint32_t)ncell;
-    const uint64_t p0 = p - (uint64_t)lane;
+    const uint64_t q0 = 4 * t;
+    const uint64_t p0 = q0 - 4 * (uint64_t)lane;  // the wave's first position
     if (p0 > n) return;  // a wave past position n owns nothing
+    // last entry owned by each position; positions past n own nothing
+    // (ncell again); ncell < 2^32 - 1, so every entry index fits 32 bits
+    uint32_t h[4];
+    if (q0 + 3 < n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(skeys + q0);
+        h[0] = v.x;
+        h[1] = v.y;
+        h[2] = v.z;
+        h[3] = v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) h[k] = q0 + k < n ? skeys[q0 + k] : (uint32_t)ncell;
+    }
     const uint64_t first = p0 == 0 ? 0 : (uint64_t)skeys[p0 - 1] + 1;  // wave-uniform
-    const uint64_t last = (uint64_t)__shfl(hi32, kWave - 1, kWave);
+    const uint64_t last = (uint64_t)__shfl(h[3], kWave - 1, kWave);
     if (first > last) return;  // the whole wave is in one cell
     if (last - first + 1 <= kWaveSpan) {
-        // a wave-uniform trip count: the shuffles read every lane's hi32,
+        // a wave-uniform trip count: the shuffles read every lane's values,
         // which a lane that had left the loop would no longer provide
         for (uint64_t base = first; base <= last; base += kWave) {
             const uint64_t e = base + (uint64_t)lane;
-            // owner: the first lane whose last entry is e or later
+            // owner: the first position whose last entry is e or later; the
+            // lane first (by its fourth position), then within the lane
             int j = 0;
             for (int step = kWave / 2; step > 0; step >>= 1) {
-                const uint32_t h = (uint32_t)__shfl(hi32, j + step - 1, kWave);
-                if ((uint64_t)h < e) j += step;
+                const uint32_t hv = (uint32_t)__shfl(h[3], j + step - 1, kWave);
+                if ((uint64_t)hv < e) j += step;
             }
-            if (e <= last) cell_ptr[e] = (uint32_t)(p0 + (uint64_t)j);
+            const uint32_t a0 = (uint32_t)__shfl(h[0], j, kWave);
+            const uint32_t a1 = (uint32_t)__shfl(h[1], j, kWave);
+            const uint32_t a2 = (uint32_t)__shfl(h[2], j, kWave);
+            const uint32_t k = (uint64_t)a0 >= e ? 0u : ((uint64_t)a1 >= e ? 1u : ((uint64_t)a2 >= e ? 2u : 3u));
+            if (e <= last) cell_ptr[e] = (uint32_t)(p0 + 4 * (uint64_t)j + k);
         }
         return;
     }
-    if (p > n) return;
-    const uint64_t lo = p == 0 ? 0 : (uint64_t)skeys[p - 1] + 1;
-    const uint64_t hi = hi32;
-    if (lo > hi) return;  // same cell as position p - 1
-    const uint64_t len = hi - lo + 1;
-    if (len <= kGapInline) {
-        for (uint64_t c = lo; c <= hi; ++c) cell_ptr[c] = (uint32_t)p;
-        return;
-    }
-    const uint32_t nch = (uint32_t)((len + kGapChunk - 1) / kGapChunk);
-    const uint32_t at = atomicAdd(ngaps, nch);
-    for (uint32_t k = 0; k < nch; ++k) {
-        const uint64_t a = lo + (uint64_t)k * kGapChunk;
-        const uint64_t b = a + kGapChunk < hi + 1 ? a + kGapChunk : hi + 1;
-        gaps[at + k] = make_uint4((uint32_t)a, (uint32_t)b, (uint32_t)p, 0u);
+    // the lane's positions in turn: the entry before each is the previous
+    // position's last (the previous lane's fourth, for the first)
+    const uint32_t up = (uint32_t)__shfl_up(h[3], 1, kWave);
+    uint64_t prev_hi = lane == 0 ? first - 1 : (uint64_t)up;  // first - 1 wraps only for p0 == 0
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const uint64_t p = q0 + (uint64_t)kk;
+        const uint64_t lo = p == 0 ? 0 : prev_hi + 1;
+        const uint64_t hi = h[kk];
+        prev_hi = hi;
+        if (p > n || lo > hi) continue;  // past n; or the same cell as position p - 1
+        const uint64_t len = hi - lo + 1;
+        if (len <= kGapInline) {
+            for (uint64_t c = lo; c <= hi; ++c) cell_ptr[c] = (uint32_t)p;
+            continue;
+        }
+        const uint32_t nch = (uint32_t)((len + kGapChunk - 1) / kGapChunk);
+        const uint32_t at = atomicAdd(ngaps, nch);
+        for (uint32_t k = 0; k < nch; ++k) {
+            const uint64_t a = lo + (uint64_t)k * kGapChunk;
+            const uint64_t b = a + kGapChunk < hi + 1 ? a + kGapChunk : hi + 1;
+            gaps[at + k] = make_uint4((uint32_t)a, (uint32_t)b, (uint32_t)p, 0u);
+        }
     }
 }
 
@@ -2400,7 +2429,7 @@ static int32_t run_fof(Ctx& C, const CallArgs& A) {
         PYFOF_LAUNCH_CHECK();
         uint32_t* cell_ptr = C.ends.as<uint32_t>(ncell + 1);
         uint4* gaps = C.buf("gaps").as<uint4>(gap_records(ncell));
-        k_cell_starts<<<nblk((uint64_t)N + 1), kBlk, 0, s>>>(skeys, N, ncell, cell_ptr, gaps, scal + 16);
+        k_cell_starts<<<nblk(((uint64_t)N + 1 + 3) / 4), kBlk, 0, s>>>(skeys, N, ncell, cell_ptr, gaps, scal + 16);
         PYFOF_LAUNCH_CHECK();
         k_fill_gaps<<<1024, kBlk, 0, s>>>(gaps, scal + 16, cell_ptr);
         PYFOF_LAUNCH_CHECK();
